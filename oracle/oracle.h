@@ -1,0 +1,178 @@
+/*
+ * CPU oracle for the HYOWON particle-mesh gravity step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product in
+ * hyowon_amd/: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it.  Nothing in the product links, loads or calls it.
+ *
+ * It restates, in plain C (binary64, -ffp-contract=off, OpenMP over independent
+ * planes/lines), the reference's pure-PM force refresh and kick-drift-kick
+ * update.  Every function cites the reference file:line it follows (paths are
+ * relative to the reference source tree).
+ *
+ * Pinning (what makes each function trustworthy; see tests/test_oracle_*.py):
+ *   or_wrap / or_minimum_image / or_cic_coordinate : reference build (oracle/_ref)
+ *       on boundary cases + brute force invariants.
+ *   or_xplane_order / or_bucket_order  : brute-force stable sort (numpy argsort
+ *       kind="stable") on the same keys; bit-exact.
+ *   or_deposit        : reference CICMassAssignment::deposit via oracle/_ref,
+ *       bit-exact; single-particle closed form (8 weights).
+ *   or_density_contrast : reference DensityNormalization via oracle/_ref.
+ *   or_fft_r2c_3d / or_fft_c2r_3d : numpy.fft.rfftn/irfftn (pocketfft) and an
+ *       O(N^4) long-double DFT on tiny meshes.
+ *   or_green_apply    : reference GreenFunction::apply via oracle/_ref, bit-exact.
+ *   or_gather_centered_gradient_add : reference
+ *       interpolate_centered_gradient3_add_full_mesh via oracle/_ref, bit-exact;
+ *       plane-wave closed form.
+ *   or_kick_drift / or_kick : reference particle_update.hpp via oracle/_ref.
+ *   or_drift_factor / or_kick_factor : reference DriftKickIntegrator via
+ *       oracle/_ref (bit-exact) and Einstein-de Sitter closed forms.
+ *   or_time_boundaries / or_geometric_mean : reference TimeStepper via oracle/_ref.
+ *   or_pm_force       : composition of the above; checked against the _ref
+ *       composition (reference deposit/Green/gather + this FFT).
+ */
+#ifndef HYOWON_ORACLE_H
+#define HYOWON_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    OR_OK = 0,
+    OR_EINVAL = 1,    /* std::invalid_argument in the reference */
+    OR_ELOGIC = 2,    /* std::logic_error */
+    OR_EOVERFLOW = 3, /* std::overflow_error */
+    OR_EUNDERFLOW = 4,/* std::underflow_error */
+    OR_ERUNTIME = 6   /* std::runtime_error */
+};
+
+/* Physical constants restated from include/cosmo_nbody/cosmology/units.hpp:14-30. */
+double or_gravitational_constant(void);
+double or_rho_crit0(void);
+
+/* include/cosmo_nbody/math/periodic_box.hpp:27-41 and :44-61. */
+double or_wrap(double x, double L);
+double or_minimum_image(double d, double L);
+
+/* src/mesh/cic_coordinate.hpp:95-121 (CICCoordinateAxis::operator()).
+ * upper = nextafter((double)N, 0).  Returns OR_OK or OR_ELOGIC. */
+int or_cic_coordinate(double position, double L, double dx, uint64_t N,
+                      int64_t* base, double* fraction);
+
+/* src/mesh/mass_assignment_workspace.inc:264-374 (build_x_plane_bins):
+ * stable counting sort of particle indices by wrapped base x-cell.
+ * offsets has N+1 entries, order has n entries. */
+int or_xplane_order(uint64_t n, const double* x, double L, uint64_t N,
+                    uint64_t* offsets, uint32_t* order);
+
+/* Deposit bucket key used by the HIP deposit (no reference counterpart; the
+ * reference sorts by x-plane only): tile edge T divides N, tile = base cell / T
+ * per axis, key = ((tx*nT+ty)*nT+tz)*8 + class, class bit2/1/0 set when the
+ * base cell is the last cell of its tile along x/y/z. */
+uint64_t or_bucket_tile_edge(uint64_t N);
+int or_bucket_order(uint64_t n, const double* x, const double* y, const double* z,
+                    double L, uint64_t N, uint64_t T,
+                    uint64_t* offsets /* nbuckets+1 */, uint32_t* order);
+
+/* src/mesh/mass_assignment_deposit.inc:15-334 (deposit_impl), serial mesh,
+ * reference per-cell accumulation order.  masses==NULL selects uniform_mass.
+ * mesh has N^3 entries, row-major (ix*N+iy)*N+iz. */
+int or_deposit(uint64_t n, const double* x, const double* y, const double* z,
+               const double* masses, double uniform_mass, double L, uint64_t N,
+               double* mesh);
+
+/* Exact (single-rounding) sum of non-negative doubles; restates the contract of
+ * include/cosmo_nbody/math/exact_positive_sum.hpp (value rounded once, ties to
+ * even).  Returns NaN for a negative or non-finite term. */
+double or_exact_sum(const double* values, uint64_t count);
+double or_exact_repeated(double value, uint64_t count);
+
+/* src/math/cic_mass_conservation.cpp:25-63.  Returns OR_OK or OR_ERUNTIME. */
+int or_require_cic_mass_conservation(double particle_mass, double deposited_mass,
+                                     uint64_t particle_count);
+
+/* src/gravity/pm_solver.cpp:700-759 with
+ * include/cosmo_nbody/math/density_normalization.hpp:32-120.
+ * Converts a deposited mass mesh to (m - M/N_c)/dx^3 in place. */
+int or_density_contrast(double* mesh, uint64_t N, double dx, double total_mass);
+
+/* FFT conventions of src/mesh/fft_backend.cpp:1088-1145: forward r2c
+ * unnormalized; inverse c2r multiplied by 1/((N*N)*N) in a separate pass.
+ * Complex layout is interleaved re,im with index iz+(N/2+1)*(iy+N*ix)
+ * (include/cosmo_nbody/mesh/mesh_geometry.hpp:90-96). */
+int or_fft_r2c_3d(uint64_t N, const double* real, double* cplx);
+int or_fft_c2r_3d(uint64_t N, const double* cplx, double* real);
+/* 1-D complex DFT of length n (sign -1 forward, +1 backward), in place. */
+int or_fft_1d(uint64_t n, double* data, int sign);
+
+/* src/mesh/green_function.cpp:15-135 (tables), :165-207 (evaluate_kernel),
+ * :209-241 (apply); pure PM (discrete Laplacian), optional CIC deconvolution. */
+int or_green_apply(uint64_t N, double L, int deconvolve_cic, double* cplx);
+double or_green_kernel(uint64_t N, double L, int deconvolve_cic,
+                       uint64_t ix, uint64_t iy, uint64_t iz);
+
+/* src/mesh/mass_assignment_gradient.cpp:46-226: fused centered-gradient CIC
+ * gather; adds into ax/ay/az. */
+int or_gather_centered_gradient_add(uint64_t n, const double* phi,
+                                    const double* x, const double* y, const double* z,
+                                    double L, uint64_t N,
+                                    double* ax, double* ay, double* az);
+
+/* src/gravity/pm_solver.cpp:569-852 (compute_forces_in_place, pure PM, serial
+ * mesh): deposit, exact mass identity and conservation check, density source,
+ * r2c, Green, c2r, centered-gradient gather.  Adds into ax/ay/az.
+ * phi_out (optional, N^3) receives the real-space potential. */
+int or_pm_force(uint64_t n, const double* x, const double* y, const double* z,
+                const double* masses, double uniform_mass,
+                double L, uint64_t N, int deconvolve_cic,
+                double* ax, double* ay, double* az, double* phi_out);
+
+/* include/cosmo_nbody/time/particle_update.hpp:20-116 and
+ * src/time/leapfrog_integrator.cpp:143-279: K1 kick + periodic drift
+ * (preflight then publish), and K2 kick. */
+int or_kick_drift(uint64_t n, double* x, double* y, double* z,
+                  double* px, double* py, double* pz,
+                  const double* ax, const double* ay, const double* az,
+                  double K, double D, double L);
+int or_kick(uint64_t n, double* px, double* py, double* pz,
+            const double* ax, const double* ay, const double* az, double K);
+
+/* src/cosmology/cosmology_model.cpp:25-57,167-178 (H(a) for flat matter+Lambda)
+ * and src/cosmology/drift_kick_factors.cpp:28-141 with
+ * include/cosmo_nbody/math/roundoff_resolved_gauss_legendre_16.hpp. */
+double or_hubble(double a, double omega_m, double omega_lambda);
+int or_drift_factor(double a1, double a2, double omega_m, double omega_lambda, double* out);
+int or_kick_factor(double a1, double a2, double omega_m, double omega_lambda, double* out);
+
+/* include/cosmo_nbody/math/exact_geometric_mean.hpp:168-275 (correctly rounded
+ * sqrt(lhs*rhs)). */
+double or_geometric_mean(double lhs, double rhs);
+
+/* src/time/time_stepper.cpp:194-276: boundary table. Writes at most cap
+ * boundaries, returns the count through *count. */
+int or_time_boundaries(double a_start, double a_final, double delta_ln_a,
+                       const double* stops, uint64_t n_stops,
+                       double* out, uint64_t cap, uint64_t* count);
+
+/* Full KDK evolution (src/runtime/simulation_runner.cpp:1348-1527 step loop
+ * with src/time/leapfrog_integrator.cpp:143-279), pure PM, diagnostics off:
+ * initial force at boundaries[0], then per step K1+drift, force at a_end, K2.
+ * Accelerations are left as the last force. */
+int or_run(uint64_t n, double* x, double* y, double* z,
+           double* px, double* py, double* pz, double* ax, double* ay, double* az,
+           const double* masses, double uniform_mass, double L, uint64_t N,
+           int deconvolve_cic, double omega_m, double omega_lambda,
+           const double* boundaries, uint64_t n_boundaries);
+
+/* Thread count used by OpenMP regions (0 = runtime default). */
+void or_set_threads(int threads);
+int or_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

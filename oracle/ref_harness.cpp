@@ -1,0 +1,261 @@
+// Test-only C entry points over the reference's own hot-path classes.
+//
+// TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles this file together with
+// the reference translation units listed there (straight from the reference
+// source tree, unmodified) into oracle/_ref/libhyref.so.  It exists to pin the
+// C oracle (oracle/oracle.c) against the reference implementation on the same
+// inputs.  It is never linked into the product.
+//
+// Every entry point returns 0 on success, or a code naming the exception type
+// the reference threw: 1 invalid_argument, 2 logic_error, 3 overflow_error,
+// 4 underflow_error, 6 runtime_error, 7 other.
+#include "cic_coordinate.hpp"
+
+#include "cosmo_nbody/core/particle_store.hpp"
+#include "cosmo_nbody/cosmology/cosmology_model.hpp"
+#include "cosmo_nbody/cosmology/drift_kick_factors.hpp"
+#include "cosmo_nbody/cosmology/units.hpp"
+#include "cosmo_nbody/math/cic_mass_conservation.hpp"
+#include "cosmo_nbody/math/density_normalization.hpp"
+#include "cosmo_nbody/math/exact_geometric_mean.hpp"
+#include "cosmo_nbody/math/exact_positive_sum.hpp"
+#include "cosmo_nbody/math/periodic_box.hpp"
+#include "cosmo_nbody/mesh/green_function.hpp"
+#include "cosmo_nbody/mesh/mass_assignment.hpp"
+#include "cosmo_nbody/mesh/mesh_geometry.hpp"
+#include "cosmo_nbody/mesh/pm_force_method.hpp"
+#include "cosmo_nbody/time/leapfrog_integrator.hpp"
+#include "cosmo_nbody/time/particle_update.hpp"
+#include "cosmo_nbody/time/time_stepper.hpp"
+
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <optional>
+#include <span>
+#include <stdexcept>
+#include <vector>
+
+namespace cn = cosmo_nbody;
+
+namespace {
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::invalid_argument&) {
+        return 1;
+    } catch (const std::underflow_error&) {
+        return 4;
+    } catch (const std::overflow_error&) {
+        return 3;
+    } catch (const std::logic_error&) {
+        return 2;
+    } catch (const std::runtime_error&) {
+        return 6;
+    } catch (...) {
+        return 7;
+    }
+}
+
+cn::config::CosmologyParams cosmology(double om, double ol) {
+    cn::config::CosmologyParams p;
+    p.h = 0.7;
+    p.omega_m = om;
+    p.omega_lambda = ol;
+    p.omega_b = 0.05;
+    p.sigma8 = 0.8;
+    p.n_s = 0.96;
+    return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+double ref_gravitational_constant() { return cn::cosmology::units::G; }
+double ref_rho_crit0() { return cn::cosmology::units::rho_crit0; }
+
+double ref_wrap(double x, double L) { return cn::math::wrap(x, L); }
+double ref_minimum_image(double d, double L) { return cn::math::minimum_image(d, L); }
+
+int ref_cic_coordinate(double position, double L, uint64_t N, int64_t* base, double* fraction) {
+    return guarded([&] {
+        const cn::mesh::MeshGeometry geom(L, N);
+        const auto c = cn::mesh::detail::cic_coordinate_1d(position, L, geom.cell_size(), N);
+        *base = c.base;
+        *fraction = c.fraction;
+    });
+}
+
+int ref_deposit(uint64_t n, const double* x, const double* y, const double* z,
+                const double* masses, double uniform_mass, double L, uint64_t N,
+                double* mesh) {
+    return guarded([&] {
+        const cn::mesh::MeshGeometry geom(L, N);
+        cn::mesh::CICMassAssignment cic(geom);
+        cn::mesh::RealField field(geom.padded_real_size());
+        std::optional<double> uniform;
+        std::span<const double> m;
+        if (masses) m = std::span<const double>(masses, n);
+        else uniform = uniform_mass;
+        cic.deposit(std::span<const double>(x, n), std::span<const double>(y, n),
+                    std::span<const double>(z, n), m, uniform, field);
+        std::memcpy(mesh, field.data(), geom.real_size() * sizeof(double));
+    });
+}
+
+// Density source exactly as src/gravity/pm_solver.cpp:700-759.
+int ref_density_contrast(double* mesh, uint64_t N, double dx, double total_mass) {
+    return guarded([&] {
+        const std::size_t cells = N * N * N;
+        const cn::math::DensityNormalization dn(total_mass, cells, dx, "ref density");
+        if (dn.direct_available()) {
+            const double mean = dn.mean_mass_per_cell();
+            const double inv = dn.inverse_cell_volume();
+            for (std::size_t i = 0; i < cells; ++i) mesh[i] = (mesh[i] - mean) * inv;
+        } else {
+            for (std::size_t i = 0; i < cells; ++i) mesh[i] = dn.normalize(mesh[i], "ref density");
+        }
+    });
+}
+
+int ref_density_direct_available(uint64_t N, double dx, double total_mass, int* direct) {
+    return guarded([&] {
+        const cn::math::DensityNormalization dn(total_mass, N * N * N, dx, "ref density");
+        *direct = dn.direct_available() ? 1 : 0;
+    });
+}
+
+int ref_green_apply(uint64_t N, double L, int deconvolve, double* cplx) {
+    return guarded([&] {
+        const cn::mesh::MeshGeometry geom(L, N);
+        const cn::mesh::GreenFunction green(geom, cn::mesh::PMForceMethod::pure_pm(deconvolve != 0));
+        cn::mesh::ComplexField field(reinterpret_cast<std::complex<double>*>(cplx),
+                                     geom.complex_size(), cn::mesh::external_mesh_storage);
+        green.apply(field);
+    });
+}
+
+int ref_gather(uint64_t n, const double* phi, const double* x, const double* y, const double* z,
+               double L, uint64_t N, double* ax, double* ay, double* az) {
+    return guarded([&] {
+        const cn::mesh::MeshGeometry geom(L, N);
+        cn::mesh::CICMassAssignment cic(geom);
+        cn::mesh::RealField field(const_cast<double*>(phi), geom.real_size(),
+                                  cn::mesh::external_mesh_storage);
+        cic.interpolate_centered_gradient3_add_full_mesh(
+            field, std::span<const double>(x, n), std::span<const double>(y, n),
+            std::span<const double>(z, n), std::span<double>(ax, n),
+            std::span<double>(ay, n), std::span<double>(az, n));
+    });
+}
+
+int ref_mass_conservation(double particle_mass, double deposited_mass, uint64_t count) {
+    return guarded([&] { cn::math::require_cic_mass_conservation(particle_mass, deposited_mass, count); });
+}
+
+double ref_exact_sum(const double* v, uint64_t n) {
+    cn::math::ExactPositiveDoubleSum s;
+    for (uint64_t i = 0; i < n; ++i) s.add(v[i]);
+    return s.value();
+}
+
+double ref_exact_repeated(double v, uint64_t n) {
+    cn::math::ExactPositiveDoubleSum s;
+    s.add_repeated(v, n);
+    return s.value();
+}
+
+// One particle through particle_update.hpp (K1 kick + periodic drift).
+int ref_kick_drift_one(double* pos, double* mom, const double* acc, double K, double D, double L) {
+    cn::time::KickDriftCandidate c;
+    const bool ok = cn::time::finite_kick_drift_candidate(
+        pos[0], pos[1], pos[2], mom[0], mom[1], mom[2], acc[0], acc[1], acc[2], K, D, L, c);
+    if (!ok) return 3;
+    pos[0] = c.position_x; pos[1] = c.position_y; pos[2] = c.position_z;
+    mom[0] = c.momentum_x; mom[1] = c.momentum_y; mom[2] = c.momentum_z;
+    return 0;
+}
+
+double ref_hubble(double a, double om, double ol) {
+    const cn::cosmology::CosmologyModel model(cosmology(om, ol));
+    return model.H(a);
+}
+
+int ref_drift_kick(double a1, double a2, double om, double ol, double* drift, double* kick) {
+    return guarded([&] {
+        const cn::cosmology::CosmologyModel model(cosmology(om, ol));
+        cn::cosmology::DriftKickIntegrator dk(model);
+        *drift = dk.drift_factor(a1, a2);
+        *kick = dk.kick_factor(a1, a2);
+    });
+}
+
+double ref_geometric_mean(double a, double b) { return cn::math::exact_geometric_mean(a, b); }
+
+int ref_time_boundaries(double a_start, double a_final, double dlna, const double* stops,
+                        uint64_t n_stops, double* out, uint64_t cap, uint64_t* count) {
+    return guarded([&] {
+        cn::time::TimeStepper ts(a_start, a_final, dlna,
+                                 std::vector<double>(stops, stops + n_stops));
+        const auto& b = ts.boundaries();
+        if (b.size() > cap) throw std::overflow_error("capacity");
+        std::memcpy(out, b.data(), b.size() * sizeof(double));
+        *count = b.size();
+    });
+}
+
+// Reference LeapfrogIntegrator + TimeStepper + DriftKickIntegrator driving a
+// caller-supplied force (C callback adds g into ax/ay/az, which the integrator
+// has zeroed).  Arrays are updated in place; n_steps_out reports the steps run.
+typedef int (*ref_force_cb)(uint64_t n, const double* x, const double* y, const double* z,
+                            double* ax, double* ay, double* az, double a, void* user);
+
+int ref_leapfrog_run(uint64_t n, double* x, double* y, double* z, double* px, double* py,
+                     double* pz, double* ax, double* ay, double* az, double uniform_mass,
+                     double L, double om, double ol, double a_start, double a_final,
+                     double dlna, ref_force_cb cb, void* user, uint64_t* n_steps_out) {
+    return guarded([&] {
+        cn::core::ParticleStore ps;
+        ps.set_uniform_mass(uniform_mass);
+        ps.resize(n);
+        std::memcpy(ps.get_positions_x().data(), x, n * sizeof(double));
+        std::memcpy(ps.get_positions_y().data(), y, n * sizeof(double));
+        std::memcpy(ps.get_positions_z().data(), z, n * sizeof(double));
+        std::memcpy(ps.get_momenta_x().data(), px, n * sizeof(double));
+        std::memcpy(ps.get_momenta_y().data(), py, n * sizeof(double));
+        std::memcpy(ps.get_momenta_z().data(), pz, n * sizeof(double));
+        const cn::cosmology::CosmologyModel model(cosmology(om, ol));
+        cn::cosmology::DriftKickIntegrator dk(model);
+        auto force = [&](std::span<const double> fx, std::span<const double> fy,
+                         std::span<const double> fz, std::span<const double>,
+                         std::span<double> gx, std::span<double> gy, std::span<double> gz,
+                         double a) {
+            const int rc = cb(fx.size(), fx.data(), fy.data(), fz.data(), gx.data(), gy.data(),
+                              gz.data(), a, user);
+            if (rc != 0) throw std::runtime_error("force callback failed");
+        };
+        cn::time::LeapfrogIntegrator integ(ps, dk, force, L);
+        cn::time::TimeStepper stepper(a_start, a_final, dlna, {a_final});
+        uint64_t steps = 0;
+        while (auto step = stepper.next_step()) {
+            integ.step(*step);
+            ++steps;
+        }
+        *n_steps_out = steps;
+        std::memcpy(x, ps.get_positions_x().data(), n * sizeof(double));
+        std::memcpy(y, ps.get_positions_y().data(), n * sizeof(double));
+        std::memcpy(z, ps.get_positions_z().data(), n * sizeof(double));
+        std::memcpy(px, ps.get_momenta_x().data(), n * sizeof(double));
+        std::memcpy(py, ps.get_momenta_y().data(), n * sizeof(double));
+        std::memcpy(pz, ps.get_momenta_z().data(), n * sizeof(double));
+        std::memcpy(ax, ps.get_accelerations_x().data(), n * sizeof(double));
+        std::memcpy(ay, ps.get_accelerations_y().data(), n * sizeof(double));
+        std::memcpy(az, ps.get_accelerations_z().data(), n * sizeof(double));
+    });
+}
+
+}  // extern "C"

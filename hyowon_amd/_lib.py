@@ -1,0 +1,128 @@
+"""ctypes binding of libhyowon_pm.so (include/hyowon_pm.h)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhyowon_pm.so")
+
+HY_OK = 0
+HY_PREC_FP64 = 0
+HY_PREC_FP32_MIXED = 1
+HY_HOST_POINTERS = 0x1
+HY_HAS_PREV = 0x2
+HY_HAS_NEXT = 0x4
+HY_ASYNC = 0x8
+
+EXPORTS = [
+    "hy_pm_create", "hy_pm_destroy", "hy_pm_get_info", "hy_last_error", "hy_abi_version",
+    "hy_pm_force", "hy_pm_kdk", "hy_pm_density", "hy_pm_potential", "hy_pm_gather",
+    "hy_pm_bucket_order", "hy_pm_xplane_order", "hy_pm_check", "hy_hubble", "hy_drift_factor",
+    "hy_kick_factor", "hy_geometric_mean", "hy_time_boundaries", "hy_exact_sum",
+    "hy_gravitational_constant", "hy_rho_crit0",
+]
+
+
+class HyError(RuntimeError):
+    """Failure reported by libhyowon_pm; `code` is the hy_status value."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+# hy_status -> Python exception class mirroring the reference's C++ exception
+class HyInvalidArgument(HyError, ValueError):
+    pass
+
+
+class HyLogicError(HyError):
+    pass
+
+
+class HyOverflowError(HyError, OverflowError):
+    pass
+
+
+class HyUnderflowError(HyError, ArithmeticError):
+    pass
+
+
+class HyDeviceError(HyError):
+    pass
+
+
+class HyRuntimeError(HyError):
+    pass
+
+
+class HyMemoryError(HyError, MemoryError):
+    pass
+
+
+_BY_CODE = {1: HyInvalidArgument, 2: HyLogicError, 3: HyOverflowError, 4: HyUnderflowError,
+            5: HyDeviceError, 6: HyRuntimeError, 7: HyMemoryError}
+
+
+class hy_pm_desc(C.Structure):
+    _fields_ = [("box_size", C.c_double), ("mesh_size", C.c_uint64), ("deconvolve_cic", C.c_int),
+                ("precision", C.c_int), ("device", C.c_int), ("stream", C.c_void_p),
+                ("max_particles", C.c_uint64)]
+
+
+class hy_pm_info(C.Structure):
+    _fields_ = [("mesh_size", C.c_uint64), ("tile_edge", C.c_uint64), ("bucket_count", C.c_uint64),
+                ("device_bytes", C.c_uint64), ("last_input_mass", C.c_double),
+                ("last_deposited_mass", C.c_double), ("precision", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libhyowon_pm.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m hyowon_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, u64, d, i, u32 = C.c_void_p, C.c_uint64, C.c_double, C.c_int, C.c_uint32
+    pd = C.POINTER(C.c_double)
+
+    def fn(name, res, *args):
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = list(args)
+
+    fn("hy_pm_create", i, C.POINTER(hy_pm_desc), C.POINTER(vp))
+    fn("hy_pm_destroy", i, vp)
+    fn("hy_pm_get_info", i, vp, C.POINTER(hy_pm_info))
+    fn("hy_last_error", C.c_char_p)
+    fn("hy_abi_version", i)
+    fn("hy_pm_force", i, vp, u64, vp, vp, vp, vp, d, vp, vp, vp, u32)
+    fn("hy_pm_kdk", i, vp, u64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, d, d, d, d,
+       vp, vp, vp, u32)
+    fn("hy_pm_density", i, vp, u64, vp, vp, vp, vp, d, vp)
+    fn("hy_pm_potential", i, vp, vp, vp)
+    fn("hy_pm_gather", i, vp, u64, vp, vp, vp, vp, vp, vp, vp)
+    fn("hy_pm_bucket_order", i, vp, u64, vp, vp, vp, vp, vp)
+    fn("hy_pm_xplane_order", i, vp, u64, vp, vp, vp)
+    fn("hy_pm_check", i, vp)
+    fn("hy_hubble", d, d, d, d)
+    fn("hy_drift_factor", i, d, d, d, d, pd)
+    fn("hy_kick_factor", i, d, d, d, d, pd)
+    fn("hy_geometric_mean", d, d, d)
+    fn("hy_time_boundaries", i, d, d, d, pd, u64, pd, u64, C.POINTER(C.c_uint64))
+    fn("hy_exact_sum", d, pd, u64)
+    fn("hy_gravitational_constant", d)
+    fn("hy_rho_crit0", d)
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != HY_OK:
+        msg = lib().hy_last_error().decode(errors="replace")
+        raise _BY_CODE.get(rc, HyError)(rc, msg)

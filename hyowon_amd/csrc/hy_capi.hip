@@ -1,0 +1,839 @@
+// C ABI of libhyowon_pm.so: context, workspace and launch orchestration of the
+// MI355X particle-mesh step.  Every entry point is extern "C" with plain
+// pointers; HIP failures and kernel-raised error bits are mapped onto the
+// reference's exception classes (see include/hyowon_pm.h).
+#include "hyowon_pm.h"
+
+#include "hy_host_scalars.hpp"
+#include "hy_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <limits>
+#include <numbers>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct HyError {
+    int code;
+    std::string what;
+};
+
+[[noreturn]] void fail(int code, const std::string& what) { throw HyError{code, what}; }
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) fail(HY_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return HY_OK;
+    } catch (const HyError& e) {
+        g_last_error = e.what;
+        return e.code;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return HY_EINVAL;
+    } catch (const std::underflow_error& e) {
+        g_last_error = e.what();
+        return HY_EUNDERFLOW;
+    } catch (const std::overflow_error& e) {
+        g_last_error = e.what();
+        return HY_EOVERFLOW;
+    } catch (const std::logic_error& e) {
+        g_last_error = e.what();
+        return HY_ELOGIC;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "allocation failed";
+        return HY_ENOMEM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return HY_ERUNTIME;
+    }
+}
+
+// Supported mesh sizes (compile-time FFT plans).
+#define HY_MESH_SIZES(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(1024)
+
+template <typename T>
+struct FftKernels {
+    void (*z_r2c)(hy::FftArgs<T>);
+    void (*y_fwd)(hy::FftArgs<T>);
+    void (*x_green)(hy::FftArgs<T>);
+    void (*y_bwd)(hy::FftArgs<T>);
+    void (*z_c2r)(hy::FftArgs<T>);
+    int rows_z, threads_z, lds_z;
+    int rows_y, threads_y, lds_y;
+    int rows_x, threads_x, lds_x;
+};
+
+template <typename T, int N>
+FftKernels<T> make_fft() {
+    using RZ = hy::RowsCfg<T, N / 2, N>;
+    using RY = hy::RowsCfg<T, N, N>;
+    using RX = hy::RowsCfg<T, N, (N / 2 + 1) * N>;
+    return {hy::k_fft_z_r2c<T, N>, hy::k_fft_y_fwd<T, N>, hy::k_fft_x_green<T, N>,
+            hy::k_fft_y_bwd<T, N>, hy::k_fft_z_c2r<T, N>,
+            RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES,
+            RY::ROWS, RY::THREADS, RY::LDS_BYTES,
+            RX::ROWS, RX::THREADS, RX::LDS_BYTES};
+}
+
+template <typename T>
+bool lookup_fft(int N, FftKernels<T>& out) {
+    switch (N) {
+#define HY_CASE(n) case n: out = make_fft<T, n>(); return true;
+        HY_MESH_SIZES(HY_CASE)
+#undef HY_CASE
+        default: return false;
+    }
+}
+
+int tile_edge_for(int N) {
+    for (int t = 16; t >= 1; --t)
+        if (N % t == 0) return t;
+    return 1;
+}
+
+uint64_t blocks_for(uint64_t n, uint64_t per) { return (n + per - 1) / per; }
+
+}  // namespace
+
+struct hy_pm_ctx {
+    hy_pm_desc desc{};
+    hy::Geom geom{};
+    int N = 0;
+    int prec = HY_PREC_FP64;
+    size_t real_bytes = 1;  // sizeof(T)
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    uint64_t cap = 0;
+    uint64_t ntiles = 0, nb = 0, nb_scan_blocks = 0;
+    uint64_t device_bytes = 0;
+    double green_num = 0.0;
+    double last_input_mass = 0.0, last_deposited_mass = 0.0;
+
+    // mesh
+    void* R = nullptr;
+    void* A = nullptr;
+    void* B = nullptr;
+    void* tw = nullptr;
+    void* lap = nullptr;
+    void* win = nullptr;
+    // particles
+    uint32_t* keys = nullptr;
+    uint32_t* ranks = nullptr;
+    uint32_t* order = nullptr;
+    uint32_t* counts = nullptr;
+    uint32_t* offsets = nullptr;
+    uint32_t* block_sums = nullptr;
+    uint32_t* grand_total = nullptr;
+    uint32_t* xcounts = nullptr;   // x-plane bins (N)
+    uint32_t* xoffsets = nullptr;  // N+1
+    double* tile_mass = nullptr;
+    double* mass_total = nullptr;
+    uint64_t* scratch = nullptr;
+    uint64_t scratch_cap = 0;
+    unsigned long long* scratch_cursor = nullptr;
+    uint32_t* err = nullptr;
+    void* gx = nullptr;
+    void* gy = nullptr;
+    void* gz = nullptr;
+    double* hpos = nullptr;  // device staging for host-pointer calls (3*cap doubles + masses)
+    double* hacc = nullptr;  // 3*cap
+
+    FftKernels<float> fft32{};
+    FftKernels<double> fft64{};
+
+    // pinned host mirror of the status words
+    struct Status {
+        uint32_t err;
+        uint32_t total;
+        double mass;
+    };
+    Status* hstatus = nullptr;
+    Status* dstatus = nullptr;
+
+    std::vector<void*> allocations;
+
+    void* alloc(size_t bytes) {
+        void* p = nullptr;
+        hip_check(hipMalloc(&p, bytes ? bytes : 16), "hipMalloc");
+        allocations.push_back(p);
+        device_bytes += bytes;
+        return p;
+    }
+    ~hy_pm_ctx() {
+        for (void* p : allocations) (void)hipFree(p);
+        if (hstatus) (void)hipHostFree(hstatus);
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+void set_device(const hy_pm_ctx* c) { hip_check(hipSetDevice(c->desc.device), "hipSetDevice"); }
+
+void launch_check(const char* what) { hip_check(hipGetLastError(), what); }
+
+// Map the kernel error word onto the reference's exceptions.
+void raise_errors(uint32_t e) {
+    using namespace hy;
+    if (e & ERR_POSITION_NONFINITE) fail(HY_EINVAL, "CIC positions must be finite");
+    if (e & ERR_MASS_INVALID) fail(HY_EINVAL, "CIC masses must be finite and non-negative");
+    if (e & ERR_COORDINATE) fail(HY_ELOGIC, "CIC bounded coordinate escaped the mesh domain");
+    if (e & ERR_SCRATCH) fail(HY_ERUNTIME, "deposit fallback scratch exhausted");
+    if (e & ERR_FIELD_NONFINITE) fail(HY_EINVAL, "CIC centered-gradient potential must be finite");
+    if (e & ERR_FORCE_NONFINITE) fail(HY_EOVERFLOW, "CIC centered-gradient gathered acceleration is non-finite");
+    if (e & ERR_MOMENTUM_NONFINITE) fail(HY_EOVERFLOW, "Leapfrog kick input is not finite");
+    if (e & ERR_UPDATE_NONFINITE) fail(HY_EOVERFLOW, "Leapfrog K1/drift update is not finite and representable");
+}
+
+// Input mass identity (src/gravity/pm_solver.cpp:120-130,665-684).
+double input_mass(hy_pm_ctx* c, uint64_t n, const double* masses, double uniform, bool host_ptrs) {
+    if (n == 0) fail(HY_ERUNTIME, "PMSolver exact input mass identity must be finite and positive");
+    double M;
+    if (!masses) {
+        if (!std::isfinite(uniform) || uniform <= 0.0)
+            fail(HY_EINVAL, "PMSolver uniform mass must be finite and positive");
+        // exact m*n rounded once: one binary64 product is correctly rounded
+        // because n (< 2^53) is exact.
+        M = uniform * static_cast<double>(n);
+    } else {
+        std::vector<double> h(n);
+        if (host_ptrs) std::memcpy(h.data(), masses, n * sizeof(double));
+        else hip_check(hipMemcpyAsync(h.data(), masses, n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "mass copy");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        M = hy::host::exact_sum(h.data(), n);
+        if (!std::isfinite(M)) fail(HY_EINVAL, "PMSolver explicit masses must be finite and non-negative");
+    }
+    if (!std::isfinite(M) || M <= 0.0)
+        fail(HY_ERUNTIME, "PMSolver exact input mass identity must be finite and positive");
+    return M;
+}
+
+// Bucket count/rank + scan + scatter.  mode 0: deposit buckets, 1: x-planes.
+void bucketize(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z, int mode,
+               uint32_t* counts, uint32_t* offsets, uint64_t nb, uint32_t* order) {
+    hip_check(hipMemsetAsync(counts, 0, nb * sizeof(uint32_t), c->stream), "memset counts");
+    hipLaunchKernelGGL(hy::k_bucket_key, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, x, y, z,
+                       c->geom, mode, c->keys, c->ranks, counts, c->err);
+    launch_check("k_bucket_key");
+    const uint64_t nblk = blocks_for(nb, hy::SCAN_TILE);
+    hipLaunchKernelGGL(hy::k_scan_reduce, dim3(nblk), dim3(hy::SCAN_THREADS), 0, c->stream, counts, nb,
+                       c->block_sums);
+    hipLaunchKernelGGL(hy::k_scan_blocks, dim3(1), dim3(hy::SCAN_THREADS), 0, c->stream, c->block_sums,
+                       (uint32_t)nblk, c->grand_total);
+    hipLaunchKernelGGL(hy::k_scan_apply, dim3(nblk), dim3(hy::SCAN_THREADS), 0, c->stream, counts, nb,
+                       c->block_sums, c->grand_total, offsets);
+    launch_check("k_scan");
+    hipLaunchKernelGGL(hy::k_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->keys,
+                       c->ranks, offsets, order);
+    launch_check("k_scatter");
+}
+
+void reset_status(hy_pm_ctx* c) {
+    hip_check(hipMemsetAsync(c->err, 0, sizeof(uint32_t), c->stream), "memset err");
+    hip_check(hipMemsetAsync(c->scratch_cursor, 0, sizeof(unsigned long long), c->stream), "memset cursor");
+}
+
+// Copy err / grand total / deposited mass to the host and synchronise.
+hy_pm_ctx::Status fetch_status(hy_pm_ctx* c) {
+    hip_check(hipMemcpyAsync(&c->hstatus->err, c->err, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream), "status");
+    hip_check(hipMemcpyAsync(&c->hstatus->total, c->grand_total, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream), "status");
+    hip_check(hipMemcpyAsync(&c->hstatus->mass, c->mass_total, sizeof(double), hipMemcpyDeviceToHost, c->stream), "status");
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return *c->hstatus;
+}
+
+struct DepositPlan {
+    double M;
+    hy::DepParams P;
+};
+
+DepositPlan plan_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z,
+                         const double* masses, double uniform, double Min) {
+    const double dx = c->geom.dx;
+    const double cells = static_cast<double>(static_cast<uint64_t>(c->N) * c->N * c->N);
+    const double mean = Min / cells;
+    const double inv_volume = 1.0 / ((dx * dx) * dx);
+    const bool exact_mean = std::isfinite(mean) && mean > 0.0 && std::fma(mean, cells, -Min) == 0.0;
+    hy::DepParams P{};
+    P.g = c->geom;
+    P.x = x; P.y = y; P.z = z;
+    P.masses = masses;
+    P.uniform_mass = uniform;
+    P.offsets = c->offsets;
+    P.order = c->order;
+    P.mean_mass = mean;
+    P.inv_volume = inv_volume;
+    P.total_mass = Min;
+    P.cells_real = cells;
+    P.centred_scale = inv_volume * (1.0 / cells);
+    P.direct = (exact_mean && std::isfinite(inv_volume) && inv_volume > 0.0) ? 1 : 0;
+    P.tile_mass = c->tile_mass;
+    P.scratch = c->scratch;
+    P.scratch_cap = c->scratch_cap;
+    P.scratch_cursor = c->scratch_cursor;
+    P.err = c->err;
+    (void)n;
+    return {Min, P};
+}
+
+void launch_deposit(hy_pm_ctx* c, const hy::DepParams& P) {
+    if (c->prec == HY_PREC_FP64)
+        hipLaunchKernelGGL(hy::k_deposit<double>, dim3(c->ntiles), dim3(hy::DEP_THREADS), 0, c->stream, P,
+                           static_cast<double*>(c->R));
+    else
+        hipLaunchKernelGGL(hy::k_deposit<float>, dim3(c->ntiles), dim3(hy::DEP_THREADS), 0, c->stream, P,
+                           static_cast<float*>(c->R));
+    launch_check("k_deposit");
+    hipLaunchKernelGGL(hy::k_reduce_sum, dim3(1), dim3(256), 0, c->stream, c->tile_mass, c->ntiles,
+                       c->mass_total);
+    launch_check("k_reduce_sum");
+}
+
+template <typename T>
+void launch_fft(hy_pm_ctx* c, const FftKernels<T>& K) {
+    hy::FftArgs<T> a{};
+    a.R = static_cast<T*>(c->R);
+    a.A = static_cast<hy::cplx<T>*>(c->A);
+    a.B = static_cast<hy::cplx<T>*>(c->B);
+    a.tw = static_cast<const hy::cplx<T>*>(c->tw);
+    a.green_lap = static_cast<const T*>(c->lap);
+    a.green_win = static_cast<const T*>(c->win);
+    a.green_num = c->green_num;
+    a.norm = 1.0 / (static_cast<double>(c->N) * static_cast<double>(c->N) * static_cast<double>(c->N));
+    a.deconvolve = c->desc.deconvolve_cic ? 1 : 0;
+    a.err = c->err;
+    const int N = c->N, NH = N / 2 + 1;
+    hipLaunchKernelGGL(K.z_r2c, dim3(N * (N / K.rows_z)), dim3(K.threads_z), K.lds_z, c->stream, a);
+    launch_check("k_fft_z_r2c");
+    hipLaunchKernelGGL(K.y_fwd, dim3(NH * (N / K.rows_y)), dim3(K.threads_y), K.lds_y, c->stream, a);
+    launch_check("k_fft_y_fwd");
+    hipLaunchKernelGGL(K.x_green, dim3((uint64_t)NH * N / K.rows_x), dim3(K.threads_x), K.lds_x, c->stream, a);
+    launch_check("k_fft_x_green");
+    hipLaunchKernelGGL(K.y_bwd, dim3(NH * (N / K.rows_y)), dim3(K.threads_y), K.lds_y, c->stream, a);
+    launch_check("k_fft_y_bwd");
+    hipLaunchKernelGGL(K.z_c2r, dim3(N * (N / K.rows_z)), dim3(K.threads_z), K.lds_z, c->stream, a);
+    launch_check("k_fft_z_c2r");
+}
+
+void run_fft(hy_pm_ctx* c) {
+    if (c->prec == HY_PREC_FP64) launch_fft<double>(c, c->fft64);
+    else launch_fft<float>(c, c->fft32);
+}
+
+template <typename T, typename PT>
+void launch_gather(hy_pm_ctx* c, const hy::GatherArgs<T, PT>& a) {
+    hipLaunchKernelGGL((hy::k_gather_kdk<T, PT>), dim3(blocks_for(a.n, 256)), dim3(256), 0, c->stream, a);
+    launch_check("k_gather_kdk");
+}
+
+// Mass conservation (src/math/cic_mass_conservation.cpp:25-63).  The FP32
+// deposit sums cells in binary32, so its bound uses binary32 epsilon per cell
+// plus the binary64 bound of the reference.
+void check_mass(hy_pm_ctx* c, double input, double deposited, uint64_t n) {
+    if (!std::isfinite(deposited) || deposited <= 0.0)
+        fail(HY_ERUNTIME, "PMSolver total deposited mass must be finite and positive");
+    const long double eps = std::numeric_limits<double>::epsilon();
+    const long double nu = (32.0L * static_cast<long double>(n) + 1.0L) * eps;
+    if (!(nu < 0.25L)) fail(HY_EOVERFLOW, "CIC mass conservation forward-error bound is not informative at this particle count");
+    const long double gamma = nu / (1.0L - nu);
+    const long double scale = std::max<long double>(input, deposited);
+    long double tol = (gamma + 64.0L * eps) * scale;
+    if (c->prec == HY_PREC_FP32_MIXED) tol += 1e-4L * scale;
+    if (std::fabs(static_cast<long double>(deposited) - static_cast<long double>(input)) > tol)
+        fail(HY_ERUNTIME, "CIC deposited mass violates the forward-error bound");
+}
+
+// Deposit stage: bucketize, deposit (+ retry with a larger fallback scratch).
+void run_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z,
+                 const double* masses, double uniform, double Min, bool sync) {
+    bucketize(c, n, x, y, z, 0, c->counts, c->offsets, c->nb, c->order);
+    auto plan = plan_deposit(c, n, x, y, z, masses, uniform, Min);
+    launch_deposit(c, plan.P);
+    c->last_input_mass = Min;
+    if (!sync) return;
+    for (int attempt = 0;; ++attempt) {
+        const auto st = fetch_status(c);
+        if ((st.err & hy::ERR_SCRATCH) && attempt < 3) {
+            // grow the fallback scratch (records of the hot tiles) and redo the deposit
+            const uint64_t grow = c->scratch_cap * 2;
+            void* p = nullptr;
+            hip_check(hipMalloc(&p, grow * sizeof(uint64_t)), "hipMalloc scratch");
+            c->allocations.push_back(p);
+            c->device_bytes += grow * sizeof(uint64_t);
+            c->scratch = static_cast<uint64_t*>(p);
+            c->scratch_cap = grow;
+            reset_status(c);
+            plan = plan_deposit(c, n, x, y, z, masses, uniform, Min);
+            launch_deposit(c, plan.P);
+            continue;
+        }
+        raise_errors(st.err);
+        if (st.total != n) fail(HY_ELOGIC, "CIC bucket routing does not cover each particle exactly once");
+        c->last_deposited_mass = st.mass;
+        check_mass(c, Min, st.mass, n);
+        return;
+    }
+}
+
+void require_particles(hy_pm_ctx* c, uint64_t n) {
+    if (n > c->cap) fail(HY_EINVAL, "particle count exceeds the context capacity (max_particles)");
+    if (n > 0xffffffffull) fail(HY_EOVERFLOW, "particle count exceeds the 32-bit index range");
+}
+
+}  // namespace
+
+extern "C" {
+
+int hy_abi_version(void) { return HY_ABI_VERSION; }
+
+const char* hy_last_error(void) { return g_last_error.c_str(); }
+
+int hy_pm_create(const hy_pm_desc* desc, hy_pm_ctx** out) {
+    return guarded([&] {
+        if (!desc || !out) fail(HY_EINVAL, "hy_pm_create: null argument");
+        *out = nullptr;
+        const double L = desc->box_size;
+        const uint64_t Nu = desc->mesh_size;
+        if (!std::isfinite(L) || L <= 0.0) fail(HY_EINVAL, "MeshGeometry box size must be finite and positive");
+        if (desc->precision != HY_PREC_FP64 && desc->precision != HY_PREC_FP32_MIXED)
+            fail(HY_EINVAL, "unknown precision");
+        FftKernels<float> k32{};
+        FftKernels<double> k64{};
+        if (Nu > 4096 || !lookup_fft<float>((int)Nu, k32) || !lookup_fft<double>((int)Nu, k64))
+            fail(HY_EINVAL, "mesh size must be 2^a (8..1024) or 3*2^a (12..384)");
+        const int N = (int)Nu;
+        auto c = std::make_unique<hy_pm_ctx>();
+        c->desc = *desc;
+        c->N = N;
+        c->prec = desc->precision;
+        c->real_bytes = c->prec == HY_PREC_FP64 ? sizeof(double) : sizeof(float);
+        c->fft32 = k32;
+        c->fft64 = k64;
+        const double dx = L / (double)N;
+        if (!std::isfinite(dx) || dx <= 0.0) fail(HY_EOVERFLOW, "MeshGeometry cell size is not representable");
+        c->geom.L = L;
+        c->geom.dx = dx;
+        c->geom.upper = std::nextafter((double)N, 0.0);
+        c->geom.N = N;
+        c->geom.T = tile_edge_for(N);
+        c->geom.nT = N / c->geom.T;
+        c->cap = desc->max_particles;
+        set_device(c.get());
+        if (desc->stream) {
+            c->stream = static_cast<hipStream_t>(desc->stream);
+        } else {
+            hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+            c->own_stream = true;
+        }
+
+        const uint64_t cells = (uint64_t)N * N * N;
+        const uint64_t NH = N / 2 + 1;
+        const uint64_t ccount = (uint64_t)N * NH * N;
+        const size_t rb = c->real_bytes;
+        c->R = c->alloc(cells * rb);
+        c->A = c->alloc(ccount * 2 * rb);
+        c->B = c->alloc(ccount * 2 * rb);
+        c->tw = c->alloc((size_t)N * 2 * rb);
+        c->lap = c->alloc((size_t)N * rb);
+        c->win = c->alloc((size_t)N * rb);
+
+        // Tables: twiddles exp(-2 pi i m/N) in long double; Laplacian symbol and
+        // CIC window exactly as src/mesh/green_function.cpp:59-87,137-163.
+        const long double pil = 3.141592653589793238462643383279502884L;
+        std::vector<double> tw(2 * N), lap(N), win(N);
+        const double inv = 1.0 / dx;
+        const double inv2 = inv * inv;
+        for (int m = 0; m < N; ++m) {
+            const long double ang = -2.0L * pil * (long double)m / (long double)N;
+            tw[2 * m] = (double)std::cos(ang);
+            tw[2 * m + 1] = (double)std::sin(ang);
+            const int64_t mode = m <= N / 2 ? m : (int64_t)m - N;
+            const double h = std::numbers::pi * (double)mode / (double)N;
+            const double s = std::sin(h);
+            lap[m] = -4.0 * inv2 * s * s;
+            const long double arg = pil * (long double)mode / (long double)N;
+            if (arg == 0.0L) {
+                win[m] = 1.0;
+            } else {
+                long double sc = std::fabs(std::sin(arg) / arg);
+                if (sc > 1.0L) sc = 1.0L;
+                win[m] = (double)(sc * sc);
+            }
+        }
+        c->green_num = 4.0 * std::numbers::pi * hy::host::gravitational_constant();
+        auto upload = [&](void* dst, const std::vector<double>& v) {
+            if (c->prec == HY_PREC_FP64) {
+                hip_check(hipMemcpy(dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice), "upload");
+            } else {
+                std::vector<float> f(v.begin(), v.end());
+                hip_check(hipMemcpy(dst, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice), "upload");
+            }
+        };
+        upload(c->tw, tw);
+        upload(c->lap, lap);
+        upload(c->win, win);
+
+        const uint64_t cap = std::max<uint64_t>(c->cap, 1);
+        c->keys = (uint32_t*)c->alloc(cap * 4);
+        c->ranks = (uint32_t*)c->alloc(cap * 4);
+        c->order = (uint32_t*)c->alloc(cap * 4);
+        c->ntiles = (uint64_t)c->geom.nT * c->geom.nT * c->geom.nT;
+        c->nb = c->ntiles * 8;
+        const uint64_t nb_max = std::max<uint64_t>(c->nb, (uint64_t)N);
+        c->counts = (uint32_t*)c->alloc(nb_max * 4);
+        c->offsets = (uint32_t*)c->alloc((nb_max + 1) * 4);
+        c->block_sums = (uint32_t*)c->alloc((blocks_for(nb_max, hy::SCAN_TILE) + 1) * 4);
+        c->grand_total = (uint32_t*)c->alloc(16);
+        c->xcounts = (uint32_t*)c->alloc((size_t)N * 4);
+        c->xoffsets = (uint32_t*)c->alloc(((size_t)N + 1) * 4);
+        c->tile_mass = (double*)c->alloc(c->ntiles * 8);
+        c->mass_total = (double*)c->alloc(16);
+        c->scratch_cap = 2 * cap + (1u << 16);
+        c->scratch = (uint64_t*)c->alloc(c->scratch_cap * 8);
+        c->scratch_cursor = (unsigned long long*)c->alloc(16);
+        c->err = (uint32_t*)c->alloc(16);
+        c->gx = c->alloc(cap * rb);
+        c->gy = c->alloc(cap * rb);
+        c->gz = c->alloc(cap * rb);
+        hip_check(hipHostMalloc((void**)&c->hstatus, sizeof(hy_pm_ctx::Status), hipHostMallocDefault), "hipHostMalloc");
+        hip_check(hipMemset(c->err, 0, 16), "memset");
+        hip_check(hipMemset(c->mass_total, 0, 16), "memset");
+        hip_check(hipMemset(c->grand_total, 0, 16), "memset");
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        *out = c.release();
+    });
+}
+
+int hy_pm_destroy(hy_pm_ctx* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->desc.device);
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        delete ctx;
+    });
+}
+
+int hy_pm_get_info(const hy_pm_ctx* c, hy_pm_info* info) {
+    return guarded([&] {
+        if (!c || !info) fail(HY_EINVAL, "hy_pm_get_info: null argument");
+        info->mesh_size = c->N;
+        info->tile_edge = c->geom.T;
+        info->bucket_count = c->nb;
+        info->device_bytes = c->device_bytes;
+        info->last_input_mass = c->last_input_mass;
+        info->last_deposited_mass = c->last_deposited_mass;
+        info->precision = c->prec;
+    });
+}
+
+int hy_pm_check(hy_pm_ctx* c) {
+    return guarded([&] {
+        if (!c) fail(HY_EINVAL, "null context");
+        set_device(c);
+        const auto st = fetch_status(c);
+        raise_errors(st.err);
+    });
+}
+
+int hy_pm_density(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z,
+                  const double* masses, double uniform, void* mesh_out) {
+    return guarded([&] {
+        if (!c || !x || !y || !z || !mesh_out) fail(HY_EINVAL, "hy_pm_density: null argument");
+        set_device(c);
+        require_particles(c, n);
+        const double M = input_mass(c, n, masses, uniform, false);
+        reset_status(c);
+        if (masses) {
+            hipLaunchKernelGGL(hy::k_check_mass, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, masses, c->err);
+        }
+        run_deposit(c, n, x, y, z, masses, uniform, M, true);
+        const size_t bytes = (size_t)c->N * c->N * c->N * c->real_bytes;
+        hip_check(hipMemcpyAsync(mesh_out, c->R, bytes, hipMemcpyDeviceToDevice, c->stream), "copy mesh");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+int hy_pm_potential(hy_pm_ctx* c, const void* source_in, void* phi_out) {
+    return guarded([&] {
+        if (!c || !source_in || !phi_out) fail(HY_EINVAL, "hy_pm_potential: null argument");
+        set_device(c);
+        const size_t bytes = (size_t)c->N * c->N * c->N * c->real_bytes;
+        reset_status(c);
+        hip_check(hipMemcpyAsync(c->R, source_in, bytes, hipMemcpyDeviceToDevice, c->stream), "copy source");
+        run_fft(c);
+        hip_check(hipMemcpyAsync(phi_out, c->R, bytes, hipMemcpyDeviceToDevice, c->stream), "copy phi");
+        const auto st = fetch_status(c);
+        raise_errors(st.err);
+    });
+}
+
+int hy_pm_gather(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z,
+                 const void* phi, double* ax, double* ay, double* az) {
+    return guarded([&] {
+        if (!c || !x || !y || !z || !phi || !ax || !ay || !az) fail(HY_EINVAL, "hy_pm_gather: null argument");
+        set_device(c);
+        require_particles(c, n);
+        reset_status(c);
+        auto fill = [&](auto& a) {
+            a.g = c->geom; a.n = n;
+            a.x_in = x; a.y_in = y; a.z_in = z;
+            a.acc_x = ax; a.acc_y = ay; a.acc_z = az;
+            a.err = c->err;
+        };
+        if (c->prec == HY_PREC_FP64) {
+            hy::GatherArgs<double, double> a{};
+            fill(a);
+            a.phi = static_cast<const double*>(phi);
+            launch_gather(c, a);
+        } else {
+            hy::GatherArgs<float, float> a{};
+            fill(a);
+            a.phi = static_cast<const float*>(phi);
+            launch_gather(c, a);
+        }
+        const auto st = fetch_status(c);
+        raise_errors(st.err);
+    });
+}
+
+int hy_pm_force(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z,
+                const double* masses, double uniform, double* ax, double* ay, double* az,
+                uint32_t flags) {
+    return guarded([&] {
+        if (!c) fail(HY_EINVAL, "hy_pm_force: null context");
+        if (n == 0) fail(HY_ERUNTIME, "PMSolver exact input mass identity must be finite and positive");
+        if (!x || !y || !z || !ax || !ay || !az) fail(HY_EINVAL, "hy_pm_force: null argument");
+        set_device(c);
+        require_particles(c, n);
+        const bool host = (flags & HY_HOST_POINTERS) != 0;
+        if (host) {
+            // reject overlapping output/input spans (src/gravity/pm_solver.cpp:231-263)
+            auto overlap = [](const double* a, const double* b, uint64_t k) {
+                return k && a < b + k && b < a + k;
+            };
+            const double* outs[3] = {ax, ay, az};
+            const double* ins[4] = {x, y, z, masses};
+            for (int i = 0; i < 3; ++i) {
+                for (int j = i + 1; j < 3; ++j)
+                    if (overlap(outs[i], outs[j], n)) fail(HY_EINVAL, "PMSolver force spans overlap");
+                for (int j = 0; j < 4; ++j)
+                    if (ins[j] && overlap(outs[i], ins[j], n)) fail(HY_EINVAL, "PMSolver force spans overlap");
+            }
+        }
+        const double M = input_mass(c, n, masses, uniform, host);
+        const double *dx_ = x, *dy_ = y, *dz_ = z, *dm = masses;
+        if (host) {
+            if (!c->hpos) {
+                c->hpos = (double*)c->alloc(c->cap * 4 * sizeof(double));
+                c->hacc = (double*)c->alloc(c->cap * 3 * sizeof(double));
+            }
+            double* s = c->hpos;
+            hip_check(hipMemcpyAsync(s, x, n * 8, hipMemcpyHostToDevice, c->stream), "h2d");
+            hip_check(hipMemcpyAsync(s + c->cap, y, n * 8, hipMemcpyHostToDevice, c->stream), "h2d");
+            hip_check(hipMemcpyAsync(s + 2 * c->cap, z, n * 8, hipMemcpyHostToDevice, c->stream), "h2d");
+            dx_ = s; dy_ = s + c->cap; dz_ = s + 2 * c->cap;
+            if (masses) {
+                hip_check(hipMemcpyAsync(s + 3 * c->cap, masses, n * 8, hipMemcpyHostToDevice, c->stream), "h2d");
+                dm = s + 3 * c->cap;
+            }
+        }
+        reset_status(c);
+        if (dm) hipLaunchKernelGGL(hy::k_check_mass, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, dm, c->err);
+        run_deposit(c, n, dx_, dy_, dz_, dm, uniform, M, true);
+        run_fft(c);
+        auto fill = [&](auto& a) {
+            a.g = c->geom; a.n = n;
+            a.x_in = dx_; a.y_in = dy_; a.z_in = dz_;
+            a.err = c->err;
+        };
+        if (c->prec == HY_PREC_FP64) {
+            hy::GatherArgs<double, double> a{};
+            fill(a);
+            a.phi = static_cast<const double*>(c->R);
+            a.g_x = (double*)c->gx; a.g_y = (double*)c->gy; a.g_z = (double*)c->gz;
+            launch_gather(c, a);
+        } else {
+            hy::GatherArgs<float, float> a{};
+            fill(a);
+            a.phi = static_cast<const float*>(c->R);
+            a.g_x = (float*)c->gx; a.g_y = (float*)c->gy; a.g_z = (float*)c->gz;
+            launch_gather(c, a);
+        }
+        const auto st = fetch_status(c);
+        raise_errors(st.err);
+        // commit: acc += g (only after every check passed)
+        double *cx = ax, *cy = ay, *cz = az;
+        if (host) {
+            cx = c->hacc; cy = c->hacc + c->cap; cz = c->hacc + 2 * c->cap;
+            hip_check(hipMemcpyAsync(cx, ax, n * 8, hipMemcpyHostToDevice, c->stream), "h2d");
+            hip_check(hipMemcpyAsync(cy, ay, n * 8, hipMemcpyHostToDevice, c->stream), "h2d");
+            hip_check(hipMemcpyAsync(cz, az, n * 8, hipMemcpyHostToDevice, c->stream), "h2d");
+        }
+        if (c->prec == HY_PREC_FP64)
+            hipLaunchKernelGGL(hy::k_commit_force<double>, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n,
+                               (const double*)c->gx, (const double*)c->gy, (const double*)c->gz, cx, cy, cz);
+        else
+            hipLaunchKernelGGL(hy::k_commit_force<float>, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n,
+                               (const float*)c->gx, (const float*)c->gy, (const float*)c->gz, cx, cy, cz);
+        launch_check("k_commit_force");
+        if (host) {
+            hip_check(hipMemcpyAsync(ax, cx, n * 8, hipMemcpyDeviceToHost, c->stream), "d2h");
+            hip_check(hipMemcpyAsync(ay, cy, n * 8, hipMemcpyDeviceToHost, c->stream), "d2h");
+            hip_check(hipMemcpyAsync(az, cz, n * 8, hipMemcpyDeviceToHost, c->stream), "d2h");
+        }
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+int hy_pm_kdk(hy_pm_ctx* c, uint64_t n, const double* x_in, const double* y_in, const double* z_in,
+              const void* px_in, const void* py_in, const void* pz_in, double* x_out, double* y_out,
+              double* z_out, void* px_out, void* py_out, void* pz_out, const double* masses,
+              double uniform, double k_prev, double k_next, double d_next, double* acc_x,
+              double* acc_y, double* acc_z, uint32_t flags) {
+    return guarded([&] {
+        if (!c) fail(HY_EINVAL, "hy_pm_kdk: null context");
+        if (n == 0) fail(HY_ERUNTIME, "PMSolver exact input mass identity must be finite and positive");
+        if (!x_in || !y_in || !z_in) fail(HY_EINVAL, "hy_pm_kdk: null position");
+        const bool has_prev = (flags & HY_HAS_PREV) != 0, has_next = (flags & HY_HAS_NEXT) != 0;
+        if ((has_prev || has_next) && (!px_in || !py_in || !pz_in || !px_out || !py_out || !pz_out))
+            fail(HY_EINVAL, "hy_pm_kdk: null momentum");
+        if (has_next && (!x_out || !y_out || !z_out)) fail(HY_EINVAL, "hy_pm_kdk: null output position");
+        if (flags & HY_HOST_POINTERS) fail(HY_EINVAL, "hy_pm_kdk takes device pointers only");
+        if ((has_prev && !std::isfinite(k_prev)) || (has_next && (!std::isfinite(k_next) || !std::isfinite(d_next))))
+            fail(HY_EOVERFLOW, "Leapfrog drift/kick factors must be finite");
+        set_device(c);
+        require_particles(c, n);
+        const bool sync = (flags & HY_ASYNC) == 0;
+        const double M = input_mass(c, n, masses, uniform, false);
+        if (sync) reset_status(c);
+        else hip_check(hipMemsetAsync(c->scratch_cursor, 0, sizeof(unsigned long long), c->stream), "memset");
+        if (masses) hipLaunchKernelGGL(hy::k_check_mass, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, masses, c->err);
+        run_deposit(c, n, x_in, y_in, z_in, masses, uniform, M, sync);
+        run_fft(c);
+        auto fill = [&](auto& a) {
+            a.g = c->geom; a.n = n;
+            a.x_in = x_in; a.y_in = y_in; a.z_in = z_in;
+            a.x_out = x_out ? x_out : const_cast<double*>(x_in);
+            a.y_out = y_out ? y_out : const_cast<double*>(y_in);
+            a.z_out = z_out ? z_out : const_cast<double*>(z_in);
+            a.acc_x = acc_x; a.acc_y = acc_y; a.acc_z = acc_z;
+            a.k_prev = k_prev; a.k_next = k_next; a.d_next = d_next;
+            a.has_prev = has_prev; a.has_next = has_next;
+            a.err = c->err;
+        };
+        if (c->prec == HY_PREC_FP64) {
+            hy::GatherArgs<double, double> a{};
+            fill(a);
+            a.phi = static_cast<const double*>(c->R);
+            a.px_in = (const double*)px_in; a.py_in = (const double*)py_in; a.pz_in = (const double*)pz_in;
+            a.px_out = (double*)px_out; a.py_out = (double*)py_out; a.pz_out = (double*)pz_out;
+            launch_gather(c, a);
+        } else {
+            hy::GatherArgs<float, float> a{};
+            fill(a);
+            a.phi = static_cast<const float*>(c->R);
+            a.px_in = (const float*)px_in; a.py_in = (const float*)py_in; a.pz_in = (const float*)pz_in;
+            a.px_out = (float*)px_out; a.py_out = (float*)py_out; a.pz_out = (float*)pz_out;
+            launch_gather(c, a);
+        }
+        if (sync) {
+            const auto st = fetch_status(c);
+            raise_errors(st.err);
+        }
+    });
+}
+
+int hy_pm_bucket_order(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z,
+                       uint32_t* order, uint32_t* offsets) {
+    return guarded([&] {
+        if (!c || !x || !y || !z || !order || !offsets) fail(HY_EINVAL, "hy_pm_bucket_order: null argument");
+        set_device(c);
+        require_particles(c, n);
+        reset_status(c);
+        bucketize(c, n, x, y, z, 0, c->counts, offsets, c->nb, order);
+        hipLaunchKernelGGL(hy::k_canonicalize, dim3(std::min<uint64_t>(c->nb, 65535)), dim3(hy::CANON_THREADS), 0,
+                           c->stream, c->nb, offsets, order, c->scratch, c->scratch_cap, c->scratch_cursor, c->err);
+        launch_check("k_canonicalize");
+        const auto st = fetch_status(c);
+        raise_errors(st.err);
+        if (st.total != n) fail(HY_ELOGIC, "bucket routing does not cover each particle exactly once");
+    });
+}
+
+int hy_pm_xplane_order(hy_pm_ctx* c, uint64_t n, const double* x, uint32_t* order, uint32_t* offsets) {
+    return guarded([&] {
+        if (!c || !x || !order || !offsets) fail(HY_EINVAL, "hy_pm_xplane_order: null argument");
+        set_device(c);
+        require_particles(c, n);
+        reset_status(c);
+        bucketize(c, n, x, x, x, 1, c->xcounts, offsets, (uint64_t)c->N, order);
+        hipLaunchKernelGGL(hy::k_canonicalize, dim3(c->N), dim3(hy::CANON_THREADS), 0, c->stream,
+                           (uint64_t)c->N, offsets, order, c->scratch, c->scratch_cap, c->scratch_cursor, c->err);
+        launch_check("k_canonicalize");
+        const auto st = fetch_status(c);
+        raise_errors(st.err);
+        if (st.total != n) fail(HY_ELOGIC, "CIC x-plane binning does not cover the complete particle population");
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Host scalars
+// ---------------------------------------------------------------------------
+double hy_hubble(double a, double om, double ol) {
+    double r = std::numeric_limits<double>::quiet_NaN();
+    guarded([&] { r = hy::host::hubble(a, om, ol); });
+    return r;
+}
+
+int hy_drift_factor(double a1, double a2, double om, double ol, double* out) {
+    return guarded([&] {
+        if (!out) fail(HY_EINVAL, "null output");
+        *out = hy::host::drift_factor(a1, a2, om, ol);
+    });
+}
+
+int hy_kick_factor(double a1, double a2, double om, double ol, double* out) {
+    return guarded([&] {
+        if (!out) fail(HY_EINVAL, "null output");
+        *out = hy::host::kick_factor(a1, a2, om, ol);
+    });
+}
+
+double hy_geometric_mean(double a, double b) {
+    double r = std::numeric_limits<double>::quiet_NaN();
+    guarded([&] { r = hy::host::geometric_mean(a, b); });
+    return r;
+}
+
+int hy_time_boundaries(double a_start, double a_final, double dlna, const double* stops, uint64_t n_stops,
+                       double* out, uint64_t capacity, uint64_t* count) {
+    return guarded([&] {
+        if (!out || !count || (n_stops && !stops)) fail(HY_EINVAL, "null argument");
+        const auto b = hy::host::time_boundaries(a_start, a_final, dlna,
+                                                 std::vector<double>(stops, stops + n_stops));
+        if (b.size() > capacity) fail(HY_EOVERFLOW, "boundary capacity too small");
+        std::memcpy(out, b.data(), b.size() * sizeof(double));
+        *count = b.size();
+    });
+}
+
+double hy_exact_sum(const double* v, uint64_t n) { return hy::host::exact_sum(v, n); }
+double hy_gravitational_constant(void) { return hy::host::gravitational_constant(); }
+double hy_rho_crit0(void) { return hy::host::rho_crit0(); }
+
+}  // extern "C"

@@ -1,0 +1,225 @@
+// LDS-staged pencil FFT building blocks for the PM Poisson solve (gfx950).
+//
+// A workgroup holds ROWS rows of LEN complex values in LDS (row stride
+// LEN+1) and runs an in-place Stockham autosort transform over all rows at
+// once: every stage each thread loads E/R radix-R butterflies into registers,
+// twiddles and transforms them, then (after a barrier) stores them at their
+// autosorted positions.  LEN = 2^a or 3*2^a; radices 3, 8, 4, 2.
+//
+// Twiddles come from one table tw[m] = exp(-2*pi*i*m/NT) (NT = mesh size),
+// evaluated in binary64 on the host, read with a stride NT/LEN.
+//
+// Conventions follow the reference FFT backend (src/mesh/fft_backend.cpp:
+// 1088-1145): forward transforms are unnormalized, the inverse is the
+// unnormalized backward DFT; the 1/N^3 factor is folded into the Green
+// multiply of the x pass.
+#pragma once
+
+#include "hy_device.hpp"
+
+namespace hy {
+
+// ---------------------------------------------------------------------------
+// Compile-time plan
+// ---------------------------------------------------------------------------
+template <int LEN>
+struct FftPlan {
+    static constexpr bool has3 = (LEN % 3) == 0;
+    static constexpr int E = has3 ? (LEN < 24 ? LEN : 24) : (LEN < 16 ? LEN : 16);
+    static constexpr int TPR = LEN / E;  // threads per row
+    static constexpr int radix(int s) {
+        int rem = LEN;
+        int r = 1;
+        for (int i = 0; i <= s; ++i) {
+            if (rem == 1) return 1;
+            if (i == 0 && rem % 3 == 0) r = 3;
+            else if (rem % 8 == 0) r = 8;
+            else if (rem % 4 == 0) r = 4;
+            else r = 2;
+            rem /= r;
+        }
+        return r;
+    }
+    static constexpr int stages() {
+        int s = 0;
+        while (radix(s) != 1) ++s;
+        return s;
+    }
+    static constexpr int prod_before(int s) {
+        int p = 1;
+        for (int i = 0; i < s; ++i) p *= radix(i);
+        return p;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Complex helpers
+// ---------------------------------------------------------------------------
+template <typename C>
+__device__ __forceinline__ C cadd(C a, C b) { return {a.x + b.x, a.y + b.y}; }
+template <typename C>
+__device__ __forceinline__ C csub(C a, C b) { return {a.x - b.x, a.y - b.y}; }
+template <typename C>
+__device__ __forceinline__ C cmul(C a, C b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename C>
+__device__ __forceinline__ C cconj(C a) { return {a.x, -a.y}; }
+// multiply by s*i, s = +1 or -1
+template <int S, typename C>
+__device__ __forceinline__ C cmuli(C a) {
+    if constexpr (S > 0) return {-a.y, a.x};
+    else return {a.y, -a.x};
+}
+
+// In-register DFTs of radix 2, 3, 4, 8.  S = -1 forward, +1 backward.
+template <int S, typename C>
+__device__ __forceinline__ void dft2(C* u) {
+    const C a = u[0], b = u[1];
+    u[0] = cadd(a, b);
+    u[1] = csub(a, b);
+}
+
+template <int S, typename C>
+__device__ __forceinline__ void dft3(C* u) {
+    using T = decltype(u[0].x);
+    const T h = (T)0.86602540378443864676;  // sqrt(3)/2
+    const C t1 = cadd(u[1], u[2]);
+    const C t2 = {u[0].x - (T)0.5 * t1.x, u[0].y - (T)0.5 * t1.y};
+    const C d0 = csub(u[1], u[2]);
+    const C d = {(T)S * h * d0.x, (T)S * h * d0.y};
+    const C id = {-d.y, d.x};
+    u[0] = cadd(u[0], t1);
+    u[1] = cadd(t2, id);
+    u[2] = csub(t2, id);
+}
+
+template <int S, typename C>
+__device__ __forceinline__ void dft4(C* u) {
+    const C t0 = cadd(u[0], u[2]);
+    const C t1 = csub(u[0], u[2]);
+    const C t2 = cadd(u[1], u[3]);
+    const C t3 = cmuli<S>(csub(u[1], u[3]));
+    u[0] = cadd(t0, t2);
+    u[2] = csub(t0, t2);
+    u[1] = cadd(t1, t3);
+    u[3] = csub(t1, t3);
+}
+
+template <int S, typename C>
+__device__ __forceinline__ void dft8(C* u) {
+    using T = decltype(u[0].x);
+    C e[4] = {u[0], u[2], u[4], u[6]};
+    C o[4] = {u[1], u[3], u[5], u[7]};
+    dft4<S>(e);
+    dft4<S>(o);
+    const T r = (T)0.70710678118654752440;
+    // w8^q for q = 1, 2, 3 with w8 = exp(S*2*pi*i/8)
+    const C o1 = {r * (o[1].x - (T)S * o[1].y), r * ((T)S * o[1].x + o[1].y)};
+    const C o2 = cmuli<S>(o[2]);
+    const C o3 = {r * (-o[3].x - (T)S * o[3].y), r * ((T)S * o[3].x - o[3].y)};
+    u[0] = cadd(e[0], o[0]);
+    u[4] = csub(e[0], o[0]);
+    u[1] = cadd(e[1], o1);
+    u[5] = csub(e[1], o1);
+    u[2] = cadd(e[2], o2);
+    u[6] = csub(e[2], o2);
+    u[3] = cadd(e[3], o3);
+    u[7] = csub(e[3], o3);
+}
+
+template <int R, int S, typename C>
+__device__ __forceinline__ void dftR(C* u) {
+    if constexpr (R == 2) dft2<S>(u);
+    else if constexpr (R == 3) dft3<S>(u);
+    else if constexpr (R == 4) dft4<S>(u);
+    else dft8<S>(u);
+}
+
+// One Stockham stage over all rows held in LDS.
+template <typename C, int LEN, int ROWS, int RS, int STAGE, int S>
+__device__ __forceinline__ void fft_stage(C* lds, const C* __restrict__ tw, int tw_stride,
+                                          int tid) {
+    using P = FftPlan<LEN>;
+    constexpr int R = P::radix(STAGE);
+    constexpr int p = P::prod_before(STAGE);
+    constexpr int M = LEN / R;
+    constexpr int NB = P::E / R;  // butterflies per thread
+    constexpr int TPR = P::TPR;
+    const int row = tid / TPR;
+    const int t = tid % TPR;
+    C* x = lds + row * RS;
+    C v[P::E];
+    if (row < ROWS) {
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int i = t + j * TPR;
+            const int k = i % p;
+            C u[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) u[r] = x[i + r * M];
+            if (k != 0) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    const int m = (r * k * (LEN / (p * R))) % LEN;
+                    C w = tw[m * tw_stride];
+                    if constexpr (S > 0) w.y = -w.y;
+                    u[r] = cmul(u[r], w);
+                }
+            }
+            dftR<R, S>(u);
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[j * R + r] = u[r];
+        }
+    }
+    __syncthreads();
+    if (row < ROWS) {
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int i = t + j * TPR;
+            const int k = i % p;
+            const int base = (i - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) x[base + r * p] = v[j * R + r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename C, int LEN, int ROWS, int RS, int S, int STAGE = 0>
+__device__ __forceinline__ void fft_rows(C* lds, const C* __restrict__ tw, int tw_stride,
+                                         int tid) {
+    if constexpr (STAGE < FftPlan<LEN>::stages()) {
+        fft_stage<C, LEN, ROWS, RS, STAGE, S>(lds, tw, tw_stride, tid);
+        fft_rows<C, LEN, ROWS, RS, S, STAGE + 1>(lds, tw, tw_stride, tid);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-pass launch geometry
+// ---------------------------------------------------------------------------
+constexpr int kFftLdsBudget = 80 * 1024;
+
+constexpr int pow2_floor(int v) {
+    int p = 1;
+    while (p * 2 <= v) p *= 2;
+    return p;
+}
+
+// Rows per workgroup for a transform of length LEN whose rows are blocked from
+// a dimension of extent DIM (ROWS must divide DIM).
+template <typename T, int LEN, int DIM>
+struct RowsCfg {
+    using C = cplx<T>;
+    static constexpr int RS = LEN + 1;
+    static constexpr int by_threads = 512 / FftPlan<LEN>::TPR;
+    static constexpr int by_lds = kFftLdsBudget / (RS * (int)sizeof(C));
+    static constexpr int cap = by_threads < by_lds ? by_threads : by_lds;
+    static constexpr int dim_pow2 = (DIM & (-DIM));  // largest power of two dividing DIM
+    static constexpr int ROWS0 = pow2_floor(cap < 1 ? 1 : cap);
+    static constexpr int ROWS = ROWS0 < dim_pow2 ? ROWS0 : dim_pow2;
+    static constexpr int THREADS = ROWS * FftPlan<LEN>::TPR;
+    static constexpr int LDS_BYTES = ROWS * RS * (int)sizeof(C);
+};
+
+}  // namespace hy

@@ -1,0 +1,876 @@
+// HIP kernels of the MI355X particle-mesh gravity step (gfx950, wave64).
+//
+// Stage map (reference hot path, file:line relative to the reference tree):
+//   k_bucket_key / k_scan_* / k_scatter  stable bucketing of particles that
+//        feeds the deterministic deposit (reference x-plane counting sort,
+//        src/mesh/mass_assignment_workspace.inc:264-374, generalised to
+//        tile buckets; determinism restored by k_deposit's index sort).
+//   k_deposit    CIC deposit + density source, summation order per cell equal
+//        to src/mesh/mass_assignment_deposit.inc:193-240 (bin X with weight
+//        1-f, then bin X-1 with weight f, particles in index order, oy outer,
+//        oz inner), density contrast as src/gravity/pm_solver.cpp:700-759.
+//   k_fft_*      r2c / Green / c2r Poisson solve (src/mesh/fft_backend.cpp,
+//        src/mesh/green_function.cpp:165-241).
+//   k_gather_kdk fused centered-gradient CIC gather
+//        (src/mesh/mass_assignment_gradient.cpp:117-206) with the K2 kick,
+//        the next K1 kick and the periodic drift
+//        (include/cosmo_nbody/time/particle_update.hpp:20-116).
+#pragma once
+
+#include "hy_device.hpp"
+#include "hy_fft.hpp"
+
+namespace hy {
+
+// ===========================================================================
+// Bucketing
+// ===========================================================================
+
+// Deposit bucket of a base cell: tile (edge T) and boundary class
+// (bit2/1/0: last cell of the tile along x/y/z).
+__device__ __forceinline__ uint32_t bucket_of(int bx, int by, int bz, const Geom& g) {
+    const int tx = bx / g.T, ty = by / g.T, tz = bz / g.T;
+    const uint32_t cls = ((uint32_t)(bx - tx * g.T == g.T - 1) << 2)
+                       | ((uint32_t)(by - ty * g.T == g.T - 1) << 1)
+                       | (uint32_t)(bz - tz * g.T == g.T - 1);
+    return (((uint32_t)tx * g.nT + ty) * g.nT + tz) * 8u + cls;
+}
+
+// Wave-aggregated integer atomic: lanes with equal keys reserve consecutive
+// slots with one atomic.  Returns this lane's rank (slot) within its bucket.
+__device__ __forceinline__ uint32_t aggregated_rank(uint32_t* counts, uint32_t key, bool valid) {
+    uint64_t active = __ballot(valid);
+    const uint32_t lane = lane_id();
+    uint32_t rank = 0;
+    while (active) {
+        const int leader = __ffsll((unsigned long long)active) - 1;
+        const uint32_t lead_key = __shfl(key, leader);
+        const uint64_t mask = __ballot(valid && key == lead_key) & active;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&counts[lead_key], (uint32_t)__popcll(mask));
+        base = __shfl(base, leader);
+        if ((mask >> lane) & 1ull) rank = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+        active &= ~mask;
+    }
+    return rank;
+}
+
+// mode 0: deposit buckets; mode 1: reference x-plane bins (key = base x).
+__global__ void k_bucket_key(uint64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                             const double* __restrict__ z, Geom g, int mode,
+                             uint32_t* __restrict__ keys, uint32_t* __restrict__ ranks,
+                             uint32_t* __restrict__ counts, uint32_t* __restrict__ err) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = p < n;
+    uint32_t key = 0;
+    bool ok = false;
+    if (in) {
+        int bx, by = 0, bz = 0;
+        double f;
+        ok = cic_axis(x[p], g, bx, f);
+        if (mode == 0) ok = ok && cic_axis(y[p], g, by, f) && cic_axis(z[p], g, bz, f);
+        if (ok) key = mode == 0 ? bucket_of(bx, by, bz, g) : (uint32_t)bx;
+        else atomicOr(err, ERR_POSITION_NONFINITE);
+    }
+    const uint32_t rank = aggregated_rank(counts, key, in && ok);
+    if (in && !ok) key = 0xffffffffu;
+    if (in) {
+        keys[p] = key;
+        ranks[p] = rank;
+    }
+}
+
+// Exclusive scan of counts[0..nb) into offsets[0..nb], three launches.
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
+    // Hillis-Steele over 256 threads in LDS (deterministic integer sums).
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < SCAN_THREADS; off <<= 1) {
+        const uint32_t add = t >= off ? sh[t - off] : 0u;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    total = sh[SCAN_THREADS - 1];
+    const uint32_t incl = sh[t];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void k_scan_reduce(const uint32_t* __restrict__ counts, uint64_t nb,
+                              uint32_t* __restrict__ block_sums) {
+    __shared__ uint32_t sh[SCAN_THREADS];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    uint32_t s = 0;
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        const uint64_t j = base + (uint64_t)i * SCAN_THREADS + threadIdx.x;
+        if (j < nb) s += counts[j];
+    }
+    uint32_t total;
+    (void)block_exclusive_scan(s, sh, total);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+__global__ void k_scan_blocks(uint32_t* __restrict__ block_sums, uint32_t nblocks,
+                              uint32_t* __restrict__ grand_total) {
+    __shared__ uint32_t sh[SCAN_THREADS];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nblocks; base += SCAN_THREADS) {
+        const uint32_t j = base + threadIdx.x;
+        const uint32_t v = j < nblocks ? block_sums[j] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan(v, sh, total);
+        if (j < nblocks) block_sums[j] = ex + carry;
+        carry += total;
+    }
+    if (threadIdx.x == 0) *grand_total = carry;
+}
+
+__global__ void k_scan_apply(const uint32_t* __restrict__ counts, uint64_t nb,
+                             const uint32_t* __restrict__ block_sums,
+                             const uint32_t* __restrict__ grand_total,
+                             uint32_t* __restrict__ offsets) {
+    __shared__ uint32_t sh[SCAN_THREADS];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        v[i] = (base + i < nb) ? counts[base + i] : 0u;
+        s += v[i];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(s, sh, total) + block_sums[blockIdx.x];
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        if (base + i < nb) offsets[base + i] = run;
+        run += v[i];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) offsets[nb] = *grand_total;
+}
+
+__global__ void k_scatter(uint64_t n, const uint32_t* __restrict__ keys,
+                          const uint32_t* __restrict__ ranks, const uint32_t* __restrict__ offsets,
+                          uint32_t* __restrict__ order) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) {
+        const uint32_t k = keys[p];
+        if (k != 0xffffffffu) order[offsets[k] + ranks[p]] = (uint32_t)p;
+    }
+}
+
+// ===========================================================================
+// Workgroup sort (ascending u64), arbitrary n, any address space.
+// Bitonic network in the "flip" form: every comparator is ascending, so
+// virtual +inf padding past n never has to move into range.
+// ===========================================================================
+template <typename P>
+__device__ __forceinline__ void wg_sort_u64(P keys, uint32_t n) {
+    if (n < 2) return;
+    uint32_t np2 = 1;
+    while (np2 < n) np2 <<= 1;
+    const uint32_t half = np2 >> 1;
+    for (uint32_t k = 2; k <= np2; k <<= 1) {
+        // flip step: compare i with its mirror inside the block of size k
+        for (uint32_t c = threadIdx.x; c < half; c += blockDim.x) {
+            const uint32_t blk = c / (k >> 1);
+            const uint32_t off = c % (k >> 1);
+            const uint32_t i = blk * k + off;
+            const uint32_t j = blk * k + k - 1 - off;
+            if (j < n) {
+                const uint64_t a = keys[i], b = keys[j];
+                if (a > b) { keys[i] = b; keys[j] = a; }
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+        for (uint32_t d = k >> 2; d >= 1; d >>= 1) {
+            for (uint32_t c = threadIdx.x; c < half; c += blockDim.x) {
+                const uint32_t i = (c / d) * 2 * d + (c % d);
+                const uint32_t j = i + d;
+                if (j < n) {
+                    const uint64_t a = keys[i], b = keys[j];
+                    if (a > b) { keys[i] = b; keys[j] = a; }
+                }
+            }
+            __threadfence_block();
+            __syncthreads();
+        }
+    }
+}
+
+// Sort each bucket's particle indices ascending (canonical stable order).
+// Small buckets sort in LDS, large ones in place in global memory.
+constexpr int CANON_THREADS = 256;
+constexpr int CANON_LDS = 4096;
+
+__global__ void __launch_bounds__(CANON_THREADS)
+k_canonicalize(uint64_t nb, const uint32_t* __restrict__ offsets, uint32_t* __restrict__ order,
+               uint64_t* __restrict__ scratch, uint64_t scratch_cap,
+               unsigned long long* __restrict__ scratch_cursor, uint32_t* __restrict__ err) {
+    __shared__ uint64_t sh[CANON_LDS];
+    for (uint64_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        const uint32_t s = offsets[b], e = offsets[b + 1];
+        const uint32_t n = e - s;
+        if (n < 2) continue;
+        if (n <= CANON_LDS) {
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) sh[i] = order[s + i];
+            __syncthreads();
+            wg_sort_u64(sh, n);
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) order[s + i] = (uint32_t)sh[i];
+            __syncthreads();
+        } else {
+            __shared__ uint64_t base_sh;
+            if (threadIdx.x == 0) base_sh = atomicAdd(scratch_cursor, (unsigned long long)n);
+            __syncthreads();
+            const uint64_t base = base_sh;
+            __syncthreads();
+            if (base + n > scratch_cap) {
+                if (threadIdx.x == 0) atomicOr(err, ERR_SCRATCH);
+                continue;
+            }
+            uint64_t* g = scratch + base;
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) g[i] = order[s + i];
+            __threadfence_block();
+            __syncthreads();
+            wg_sort_u64(g, n);
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) order[s + i] = (uint32_t)g[i];
+            __syncthreads();
+        }
+    }
+}
+
+// ===========================================================================
+// Deterministic CIC deposit + density source
+// ===========================================================================
+constexpr int DEP_THREADS = 256;
+constexpr int DEP_MAX_T = 16;
+constexpr int DEP_EXT = (DEP_MAX_T + 1) * (DEP_MAX_T + 1) * (DEP_MAX_T + 1);  // 4913
+constexpr int DEP_NRANGES = 20;
+
+template <typename T>
+struct DepCfg;
+template <>
+struct DepCfg<float> { static constexpr int CAP = 2048; };
+template <>
+struct DepCfg<double> { static constexpr int CAP = 1024; };
+
+struct DepParams {
+    Geom g;
+    const double* x;
+    const double* y;
+    const double* z;
+    const double* masses;   // nullptr -> uniform
+    double uniform_mass;
+    const uint32_t* offsets;  // bucket offsets (nb+1)
+    const uint32_t* order;    // bucketed particle indices
+    double mean_mass;         // M / N_cells
+    double inv_volume;        // 1 / dx^3
+    double total_mass;        // M (exact input mass)
+    double cells_real;        // N_cells as double
+    double centred_scale;     // inv_volume / N_cells (normalize path)
+    int direct;               // 1: (m - mean) * inv_volume ; 0: fma-centred path
+    double* tile_mass;        // per-tile deposited mass (binary64)
+    uint64_t* scratch;        // global fallback keys
+    uint64_t scratch_cap;
+    unsigned long long* scratch_cursor;
+    uint32_t* err;
+};
+
+// Key layout: ext cell (13 bits) << 51 | particle index (32 bits) << 19 | slot (19 bits)
+__device__ __forceinline__ uint32_t key_cell(uint64_t k) { return (uint32_t)(k >> 51); }
+__device__ __forceinline__ uint32_t key_index(uint64_t k) { return (uint32_t)((k >> 19) & 0xffffffffull); }
+__device__ __forceinline__ uint32_t key_slot(uint64_t k) { return (uint32_t)(k & 0x7ffffull); }
+
+template <typename T>
+__device__ __forceinline__ T contrast_of(T sum, const DepParams& P) {
+    const double m = (double)sum;
+    double v;
+    if (P.direct) {
+        v = (m - P.mean_mass) * P.inv_volume;
+    } else {
+        const double c = fma(m, P.cells_real, -P.total_mass);
+        v = c == 0.0 ? 0.0 : c * P.centred_scale;
+    }
+    return (T)v;
+}
+
+template <typename T, bool LDS_PAYLOAD, typename KP, typename FP>
+__device__ __forceinline__ void deposit_cells(const DepParams& P, T* __restrict__ mesh, int tx,
+                                              int ty, int tz, KP keys, uint32_t M,
+                                              const uint32_t* start, FP fx, FP fy, FP fz, FP fm,
+                                              double& mass_acc) {
+    const Geom& g = P.g;
+    const int Te = g.T + 1;
+    const int cells = g.T * g.T * g.T;
+    for (int c = threadIdx.x; c < cells; c += blockDim.x) {
+        const int lz = c % g.T, ly = (c / g.T) % g.T, lx = c / (g.T * g.T);
+        T sum = (T)0;
+        for (int phase = 0; phase < 2; ++phase) {
+            const int ex = lx + 1 - phase;  // phase 0: base x == cell x; phase 1: base x == cell x - 1
+            uint32_t cur[4], end[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int oy = q >> 1, oz = q & 1;
+                const int cell = (ex * Te + (ly + 1 - oy)) * Te + (lz + 1 - oz);
+                cur[q] = start[cell];
+                end[q] = start[cell + 1];
+            }
+            // merge the four source cells by particle index
+            while (true) {
+                int best = -1;
+                uint32_t best_idx = 0xffffffffu;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (cur[q] < end[q]) {
+                        const uint32_t idx = key_index(keys[cur[q]]);
+                        if (best < 0 || idx < best_idx) { best = q; best_idx = idx; }
+                    }
+                }
+                if (best < 0) break;
+                const uint64_t k = keys[cur[best]];
+                ++cur[best];
+                T ffx, ffy, ffz, mm;
+                if constexpr (LDS_PAYLOAD) {
+                    const uint32_t s = key_slot(k);
+                    ffx = fx[s]; ffy = fy[s]; ffz = fz[s];
+                    mm = P.masses ? fm[s] : (T)P.uniform_mass;
+                } else {
+                    const uint32_t pidx = key_index(k);
+                    int b; double f;
+                    cic_axis(P.x[pidx], g, b, f); ffx = (T)f;
+                    cic_axis(P.y[pidx], g, b, f); ffy = (T)f;
+                    cic_axis(P.z[pidx], g, b, f); ffz = (T)f;
+                    mm = P.masses ? (T)P.masses[pidx] : (T)P.uniform_mass;
+                }
+                const int oy = best >> 1, oz = best & 1;
+                const T xw = phase == 0 ? (T)1 - ffx : ffx;
+                const T wy = oy ? ffy : (T)1 - ffy;
+                const T wz = oz ? ffz : (T)1 - ffz;
+                sum += mm * xw * wy * wz;
+            }
+        }
+        mass_acc += (double)sum;
+        const uint64_t gx = (uint64_t)(tx * g.T + lx), gy = (uint64_t)(ty * g.T + ly),
+                       gz = (uint64_t)(tz * g.T + lz);
+        mesh[(gx * g.N + gy) * g.N + gz] = contrast_of(sum, P);
+    }
+}
+
+// Build start[0..Te^3] (prefix positions per ext cell) from sorted keys.
+template <typename KP>
+__device__ __forceinline__ void build_cell_starts(KP keys, uint32_t M, uint32_t* start, int ncell) {
+    for (uint32_t i = threadIdx.x; i <= M; i += blockDim.x) {
+        const int c = i < M ? (int)key_cell(keys[i]) : ncell;
+        const int prev = i == 0 ? -1 : (int)key_cell(keys[i - 1]);
+        for (int q = prev + 1; q <= c; ++q) start[q] = i;
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restrict__ mesh) {
+    using Cfg = DepCfg<T>;
+    constexpr int CAP = Cfg::CAP;
+    __shared__ uint64_t sh_keys[CAP];
+    __shared__ T sh_fx[CAP], sh_fy[CAP], sh_fz[CAP], sh_m[CAP];
+    __shared__ uint32_t sh_start[DEP_EXT + 1];
+    __shared__ uint32_t sh_rs[DEP_NRANGES], sh_rc[DEP_NRANGES + 1];
+    __shared__ uint8_t sh_rd[DEP_NRANGES];
+    __shared__ uint64_t sh_gbase;
+    __shared__ double sh_red[DEP_THREADS];
+
+    const Geom& g = P.g;
+    const int nT = g.nT;
+    const uint32_t ntiles = (uint32_t)nT * nT * nT;
+    const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;
+    const int tz = tile % nT, ty = (tile / nT) % nT, tx = tile / (nT * nT);
+    const int Te = g.T + 1;
+    const int ncell = Te * Te * Te;
+
+    // Source ranges: own tile (all classes) + boundary classes of the 7 lower
+    // neighbour tiles (offset d in {0,1}^3, classes c with (c & d) == d).
+    if (threadIdx.x == 0) {
+        int r = 0;
+        uint32_t acc = 0;
+        for (int d = 0; d < 8; ++d) {
+            const int dx = (d >> 2) & 1, dy = (d >> 1) & 1, dz = d & 1;
+            const int sx = (tx - dx + nT) % nT, sy = (ty - dy + nT) % nT, sz = (tz - dz + nT) % nT;
+            const uint32_t t8 = (((uint32_t)sx * nT + sy) * nT + sz) * 8u;
+            if (d == 0) {
+                sh_rs[r] = P.offsets[t8];
+                const uint32_t cnt = P.offsets[t8 + 8] - P.offsets[t8];
+                sh_rc[r] = acc; sh_rd[r] = 0; acc += cnt; ++r;
+            } else {
+                for (int c = 0; c < 8; ++c) {
+                    if ((c & d) != d) continue;
+                    sh_rs[r] = P.offsets[t8 + c];
+                    const uint32_t cnt = P.offsets[t8 + c + 1] - P.offsets[t8 + c];
+                    sh_rc[r] = acc; sh_rd[r] = (uint8_t)d; acc += cnt; ++r;
+                }
+            }
+        }
+        sh_rc[r] = acc;
+    }
+    __syncthreads();
+    const uint32_t M = sh_rc[DEP_NRANGES];
+    const bool use_lds = M <= (uint32_t)CAP;
+
+    uint64_t* gkeys = nullptr;
+    if (!use_lds) {
+        if (threadIdx.x == 0) sh_gbase = atomicAdd(P.scratch_cursor, (unsigned long long)M);
+        __syncthreads();
+        if (sh_gbase + M > P.scratch_cap) {
+            if (threadIdx.x == 0) atomicOr(P.err, ERR_SCRATCH);
+            return;
+        }
+        gkeys = P.scratch + sh_gbase;
+    }
+
+    // Load records: CIC coordinate per particle, ext cell relative to this tile.
+    for (uint32_t v = threadIdx.x; v < M; v += blockDim.x) {
+        int r = 0;
+        while (sh_rc[r + 1] <= v) ++r;
+        const uint32_t pidx = P.order[sh_rs[r] + (v - sh_rc[r])];
+        const int d = sh_rd[r];
+        int bx = 0, by = 0, bz = 0;
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        const bool ok = cic_axis(P.x[pidx], g, bx, fx) && cic_axis(P.y[pidx], g, by, fy)
+                        && cic_axis(P.z[pidx], g, bz, fz);
+        int ex = (d & 4) ? 0 : bx - tx * g.T + 1;
+        int ey = (d & 2) ? 0 : by - ty * g.T + 1;
+        int ez = (d & 1) ? 0 : bz - tz * g.T + 1;
+        if (!ok || ex < 0 || ex > g.T || ey < 0 || ey > g.T || ez < 0 || ez > g.T) {
+            atomicOr(P.err, ERR_COORDINATE);
+            ex = ex < 0 ? 0 : (ex > g.T ? g.T : ex);
+            ey = ey < 0 ? 0 : (ey > g.T ? g.T : ey);
+            ez = ez < 0 ? 0 : (ez > g.T ? g.T : ez);
+        }
+        const uint32_t cell = (uint32_t)((ex * Te + ey) * Te + ez);
+        const uint64_t key = ((uint64_t)cell << 51) | ((uint64_t)pidx << 19) | (use_lds ? v : 0u);
+        if (use_lds) {
+            sh_keys[v] = key;
+            sh_fx[v] = (T)fx; sh_fy[v] = (T)fy; sh_fz[v] = (T)fz;
+            if (P.masses) sh_m[v] = (T)P.masses[pidx];
+        } else {
+            gkeys[v] = key;
+        }
+    }
+    __threadfence_block();
+    __syncthreads();
+
+    double mass_acc = 0.0;
+    if (use_lds) {
+        wg_sort_u64(sh_keys, M);
+        build_cell_starts(sh_keys, M, sh_start, ncell);
+        deposit_cells<T, true>(P, mesh, tx, ty, tz, sh_keys, M, sh_start, sh_fx, sh_fy, sh_fz,
+                               sh_m, mass_acc);
+    } else {
+        wg_sort_u64(gkeys, M);
+        build_cell_starts(gkeys, M, sh_start, ncell);
+        deposit_cells<T, false>(P, mesh, tx, ty, tz, gkeys, M, sh_start, sh_fx, sh_fy, sh_fz,
+                                sh_m, mass_acc);
+    }
+
+    // Deterministic per-tile mass: fixed-order tree over threads.
+    sh_red[threadIdx.x] = mass_acc;
+    __syncthreads();
+    for (int s = DEP_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh_red[threadIdx.x] += sh_red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) P.tile_mass[tile] = sh_red[0];
+}
+
+template __global__ void k_deposit<float>(DepParams, float*);
+template __global__ void k_deposit<double>(DepParams, double*);
+
+// Fixed-order reduction of per-tile masses (single workgroup).
+__global__ void k_reduce_sum(const double* __restrict__ v, uint64_t n, double* __restrict__ out) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (uint64_t i = threadIdx.x; i < n; i += 256) s += v[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+// ===========================================================================
+// FFT passes.  Layouts (NH = N/2+1):
+//   R  [x][y][z]   real, N^3            (density source in, potential out)
+//   A  [x][kz][y]  complex, N*NH*N      (after the z pass)
+//   B  [kz][ky][x] complex, NH*N*N      (after the y pass; x pass in place)
+// ===========================================================================
+template <typename T>
+struct FftArgs {
+    T* R;
+    cplx<T>* A;
+    cplx<T>* B;
+    const cplx<T>* tw;   // exp(-2 pi i m / N), m < N
+    const T* green_lap;  // discrete Laplacian symbol per index (binary64 derived)
+    const T* green_win;  // CIC window per index (deconvolution)
+    double green_num;    // 4 pi G
+    double norm;         // 1 / N^3
+    int deconvolve;
+    uint32_t* err;
+};
+
+// z pass: real rows -> half spectrum, written transposed into A.
+template <typename T, int N>
+__global__ void k_fft_z_r2c(FftArgs<T> a) {
+    using C = cplx<T>;
+    constexpr int LEN = N / 2;
+    constexpr int NH = N / 2 + 1;
+    using RC = RowsCfg<T, LEN, N>;
+    constexpr int ROWS = RC::ROWS, RS = RC::RS;
+    extern __shared__ unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    const uint32_t nblk = gridDim.x;
+    const uint32_t b = xcd_block(blockIdx.x, nblk);
+    const int yb = b % (N / ROWS), x = b / (N / ROWS);
+    const int y0 = yb * ROWS;
+    const int tid = threadIdx.x;
+    for (int l = tid; l < ROWS * LEN; l += blockDim.x) {
+        const int r = l / LEN, m = l % LEN;
+        const C* src = reinterpret_cast<const C*>(a.R + ((uint64_t)x * N + y0 + r) * N);
+        lds[r * RS + m] = src[m];
+    }
+    __syncthreads();
+    fft_rows<C, LEN, ROWS, RS, -1>(lds, a.tw, 2, tid);
+    // untangle: X[k] = E + W^k O, E = (Z[k]+conj Z[L-k])/2, O = -i/2 (Z[k]-conj Z[L-k])
+    for (int l = tid; l < ROWS * NH; l += blockDim.x) {
+        const int r = l % ROWS, k = l / ROWS;
+        const C zk = lds[r * RS + (k % LEN)];
+        const C zc = cconj(lds[r * RS + ((LEN - k) % LEN)]);
+        const C e = {(T)0.5 * (zk.x + zc.x), (T)0.5 * (zk.y + zc.y)};
+        const C d = csub(zk, zc);
+        const C o = {(T)0.5 * d.y, (T)-0.5 * d.x};
+        const C w = a.tw[k];
+        const C X = cadd(e, cmul(w, o));
+        a.A[((uint64_t)x * NH + k) * N + y0 + r] = X;
+    }
+}
+
+// y pass forward: A rows over y -> B[kz][ky][x]
+template <typename T, int N>
+__global__ void k_fft_y_fwd(FftArgs<T> a) {
+    using C = cplx<T>;
+    constexpr int NH = N / 2 + 1;
+    using RC = RowsCfg<T, N, N>;
+    constexpr int ROWS = RC::ROWS, RS = RC::RS;
+    extern __shared__ unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const int xb = b % (N / ROWS), kz = b / (N / ROWS);
+    const int x0 = xb * ROWS;
+    const int tid = threadIdx.x;
+    for (int l = tid; l < ROWS * N; l += blockDim.x) {
+        const int r = l / N, y = l % N;
+        lds[r * RS + y] = a.A[((uint64_t)(x0 + r) * NH + kz) * N + y];
+    }
+    __syncthreads();
+    fft_rows<C, N, ROWS, RS, -1>(lds, a.tw, 1, tid);
+    for (int l = tid; l < ROWS * N; l += blockDim.x) {
+        const int r = l % ROWS, ky = l / ROWS;
+        a.B[((uint64_t)kz * N + ky) * N + x0 + r] = lds[r * RS + ky];
+    }
+}
+
+// x pass: forward x FFT, Green multiply (with 1/N^3), backward x FFT, in place on B.
+template <typename T, int N>
+__global__ void k_fft_x_green(FftArgs<T> a) {
+    using C = cplx<T>;
+    constexpr int NH = N / 2 + 1;
+    using RC = RowsCfg<T, N, NH * N>;
+    constexpr int ROWS = RC::ROWS, RS = RC::RS;
+    extern __shared__ unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const uint64_t q0 = (uint64_t)b * ROWS;  // row index q = kz*N + ky
+    const int tid = threadIdx.x;
+    for (int l = tid; l < ROWS * N; l += blockDim.x) {
+        const int r = l / N, xx = l % N;
+        lds[r * RS + xx] = a.B[(q0 + r) * N + xx];
+    }
+    __syncthreads();
+    fft_rows<C, N, ROWS, RS, -1>(lds, a.tw, 1, tid);
+    for (int l = tid; l < ROWS * N; l += blockDim.x) {
+        const int r = l / N, kx = l % N;
+        const uint64_t q = q0 + r;
+        const int kz = (int)(q / N), ky = (int)(q % N);
+        T f;
+        if (kx == 0 && ky == 0 && kz == 0) {
+            f = (T)0;
+        } else {
+            const T D = a.green_lap[kx] + a.green_lap[ky] + a.green_lap[kz];
+            T ff = (T)a.green_num / D;
+            if (a.deconvolve) {
+                const T w = a.green_win[kx] * a.green_win[ky] * a.green_win[kz];
+                ff /= (w * w);
+            }
+            f = ff * (T)a.norm;
+        }
+        C v = lds[r * RS + kx];
+        v.x *= f;
+        v.y *= f;
+        lds[r * RS + kx] = v;
+    }
+    __syncthreads();
+    fft_rows<C, N, ROWS, RS, +1>(lds, a.tw, 1, tid);
+    for (int l = tid; l < ROWS * N; l += blockDim.x) {
+        const int r = l / N, xx = l % N;
+        a.B[(q0 + r) * N + xx] = lds[r * RS + xx];
+    }
+}
+
+// y pass backward: B[kz][ky][x] -> A rows over y
+template <typename T, int N>
+__global__ void k_fft_y_bwd(FftArgs<T> a) {
+    using C = cplx<T>;
+    constexpr int NH = N / 2 + 1;
+    using RC = RowsCfg<T, N, N>;
+    constexpr int ROWS = RC::ROWS, RS = RC::RS;
+    extern __shared__ unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const int xb = b % (N / ROWS), kz = b / (N / ROWS);
+    const int x0 = xb * ROWS;
+    const int tid = threadIdx.x;
+    for (int l = tid; l < ROWS * N; l += blockDim.x) {
+        const int r = l % ROWS, ky = l / ROWS;
+        lds[r * RS + ky] = a.B[((uint64_t)kz * N + ky) * N + x0 + r];
+    }
+    __syncthreads();
+    fft_rows<C, N, ROWS, RS, +1>(lds, a.tw, 1, tid);
+    for (int l = tid; l < ROWS * N; l += blockDim.x) {
+        const int r = l / N, y = l % N;
+        a.A[((uint64_t)(x0 + r) * NH + kz) * N + y] = lds[r * RS + y];
+    }
+}
+
+// z pass backward (c2r): A[x][kz][y] -> R rows; flags a non-finite potential.
+template <typename T, int N>
+__global__ void k_fft_z_c2r(FftArgs<T> a) {
+    using C = cplx<T>;
+    constexpr int LEN = N / 2;
+    constexpr int NH = N / 2 + 1;
+    using RC = RowsCfg<T, LEN, N>;
+    constexpr int ROWS = RC::ROWS, RS = RC::RS;
+    extern __shared__ unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    __shared__ int bad;
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const int yb = b % (N / ROWS), x = b / (N / ROWS);
+    const int y0 = yb * ROWS;
+    const int tid = threadIdx.x;
+    if (tid == 0) bad = 0;
+    for (int l = tid; l < ROWS * NH; l += blockDim.x) {
+        const int r = l % ROWS, k = l / ROWS;
+        C v = a.A[((uint64_t)x * NH + k) * N + y0 + r];
+        if (k == 0 || k == LEN) v.y = (T)0;  // c2r reads only the independent half
+        lds[r * RS + k] = v;
+    }
+    __syncthreads();
+    // Z'[k] = (X[k] + conj X[L-k]) + i (X[k] - conj X[L-k]) W^-k
+    for (int l = tid; l < ROWS * (LEN / 2 + 1); l += blockDim.x) {
+        const int r = l % ROWS, k = l / ROWS;
+        const C xk = lds[r * RS + k];
+        const C xm = lds[r * RS + (LEN - k)];
+        const C wk = cconj(a.tw[k]);
+        const C s1 = cadd(xk, cconj(xm));
+        const C d1 = cmul(csub(xk, cconj(xm)), wk);
+        const C zk = {s1.x - d1.y, s1.y + d1.x};
+        C zm = {(T)0, (T)0};
+        const bool pair = k > 0 && (LEN - k) != k;
+        if (pair) {
+            const C wm = cconj(a.tw[LEN - k]);
+            const C s2 = cadd(xm, cconj(xk));
+            const C d2 = cmul(csub(xm, cconj(xk)), wm);
+            zm = {s2.x - d2.y, s2.y + d2.x};
+        }
+        lds[r * RS + k] = zk;
+        if (pair) lds[r * RS + (LEN - k)] = zm;
+    }
+    __syncthreads();
+    fft_rows<C, LEN, ROWS, RS, +1>(lds, a.tw, 2, tid);
+    int mybad = 0;
+    for (int l = tid; l < ROWS * LEN; l += blockDim.x) {
+        const int r = l / LEN, m = l % LEN;
+        const C v = lds[r * RS + m];
+        if (!isfinite(v.x) || !isfinite(v.y)) mybad = 1;
+        C* dst = reinterpret_cast<C*>(a.R + ((uint64_t)x * N + y0 + r) * N);
+        dst[m] = v;
+    }
+    if (mybad) bad = 1;
+    __syncthreads();
+    if (tid == 0 && bad) atomicOr(a.err, ERR_FIELD_NONFINITE);
+}
+
+// ===========================================================================
+// Fused centered-gradient gather + kick/drift
+// ===========================================================================
+template <typename T, typename PT>
+struct GatherArgs {
+    Geom g;
+    uint64_t n;
+    const T* phi;               // R after the c2r pass
+    const double* x_in;
+    const double* y_in;
+    const double* z_in;
+    double* x_out;
+    double* y_out;
+    double* z_out;
+    const PT* px_in;
+    const PT* py_in;
+    const PT* pz_in;
+    PT* px_out;
+    PT* py_out;
+    PT* pz_out;
+    double* acc_x;              // optional: acc += g (force mode)
+    double* acc_y;
+    double* acc_z;
+    T* g_x;                     // optional: g written (transactional force mode)
+    T* g_y;
+    T* g_z;
+    double k_prev;              // K2 of the previous step
+    double k_next;              // K1 of the next step
+    double d_next;              // drift factor of the next step
+    int has_prev;
+    int has_next;
+    uint32_t* err;
+};
+
+template <typename T, typename PT>
+__global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    const Geom& g = a.g;
+    const int N = g.N;
+    int ix, iy, iz;
+    double fx, fy, fz;
+    const double xp = a.x_in[p], yp = a.y_in[p], zp = a.z_in[p];
+    if (!(cic_axis(xp, g, ix, fx) && cic_axis(yp, g, iy, fy) && cic_axis(zp, g, iz, fz))) {
+        atomicOr(a.err, ERR_POSITION_NONFINITE);
+        return;
+    }
+    const T wx[2] = {(T)(1.0 - fx), (T)fx};
+    const T wy[2] = {(T)(1.0 - fy), (T)fy};
+    const T wz[2] = {(T)(1.0 - fz), (T)fz};
+    const T* phi = a.phi;
+    const uint64_t NN = (uint64_t)N * N;
+    auto at = [&](int cx, int cy, int cz) -> T {
+        return phi[(uint64_t)cx * NN + (uint64_t)cy * N + cz];
+    };
+    const T two_dx = (T)(2.0 * g.dx);
+    T gx = 0, gy = 0, gz = 0;
+#pragma unroll
+    for (int ox = 0; ox < 2; ++ox) {
+        const int cx = wrap_cell(ix + ox, N);
+        const int cxp = wrap_cell(cx + 1, N), cxm = wrap_cell(cx - 1, N);
+#pragma unroll
+        for (int oy = 0; oy < 2; ++oy) {
+            const int cy = wrap_cell(iy + oy, N);
+            const int cyp = wrap_cell(cy + 1, N), cym = wrap_cell(cy - 1, N);
+#pragma unroll
+            for (int oz = 0; oz < 2; ++oz) {
+                const int cz = wrap_cell(iz + oz, N);
+                const int czp = wrap_cell(cz + 1, N), czm = wrap_cell(cz - 1, N);
+                const T w = wx[ox] * wy[oy] * wz[oz];
+                const T dxv = at(cxp, cy, cz) - at(cxm, cy, cz);
+                const T dyv = at(cx, cyp, cz) - at(cx, cym, cz);
+                const T dzv = at(cx, cy, czp) - at(cx, cy, czm);
+                gx += w * (-dxv / two_dx);
+                gy += w * (-dyv / two_dx);
+                gz += w * (-dzv / two_dx);
+            }
+        }
+    }
+    if (!isfinite(gx) || !isfinite(gy) || !isfinite(gz)) {
+        atomicOr(a.err, ERR_FORCE_NONFINITE);
+        return;
+    }
+    if (a.acc_x) {
+        a.acc_x[p] += (double)gx;
+        a.acc_y[p] += (double)gy;
+        a.acc_z[p] += (double)gz;
+    }
+    if (a.g_x) {
+        a.g_x[p] = gx;
+        a.g_y[p] = gy;
+        a.g_z[p] = gz;
+    }
+    if (!a.has_prev && !a.has_next) return;
+    PT qx = a.px_in[p], qy = a.py_in[p], qz = a.pz_in[p];
+    if (!isfinite(qx) || !isfinite(qy) || !isfinite(qz)) {
+        atomicOr(a.err, ERR_MOMENTUM_NONFINITE);
+        return;
+    }
+    if (a.has_prev) {
+        const PT k = (PT)a.k_prev;
+        qx = qx + (PT)gx * k;
+        qy = qy + (PT)gy * k;
+        qz = qz + (PT)gz * k;
+    }
+    double nx = xp, ny = yp, nz = zp;
+    if (a.has_next) {
+        const PT k = (PT)a.k_next;
+        qx = qx + (PT)gx * k;
+        qy = qy + (PT)gy * k;
+        qz = qz + (PT)gz * k;
+        const double D = a.d_next;
+        const double L = g.L;
+        const double ddx = (double)qx * D, ddy = (double)qy * D, ddz = (double)qz * D;
+        nx = wrap_pos(wrap_pos(xp, L) + minimum_image(ddx, L), L);
+        ny = wrap_pos(wrap_pos(yp, L) + minimum_image(ddy, L), L);
+        nz = wrap_pos(wrap_pos(zp, L) + minimum_image(ddz, L), L);
+    }
+    if (!isfinite(qx) || !isfinite(qy) || !isfinite(qz) || !isfinite(nx) || !isfinite(ny)
+        || !isfinite(nz)) {
+        atomicOr(a.err, ERR_UPDATE_NONFINITE);
+        return;
+    }
+    a.px_out[p] = qx;
+    a.py_out[p] = qy;
+    a.pz_out[p] = qz;
+    a.x_out[p] = nx;
+    a.y_out[p] = ny;
+    a.z_out[p] = nz;
+}
+
+// Transactional commit for the reference-shaped force entry: acc += g.
+template <typename T>
+__global__ void k_commit_force(uint64_t n, const T* __restrict__ gx, const T* __restrict__ gy,
+                               const T* __restrict__ gz, double* __restrict__ ax,
+                               double* __restrict__ ay, double* __restrict__ az) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    ax[p] += (double)gx[p];
+    ay[p] += (double)gy[p];
+    az[p] += (double)gz[p];
+}
+
+// Finite checks used by the force entry before anything is published.
+__global__ void k_check_finite(uint64_t n, const double* __restrict__ a, const double* __restrict__ b,
+                               const double* __restrict__ c, uint32_t bit, uint32_t* __restrict__ err) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool bad = false;
+    if (p < n) bad = !isfinite(a[p]) || !isfinite(b[p]) || !isfinite(c[p]);
+    if (__ballot(bad) && lane_id() == 0) atomicOr(err, bit);
+}
+
+__global__ void k_check_mass(uint64_t n, const double* __restrict__ m, uint32_t* __restrict__ err) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool bad = false;
+    if (p < n) bad = !isfinite(m[p]) || m[p] < 0.0;
+    if (__ballot(bad) && lane_id() == 0) atomicOr(err, ERR_MASS_INVALID);
+}
+
+}  // namespace hy

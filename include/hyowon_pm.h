@@ -1,0 +1,167 @@
+/*
+ * hyowon_pm.h -- C ABI of the MI355X-native particle-mesh gravity step.
+ *
+ * libhyowon_pm.so implements HYOWON's pure-PM force refresh and kick-drift-kick
+ * particle update with hand-written HIP kernels for gfx950.  Signatures use
+ * plain pointers and sizes only.  Reference interfaces are cited as
+ * file:line relative to the HYOWON source tree.
+ *
+ * Conventions
+ *  - Return value: HY_OK (0) or an hy_status code; the message of the last
+ *    failure on the calling thread is returned by hy_last_error().  Codes map
+ *    one-to-one onto the reference's exception classes (invalid_argument,
+ *    logic_error, overflow_error, underflow_error, runtime_error).
+ *  - Particle arrays are SoA, binary64 positions in [0, L) (any finite value is
+ *    wrapped as the reference does).  Pointers are device pointers unless
+ *    HY_HOST_POINTERS is set.
+ *  - One context owns one HIP stream and its mesh workspace; calls on a
+ *    context must be serialised (as PMSolver, include/cosmo_nbody/mesh/
+ *    mass_assignment.hpp:44-45).  All device memory is allocated by
+ *    hy_pm_create; the force and step entry points allocate nothing (except the
+ *    deposit fallback scratch growing after an HY_ERUNTIME-free retry).
+ */
+#ifndef HYOWON_PM_H
+#define HYOWON_PM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HY_ABI_VERSION 1
+
+typedef enum {
+    HY_OK = 0,
+    HY_EINVAL = 1,      /* std::invalid_argument */
+    HY_ELOGIC = 2,      /* std::logic_error */
+    HY_EOVERFLOW = 3,   /* std::overflow_error */
+    HY_EUNDERFLOW = 4,  /* std::underflow_error */
+    HY_EDEVICE = 5,     /* HIP runtime failure */
+    HY_ERUNTIME = 6,    /* std::runtime_error */
+    HY_ENOMEM = 7       /* std::bad_alloc */
+} hy_status;
+
+typedef enum {
+    HY_PREC_FP64 = 0,        /* binary64 mesh, FFT, forces, momenta */
+    HY_PREC_FP32_MIXED = 1   /* binary32 mesh/FFT/forces/momenta, binary64 positions */
+} hy_precision;
+
+/* Flags */
+#define HY_HOST_POINTERS  0x1u  /* particle arrays live in host memory */
+#define HY_HAS_PREV       0x2u  /* hy_pm_kdk: apply K2 of the previous step */
+#define HY_HAS_NEXT       0x4u  /* hy_pm_kdk: apply K1 of the next step and drift */
+#define HY_ASYNC          0x8u  /* do not synchronise; errors are reported by hy_pm_check */
+
+typedef struct {
+    double box_size;        /* L (Mpc/h) */
+    uint64_t mesh_size;     /* N cells per axis: 2^a (8..1024) or 3*2^a (12..384) */
+    int deconvolve_cic;     /* divide the Green function by W(k)^2 */
+    int precision;          /* hy_precision */
+    int device;             /* HIP device ordinal */
+    void* stream;           /* hipStream_t to use, or NULL for a private stream */
+    uint64_t max_particles; /* capacity; particle counts above it are rejected */
+} hy_pm_desc;
+
+typedef struct {
+    uint64_t mesh_size;
+    uint64_t tile_edge;        /* deposit tile edge T (divides N, <= 16) */
+    uint64_t bucket_count;     /* (N/T)^3 * 8 */
+    uint64_t device_bytes;     /* device memory held by the context */
+    double last_input_mass;    /* exact input mass of the last deposit */
+    double last_deposited_mass;/* deposited mass of the last deposit */
+    int precision;
+} hy_pm_info;
+
+typedef struct hy_pm_ctx hy_pm_ctx;
+
+/* Replaces gravity::PMSolver::PMSolver + ensure_workspace
+ * (include/cosmo_nbody/gravity/pm_solver.hpp:53-57; src/gravity/pm_solver.cpp:388-492):
+ * validates geometry, allocates every device buffer up front. */
+int hy_pm_create(const hy_pm_desc* desc, hy_pm_ctx** out);
+int hy_pm_destroy(hy_pm_ctx* ctx);
+int hy_pm_get_info(const hy_pm_ctx* ctx, hy_pm_info* info);
+const char* hy_last_error(void);
+int hy_abi_version(void);
+
+/* Replaces gravity::PMSolver::compute_forces
+ * (include/cosmo_nbody/gravity/pm_solver.hpp:61-70; src/gravity/pm_solver.cpp:494-852):
+ * adds the pure-PM acceleration into ax/ay/az.  masses == NULL selects the
+ * uniform mass.  Transactional: on error ax/ay/az are untouched. */
+int hy_pm_force(hy_pm_ctx* ctx, uint64_t n, const double* x, const double* y, const double* z,
+                const double* masses, double uniform_mass, double* ax, double* ay, double* az,
+                uint32_t flags);
+
+/* One force refresh fused with the surrounding kicks and drift of
+ * time::LeapfrogIntegrator::step (src/time/leapfrog_integrator.cpp:143-279,
+ * include/cosmo_nbody/time/particle_update.hpp:20-116), device pointers only:
+ *   g = PM force at (x_in, y_in, z_in)
+ *   HY_HAS_PREV: p += g * k_prev          (K2 of the step that ends here)
+ *   HY_HAS_NEXT: p += g * k_next          (K1 of the step that starts here)
+ *                x = wrap(wrap(x) + minimum_image(p * d_next))
+ * Momenta are binary64 (HY_PREC_FP64) or binary32 (HY_PREC_FP32_MIXED).
+ * Outputs may alias inputs.  acc_x/y/z (optional, binary64) receive g. */
+int hy_pm_kdk(hy_pm_ctx* ctx, uint64_t n, const double* x_in, const double* y_in,
+              const double* z_in, const void* px_in, const void* py_in, const void* pz_in,
+              double* x_out, double* y_out, double* z_out, void* px_out, void* py_out,
+              void* pz_out, const double* masses, double uniform_mass, double k_prev,
+              double k_next, double d_next, double* acc_x, double* acc_y, double* acc_z,
+              uint32_t flags);
+
+/* Stage entry points (device pointers; mesh arrays use the context precision,
+ * row-major (ix*N+iy)*N+iz as include/cosmo_nbody/mesh/mesh_geometry.hpp:74-79). */
+
+/* Deposit + density source (src/mesh/mass_assignment_deposit.inc:15-334,
+ * src/gravity/pm_solver.cpp:649-759): writes (m - M/N_c)/dx^3 to mesh_out. */
+int hy_pm_density(hy_pm_ctx* ctx, uint64_t n, const double* x, const double* y, const double* z,
+                  const double* masses, double uniform_mass, void* mesh_out);
+
+/* Poisson solve of a density-source mesh: r2c, Green, c2r with 1/N^3
+ * (src/mesh/fft_backend.cpp:1088-1145, src/mesh/green_function.cpp:209-241). */
+int hy_pm_potential(hy_pm_ctx* ctx, const void* source_in, void* phi_out);
+
+/* Fused centered-gradient gather of a given potential, adding into ax/ay/az
+ * (src/mesh/mass_assignment_gradient.cpp:46-226). */
+int hy_pm_gather(hy_pm_ctx* ctx, uint64_t n, const double* x, const double* y, const double* z,
+                 const void* phi, double* ax, double* ay, double* az);
+
+/* Canonical deposit-bucket permutation: particles stably sorted by the
+ * bucket key ((tile, boundary class); tile edge from hy_pm_get_info).
+ * order has n entries, offsets bucket_count+1 entries (device, uint32). */
+int hy_pm_bucket_order(hy_pm_ctx* ctx, uint64_t n, const double* x, const double* y,
+                       const double* z, uint32_t* order, uint32_t* offsets);
+
+/* Reference x-plane bins: stable counting sort of particle indices by the
+ * wrapped base x-cell (src/mesh/mass_assignment_workspace.inc:264-374).
+ * order has n entries, offsets N+1 entries (device, uint32). */
+int hy_pm_xplane_order(hy_pm_ctx* ctx, uint64_t n, const double* x, uint32_t* order,
+                       uint32_t* offsets);
+
+/* Waits for outstanding work and reports errors raised by HY_ASYNC calls. */
+int hy_pm_check(hy_pm_ctx* ctx);
+
+/* Host scalars of the step (binary64, long-double quadrature).
+ * H(a): src/cosmology/cosmology_model.cpp:25-57,167-178.
+ * D = int da/(a^3 H), K = int da/(a^2 H):
+ *   src/cosmology/drift_kick_factors.cpp:28-141 with
+ *   include/cosmo_nbody/math/roundoff_resolved_gauss_legendre_16.hpp.
+ * Geometric mean: include/cosmo_nbody/math/exact_geometric_mean.hpp:168-275.
+ * Boundaries: src/time/time_stepper.cpp:194-276 (stops, exact final a). */
+double hy_hubble(double a, double omega_m, double omega_lambda);
+int hy_drift_factor(double a1, double a2, double omega_m, double omega_lambda, double* out);
+int hy_kick_factor(double a1, double a2, double omega_m, double omega_lambda, double* out);
+double hy_geometric_mean(double lhs, double rhs);
+int hy_time_boundaries(double a_start, double a_final, double delta_ln_a, const double* stops,
+                       uint64_t n_stops, double* out, uint64_t capacity, uint64_t* count);
+/* Correctly rounded exact sum of non-negative binary64 values
+ * (contract of include/cosmo_nbody/math/exact_positive_sum.hpp). */
+double hy_exact_sum(const double* values, uint64_t count);
+/* G in (Mpc/h)(km/s)^2/(1e10 Msun/h) and rho_crit0 (include/cosmo_nbody/cosmology/units.hpp:26-30). */
+double hy_gravitational_constant(void);
+double hy_rho_crit0(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,265 @@
+"""GPU parity: every stage of the HIP PM step against the CPU oracle, through the
+C ABI (libhyowon_pm).  Integer outputs (bucket / x-plane permutations) and the
+binary64 deposit and gather must be bit-exact; FFT-based outputs agree to
+binary64 roundoff (FP64) or to the stated binary32 tolerances (FP32)."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import clumpy_particles, face_particles, uniform_particles
+
+pytestmark = pytest.mark.gpu
+
+G_REL_FP64 = 1e-10     # binary64 force vs oracle (FFT roundoff, plan-dependent in the reference too)
+G_RMS_FP32 = 1e-4      # binary32 force RMS relative error vs binary64 (target of the build)
+
+
+def _dev(a, torch_device, dtype=None):
+    import torch
+    t = torch.as_tensor(np.ascontiguousarray(a), device=torch_device)
+    return t.to(dtype) if dtype is not None else t
+
+
+def _solver(L, N, prec, n, deconv=False):
+    import hyowon_amd as H
+    return H.PMSolver(L, N, deconvolve_cic=deconv, precision=prec, max_particles=n)
+
+
+CASES = [
+    ("uniform", 16, 32.0, 4096),
+    ("uniform", 32, 32.0, 20000),
+    ("clumpy", 48, 48.0, 30000),
+    ("uniform", 12, 24.0, 3000),
+    ("clumpy", 64, 64.0, 40000),
+]
+
+
+def _positions(kind, N, L, n, seed):
+    if kind == "uniform":
+        return uniform_particles(n, L, seed)
+    return clumpy_particles(n, L, seed, n_clumps=max(4, n // 800), sigma=0.5 * L / N)
+
+
+@pytest.mark.parametrize("kind,N,L,n", CASES)
+def test_bucket_order_bit_exact(gpu, oracle_mod, kind, N, L, n):
+    pos = _positions(kind, N, L, n, seed=11)
+    s = _solver(L, N, "fp32", n)
+    x, y, z = (_dev(p, gpu) for p in pos)
+    order, offsets = s.bucket_order(x, y, z)
+    T = int(s.info().tile_edge)
+    off_ref, ord_ref = oracle_mod.bucket_order(pos[0], pos[1], pos[2], L, N, T)
+    assert np.array_equal(offsets.cpu().numpy().astype(np.uint64), off_ref)
+    assert np.array_equal(order.cpu().numpy().astype(np.uint32), ord_ref)
+
+
+@pytest.mark.parametrize("kind,N,L,n", CASES + [("faces", 16, 32.0, 0)])
+def test_xplane_order_bit_exact(gpu, oracle_mod, kind, N, L, n):
+    pos = face_particles(N, L, 5) if kind == "faces" else _positions(kind, N, L, n, seed=12)
+    s = _solver(L, N, "fp64", pos.shape[1])
+    x = _dev(pos[0], gpu)
+    order, offsets = s.xplane_order(x)
+    off_ref, ord_ref = oracle_mod.xplane_order(pos[0], L, N)
+    assert np.array_equal(offsets.cpu().numpy().astype(np.uint64), off_ref)
+    assert np.array_equal(order.cpu().numpy().astype(np.uint32), ord_ref)
+
+
+@pytest.mark.parametrize("kind,N,L,n", CASES)
+def test_density_fp64_bit_exact(gpu, oracle_mod, kind, N, L, n):
+    pos = _positions(kind, N, L, n, seed=13)
+    m = 0.75
+    s = _solver(L, N, "fp64", n)
+    x, y, z = (_dev(p, gpu) for p in pos)
+    mesh = s.density(x, y, z, None, m).cpu().numpy()
+    dep = oracle_mod.deposit(pos[0], pos[1], pos[2], L, N, uniform_mass=m)
+    ref = oracle_mod.density_contrast(dep, L, N, m * n)
+    assert np.array_equal(mesh, ref), np.abs(mesh - ref).max()
+
+
+def test_density_fp64_faces_and_explicit_masses(gpu, oracle_mod):
+    N, L = 16, 32.0
+    pos = face_particles(N, L, 7)
+    n = pos.shape[1]
+    masses = np.random.default_rng(1).uniform(0.5, 2.0, n)
+    s = _solver(L, N, "fp64", n)
+    x, y, z = (_dev(p, gpu) for p in pos)
+    mesh = s.density(x, y, z, _dev(masses, gpu), 0.0).cpu().numpy()
+    dep = oracle_mod.deposit(pos[0], pos[1], pos[2], L, N, masses=masses)
+    total = float(oracle_mod.lib().or_exact_sum(masses.ctypes.data_as(oracle_mod._pd), n))
+    ref = oracle_mod.density_contrast(dep, L, N, total)
+    assert np.array_equal(mesh, ref)
+
+
+@pytest.mark.parametrize("kind,N,L,n", CASES[:3])
+def test_density_fp32(gpu, oracle_mod, kind, N, L, n):
+    pos = _positions(kind, N, L, n, seed=14)
+    m = 1.3
+    s = _solver(L, N, "fp32", n)
+    x, y, z = (_dev(p, gpu) for p in pos)
+    mesh = s.density(x, y, z, None, m).cpu().numpy().astype(np.float64)
+    dep = oracle_mod.deposit(pos[0], pos[1], pos[2], L, N, uniform_mass=m)
+    ref = oracle_mod.density_contrast(dep, L, N, m * n)
+    scale = np.abs(ref).max()
+    assert np.abs(mesh - ref).max() <= 1e-5 * scale
+
+
+@pytest.mark.parametrize("N", [8, 12, 16, 24, 32, 48, 64, 96, 128])
+@pytest.mark.parametrize("deconv", [False, True])
+def test_potential_fp64(gpu, oracle_mod, N, deconv):
+    L = 2.0 * N
+    rng = np.random.default_rng(N)
+    src = rng.standard_normal((N, N, N))
+    src -= src.mean()
+    s = _solver(L, N, "fp64", 1, deconv)
+    phi = s.potential(_dev(src, gpu)).cpu().numpy()
+    spec = oracle_mod.rfftn(src)
+    spec = oracle_mod.green_apply(spec, L, deconv)
+    ref = oracle_mod.irfftn(spec)
+    err = np.abs(phi - ref).max() / np.abs(ref).max()
+    assert err < 1e-12, err
+
+
+@pytest.mark.parametrize("N", [16, 48, 64, 128])
+def test_potential_fp32(gpu, oracle_mod, N):
+    L = 2.0 * N
+    rng = np.random.default_rng(N + 1)
+    src = rng.standard_normal((N, N, N))
+    s = _solver(L, N, "fp32", 1)
+    phi = s.potential(_dev(src, gpu)).cpu().numpy().astype(np.float64)
+    ref = oracle_mod.irfftn(oracle_mod.green_apply(oracle_mod.rfftn(src.astype(np.float32).astype(np.float64)), L))
+    err = np.sqrt(np.mean((phi - ref) ** 2) / np.mean(ref ** 2))
+    assert err < 2e-6, err
+
+
+@pytest.mark.parametrize("kind,N,L,n", CASES)
+def test_gather_fp64_bit_exact(gpu, oracle_mod, kind, N, L, n):
+    pos = _positions(kind, N, L, n, seed=15)
+    rng = np.random.default_rng(3)
+    phi = rng.standard_normal((N, N, N))
+    acc0 = rng.standard_normal((3, n))
+    s = _solver(L, N, "fp64", n)
+    import torch
+    acc = [_dev(a, gpu) for a in acc0]
+    x, y, z = (_dev(p, gpu) for p in pos)
+    s.gather(_dev(phi, gpu), x, y, z, *acc)
+    got = np.stack([a.cpu().numpy() for a in acc])
+    ref = oracle_mod.gather(phi, pos[0], pos[1], pos[2], L, acc=acc0)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("kind,N,L,n", CASES)
+def test_force_fp64(gpu, oracle_mod, kind, N, L, n):
+    pos = _positions(kind, N, L, n, seed=16)
+    m = 2.5
+    s = _solver(L, N, "fp64", n)
+    import torch
+    acc = [torch.zeros(n, dtype=torch.float64, device=gpu) for _ in range(3)]
+    x, y, z = (_dev(p, gpu) for p in pos)
+    s.compute_forces(x, y, z, None, m, *acc)
+    got = np.stack([a.cpu().numpy() for a in acc])
+    ref = oracle_mod.pm_force(pos[0], pos[1], pos[2], L, N, uniform_mass=m)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < G_REL_FP64, err
+
+
+@pytest.mark.parametrize("kind,N,L,n", CASES)
+def test_force_fp32_rms(gpu, oracle_mod, kind, N, L, n):
+    pos = _positions(kind, N, L, n, seed=17)
+    m = 2.5
+    s = _solver(L, N, "fp32", n)
+    import torch
+    acc = [torch.zeros(n, dtype=torch.float64, device=gpu) for _ in range(3)]
+    x, y, z = (_dev(p, gpu) for p in pos)
+    s.compute_forces(x, y, z, None, m, *acc)
+    got = np.stack([a.cpu().numpy() for a in acc])
+    ref = oracle_mod.pm_force(pos[0], pos[1], pos[2], L, N, uniform_mass=m)
+    rms = math.sqrt(((got - ref) ** 2).sum() / (ref ** 2).sum())
+    assert rms < G_RMS_FP32, rms
+
+
+def test_force_host_pointers_and_accumulate(gpu, oracle_mod):
+    N, L, n = 32, 32.0, 10000
+    pos = uniform_particles(n, L, 21)
+    s = _solver(L, N, "fp64", n)
+    acc = np.random.default_rng(2).standard_normal((3, n))
+    a0 = acc.copy()
+    s.compute_forces(pos[0].copy(), pos[1].copy(), pos[2].copy(), None, 1.0, acc[0], acc[1], acc[2])
+    ref = oracle_mod.pm_force(pos[0], pos[1], pos[2], L, N, uniform_mass=1.0)
+    assert np.abs((acc - a0) - ref).max() <= G_REL_FP64 * np.abs(ref).max()
+
+
+def test_force_deterministic(gpu):
+    import torch
+    N, L, n = 64, 64.0, 60000
+    pos = clumpy_particles(n, L, 31, n_clumps=20, sigma=0.5)
+    for prec in ("fp32", "fp64"):
+        s = _solver(L, N, prec, n)
+        x, y, z = (_dev(p, gpu) for p in pos)
+        outs = []
+        for _ in range(2):
+            acc = [torch.zeros(n, dtype=torch.float64, device=gpu) for _ in range(3)]
+            s.compute_forces(x, y, z, None, 1.0, *acc)
+            outs.append(torch.stack(acc).cpu().numpy())
+        assert np.array_equal(outs[0], outs[1])
+
+
+def test_errors_map_to_reference_exceptions(gpu):
+    import torch
+    import hyowon_amd as H
+    from hyowon_amd._lib import HyInvalidArgument, HyRuntimeError
+    N, L, n = 16, 16.0, 100
+    s = _solver(L, N, "fp64", n)
+    pos = uniform_particles(n, L, 1)
+    pos[1, 7] = np.nan
+    x, y, z = (_dev(p, gpu) for p in pos)
+    acc = [torch.zeros(n, dtype=torch.float64, device=gpu) for _ in range(3)]
+    with pytest.raises(HyInvalidArgument, match="finite"):
+        s.compute_forces(x, y, z, None, 1.0, *acc)
+    assert all(float(a.abs().sum()) == 0.0 for a in acc)  # transactional
+    pos = uniform_particles(n, L, 1)
+    x, y, z = (_dev(p, gpu) for p in pos)
+    with pytest.raises(HyInvalidArgument):
+        s.compute_forces(x, y, z, None, -1.0, *acc)
+    masses = np.ones(n)
+    masses[3] = -2.0
+    with pytest.raises(HyInvalidArgument):
+        s.compute_forces(x, y, z, _dev(masses, gpu), 0.0, *acc)
+    with pytest.raises(HyInvalidArgument):
+        H.PMSolver(L, 10, precision="fp64", max_particles=4)  # unsupported mesh size
+    with pytest.raises(HyInvalidArgument):
+        big = _solver(L, N, "fp64", 10)
+        big.compute_forces(x, y, z, None, 1.0, *acc)  # exceeds capacity
+    with pytest.raises(HyRuntimeError):
+        s.compute_forces(x[:0], y[:0], z[:0], None, 1.0, acc[0][:0], acc[1][:0], acc[2][:0])
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32", 2e-3)])
+def test_kdk_run_matches_oracle(gpu, oracle_mod, prec, tol):
+    """Short evolution z=9->4 through the fused KDK path vs the oracle step loop."""
+    import hyowon_amd as H
+    N, L = 32, 64.0
+    ng = 16
+    q = (np.arange(ng) + 0.5) * (L / ng)
+    X, Y, Z = np.meshgrid(q, q, q, indexing="ij")
+    rng = np.random.default_rng(4)
+    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()]) + rng.normal(0, 0.3, (3, ng ** 3))
+    pos = np.mod(pos, L)
+    mom = rng.normal(0, 5.0, (3, ng ** 3))
+    n = ng ** 3
+    m = H.rho_crit0() * 0.3 * (L / ng) ** 3
+    a0, a1, nsteps = 0.1, 0.2, 4
+    stepper = H.TimeStepper(a0, a1, math.log(a1 / a0) / nsteps, [a1])
+    b = stepper.boundaries
+    s = _solver(L, N, prec, n)
+    st = H.ParticleState.from_numpy(pos, mom, precision=prec)
+    integ = H.LeapfrogIntegrator(s, H.DriftKickIntegrator(0.3, 0.7), m)
+    st = integ.run(st, stepper)
+    gpos, gmom = st.to_numpy()
+    (rx, ry, rz, rpx, rpy, rpz), _ = oracle_mod.run(pos[0], pos[1], pos[2], mom[0], mom[1], mom[2],
+                                                    L, N, b, m)
+    rpos = np.stack([rx, ry, rz])
+    rmom = np.stack([rpx, rpy, rpz])
+    d = np.abs(gpos - rpos)
+    d = np.minimum(d, L - d)
+    assert d.max() / (L / N) < tol
+    assert np.abs(gmom - rmom).max() / np.abs(rmom).max() < tol

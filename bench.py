@@ -1,0 +1,227 @@
+"""Benchmark: particle-updates/s of the MI355X PM KDK step.
+
+Workload (default): 512^3 particles from synthetic 1LPT initial conditions
+(BBKS spectrum, sigma_8 = 0.8, z = 49), 1024^3 CIC mesh, box 512 Mpc/h,
+log-a KDK schedule z = 49 -> 0 in 64 steps.  One "step" is one force refresh
+fused with the K2 / K1 kicks and the drift (hy_pm_kdk), i.e. one KDK step of
+the reference's step loop.  Binary32 mesh/FFT/forces/momenta, binary64
+positions and time integrals.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line (metric, value, roofline, cpu_baseline, ...).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+
+STAGES = ["bucket", "deposit", "fft_z_r2c", "fft_y_fwd", "fft_x_green", "fft_y_bwd",
+          "fft_z_c2r", "gather_kdk"]
+
+
+def stage_bytes(stage: str, n_p: int, N: int, real: int) -> float:
+    """Algorithmic HBM bytes per launch of each stage (real = bytes per real)."""
+    M = N ** 3
+    NH = N // 2 + 1
+    half = N * N * NH * 2 * real  # one half-spectrum array
+    pos = 24.0 * n_p
+    mom = 3.0 * real * n_p
+    return {
+        "bucket": pos + 20.0 * n_p,                 # positions + key/rank write/read + index write
+        "deposit": pos + real * M,                  # positions re-read + mesh write
+        "fft_z_r2c": real * M + half,               # real mesh in, half spectrum out
+        "fft_y_fwd": 2.0 * half,
+        "fft_x_green": 2.0 * half,
+        "fft_y_bwd": 2.0 * half,
+        "fft_z_c2r": half + real * M,
+        "gather_kdk": 2.0 * (pos + mom) + real * M,  # state in/out + one potential read
+    }[stage]
+
+
+def step_alg_bytes(n_p: int, N: int, real: int) -> float:
+    """Compulsory bytes of one KDK step: FP32-mixed 96*N_p + 24*M, FP64 120*N_p + 48*M."""
+    M = N ** 3
+    return (96.0 * n_p + 24.0 * M) if real == 4 else (120.0 * n_p + 48.0 * M)
+
+
+def run_gpu(args):
+    import numpy as np
+    import torch
+
+    import hyowon_amd as H
+    from hyowon_amd import ic
+    from hyowon_amd._lib import lib
+
+    n_side, N, L = args.n_side, args.mesh, args.box
+    n = n_side ** 3
+    a0, a1, nsteps = 1.0 / (1.0 + args.z_start), 1.0, args.schedule_steps
+    stepper = H.TimeStepper(a0, a1, math.log(a1 / a0) / nsteps, [a1])
+    b = stepper.boundaries
+    dk = H.DriftKickIntegrator(0.3, 0.7)
+    K1, K2, D = [], [], []
+    for i in range(len(b) - 1):
+        ah = H.geometric_mean(b[i], b[i + 1])
+        K1.append(dk.kick_factor(b[i], ah))
+        K2.append(dk.kick_factor(ah, b[i + 1]))
+        D.append(dk.drift_factor(b[i], b[i + 1]))
+    m = ic.particle_mass(n_side, L)
+
+    dev = torch.device("cuda:0")
+    pos, mom = ic.lpt_ics(n_side, L, a0, seed=1, device=dev)
+    results = {}
+    for prec in args.precisions:
+        mdt = torch.float32 if prec == "fp32" else torch.float64
+        st = H.ParticleState(pos[0].clone(), pos[1].clone(), pos[2].clone(),
+                             mom[0].to(mdt).contiguous(), mom[1].to(mdt).contiguous(),
+                             mom[2].to(mdt).contiguous())
+        sp = st.empty_like()
+        solver = H.PMSolver(L, N, precision=prec, max_particles=n)
+        real = 4 if prec == "fp32" else 8
+        steps = args.steps if prec == args.precisions[0] else max(2, args.fp64_steps)
+        warm = args.warmup if prec == args.precisions[0] else 1
+
+        def call(i, asynchronous):
+            nonlocal st, sp
+            j = i % nsteps
+            kp = K2[(i - 1) % nsteps] if i > 0 else None
+            solver.kdk(st, sp, m, kp, K1[j], D[j], asynchronous=asynchronous)
+            st, sp = sp, st
+
+        # warmup: initial force + kick/drift, then steady steps (checked synchronously)
+        for i in range(warm):
+            call(i, asynchronous=False)
+        torch.cuda.synchronize()
+        lib().hy_pm_profile(solver.handle, 1, steps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(warm, warm + steps):
+            call(i, asynchronous=True)
+        solver.check()  # synchronises the context stream, raises on any kernel error
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ms = (C.c_double * 8)()
+        cnt = (C.c_uint64 * 8)()
+        calls = C.c_uint64()
+        lib().hy_pm_profile_read(solver.handle, ms, cnt, C.byref(calls))
+        lib().hy_pm_profile(solver.handle, 0, 0)
+        stage_ms = {STAGES[k]: ms[k] / max(cnt[k], 1) for k in range(8)}
+        step_ms = (t1 - t0) * 1e3 / steps
+        results[prec] = dict(step_ms=step_ms, stage_ms=stage_ms, real=real, steps=steps)
+        info = solver.info()
+        results[prec]["device_bytes"] = int(info.device_bytes)
+        solver.close()
+        del st, sp
+        torch.cuda.empty_cache()
+    return n, N, results
+
+
+def cpu_baseline(args):
+    """The oracle (binary64 C, OpenMP over all host cores) on a bounded sample of
+    the same workload: one force refresh + K1/drift + K2 of a 128^3-particle,
+    256^3-mesh box at the same particle and mesh spacing as the GPU workload."""
+    import numpy as np
+
+    import oracle
+    from hyowon_amd import ic
+    oracle.build(ref=False) if not os.path.exists(oracle.LIB_PATH) else None
+    cores = os.cpu_count() or 1
+    oracle.set_threads(cores)
+    ns = args.cpu_sample_side
+    Ns = ns * args.mesh // args.n_side
+    Ls = args.box * ns / args.n_side
+    a0 = 1.0 / (1.0 + args.z_start)
+    pos, mom = ic.lpt_ics(ns, Ls, a0, seed=1, device="cpu")
+    pos = [p.numpy() for p in pos]
+    mom = [p.numpy() for p in mom]
+    m = ic.particle_mass(ns, Ls)
+    b = np.array([a0, a0 * math.exp(math.log(1.0 / a0) / args.schedule_steps)])
+    t0 = time.perf_counter()
+    oracle.run(pos[0], pos[1], pos[2], mom[0], mom[1], mom[2], Ls, Ns, b, m)
+    t1 = time.perf_counter()
+    # the run does 2 force refreshes for 1 step; a steady step has 1 refresh
+    step_s = (t1 - t0) / 2.0
+    return {"value": ns ** 3 / step_s, "unit": "particle-updates/s", "cores": cores,
+            "kind": "port",
+            "sample": f"oracle (binary64 C/OpenMP) one KDK step of {ns}^3 particles on a "
+                      f"{Ns}^3 mesh, box {Ls:g} Mpc/h (same spacing as the GPU workload); "
+                      f"time of 2 force refreshes + kicks + drift halved; {t1 - t0:.1f} s total"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n-side", type=int, default=512)
+    ap.add_argument("--mesh", type=int, default=1024)
+    ap.add_argument("--box", type=float, default=512.0)
+    ap.add_argument("--z-start", type=float, default=49.0)
+    ap.add_argument("--schedule-steps", type=int, default=64)
+    ap.add_argument("--fp64-steps", type=int, default=3)
+    ap.add_argument("--precisions", default="fp32,fp64")
+    ap.add_argument("--cpu-sample-side", type=int, default=128)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    args.precisions = [p for p in args.precisions.split(",") if p]
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the slab decomposition is described in DESIGN.md); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    n, N, res = run_gpu(args)
+    main_prec = args.precisions[0]
+    r = res[main_prec]
+    value = n / (r["step_ms"] * 1e-3)
+    stage_ms = r["stage_ms"]
+    dom = max(stage_ms, key=stage_ms.get)
+    ach = stage_bytes(dom, n, N, r["real"]) / (stage_ms[dom] * 1e-3)
+    roofline = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": ach / HBM_PEAK, "traffic": None, "kernel": dom,
+                "kernel_ms": stage_ms[dom],
+                "alg_bytes_per_launch": stage_bytes(dom, n, N, r["real"])}
+    out = {
+        "metric": "particle-updates/s per PM KDK step (512^3 particles, 1024^3 mesh)",
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": r["steps"],
+        "warmup": args.warmup,
+        "ms_per_step": r["step_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 mesh/FFT/forces/momenta, f64 positions" if main_prec == "fp32" else "f64",
+        "data": "synthetic 1LPT ICs (BBKS, sigma8=0.8, z=49, seed 1)",
+        "config": {"workload": "PM KDK step, config 2: 512^3 particles, 1024^3 CIC mesh, "
+                               "L=512 Mpc/h, 64 log-a steps z=49->0",
+                   "particles": n, "mesh": N, "box": args.box, "precision": main_prec},
+        "roofline": roofline,
+        "step_hbm_frac": step_alg_bytes(n, N, r["real"]) / (r["step_ms"] * 1e-3) / HBM_PEAK,
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+    }
+    if "fp64" in res and main_prec != "fp64":
+        r64 = res["fp64"]
+        out["fp64"] = {"ms_per_step": r64["step_ms"], "value": n / (r64["step_ms"] * 1e-3),
+                       "speedup_fp32_vs_fp64": r64["step_ms"] / r["step_ms"],
+                       "step_hbm_frac": step_alg_bytes(n, N, 8) / (r64["step_ms"] * 1e-3) / HBM_PEAK,
+                       "stage_ms": {k: round(v, 4) for k, v in r64["stage_ms"].items()}}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -11,6 +11,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <limits>
@@ -167,6 +168,13 @@ struct hy_pm_ctx {
 
     std::vector<void*> allocations;
 
+    // Live per-stage timing: kHyMarks events per call, recorded on `stream`.
+    static constexpr int kHyMarks = HY_STAGE_COUNT + 1;
+    bool prof = false;
+    uint64_t prof_cap = 0, prof_calls = 0;
+    std::vector<hipEvent_t> prof_ev;
+    std::vector<uint8_t> prof_used;
+
     void* alloc(size_t bytes) {
         void* p = nullptr;
         hip_check(hipMalloc(&p, bytes ? bytes : 16), "hipMalloc");
@@ -175,6 +183,7 @@ struct hy_pm_ctx {
         return p;
     }
     ~hy_pm_ctx() {
+        for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
         for (void* p : allocations) (void)hipFree(p);
         if (hstatus) (void)hipHostFree(hstatus);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -186,6 +195,19 @@ namespace {
 void set_device(const hy_pm_ctx* c) { hip_check(hipSetDevice(c->desc.device), "hipSetDevice"); }
 
 void launch_check(const char* what) { hip_check(hipGetLastError(), what); }
+
+// Record stage boundary `mark` (0 = call start, k = end of stage k-1) of the
+// current profiled call.  No-op when profiling is off or the ring is full.
+void mark(hy_pm_ctx* c, int m) {
+    if (!c->prof || c->prof_calls >= c->prof_cap) return;
+    const uint64_t slot = c->prof_calls * hy_pm_ctx::kHyMarks + (uint64_t)m;
+    hip_check(hipEventRecord(c->prof_ev[slot], c->stream), "hipEventRecord");
+    c->prof_used[slot] = 1;
+}
+
+void end_call(hy_pm_ctx* c) {
+    if (c->prof && c->prof_calls < c->prof_cap) ++c->prof_calls;
+}
 
 // Map the kernel error word onto the reference's exceptions.
 void raise_errors(uint32_t e) {
@@ -320,14 +342,19 @@ void launch_fft(hy_pm_ctx* c, const FftKernels<T>& K) {
     const int N = c->N, NH = N / 2 + 1;
     hipLaunchKernelGGL(K.z_r2c, dim3(N * (N / K.rows_z)), dim3(K.threads_z), K.lds_z, c->stream, a);
     launch_check("k_fft_z_r2c");
+    mark(c, HY_STAGE_FFT_Z + 1);
     hipLaunchKernelGGL(K.y_fwd, dim3(NH * (N / K.rows_y)), dim3(K.threads_y), K.lds_y, c->stream, a);
     launch_check("k_fft_y_fwd");
+    mark(c, HY_STAGE_FFT_Y + 1);
     hipLaunchKernelGGL(K.x_green, dim3((uint64_t)NH * N / K.rows_x), dim3(K.threads_x), K.lds_x, c->stream, a);
     launch_check("k_fft_x_green");
+    mark(c, HY_STAGE_FFT_X_GREEN + 1);
     hipLaunchKernelGGL(K.y_bwd, dim3(NH * (N / K.rows_y)), dim3(K.threads_y), K.lds_y, c->stream, a);
     launch_check("k_fft_y_bwd");
+    mark(c, HY_STAGE_FFT_YINV + 1);
     hipLaunchKernelGGL(K.z_c2r, dim3(N * (N / K.rows_z)), dim3(K.threads_z), K.lds_z, c->stream, a);
     launch_check("k_fft_z_c2r");
+    mark(c, HY_STAGE_FFT_ZINV + 1);
 }
 
 void run_fft(hy_pm_ctx* c) {
@@ -361,9 +388,12 @@ void check_mass(hy_pm_ctx* c, double input, double deposited, uint64_t n) {
 // Deposit stage: bucketize, deposit (+ retry with a larger fallback scratch).
 void run_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z,
                  const double* masses, double uniform, double Min, bool sync) {
+    mark(c, 0);
     bucketize(c, n, x, y, z, 0, c->counts, c->offsets, c->nb, c->order);
+    mark(c, HY_STAGE_BUCKET + 1);
     auto plan = plan_deposit(c, n, x, y, z, masses, uniform, Min);
     launch_deposit(c, plan.P);
+    mark(c, HY_STAGE_DEPOSIT + 1);
     c->last_input_mass = Min;
     if (!sync) return;
     for (int attempt = 0;; ++attempt) {
@@ -674,6 +704,8 @@ int hy_pm_force(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, cons
             a.g_x = (float*)c->gx; a.g_y = (float*)c->gy; a.g_z = (float*)c->gz;
             launch_gather(c, a);
         }
+        mark(c, HY_STAGE_GATHER_KDK + 1);
+        end_call(c);
         const auto st = fetch_status(c);
         raise_errors(st.err);
         // commit: acc += g (only after every check passed)
@@ -751,6 +783,8 @@ int hy_pm_kdk(hy_pm_ctx* c, uint64_t n, const double* x_in, const double* y_in, 
             a.px_out = (float*)px_out; a.py_out = (float*)py_out; a.pz_out = (float*)pz_out;
             launch_gather(c, a);
         }
+        mark(c, HY_STAGE_GATHER_KDK + 1);
+        end_call(c);
         if (sync) {
             const auto st = fetch_status(c);
             raise_errors(st.err);
@@ -788,6 +822,50 @@ int hy_pm_xplane_order(hy_pm_ctx* c, uint64_t n, const double* x, uint32_t* orde
         const auto st = fetch_status(c);
         raise_errors(st.err);
         if (st.total != n) fail(HY_ELOGIC, "CIC x-plane binning does not cover the complete particle population");
+    });
+}
+
+int hy_pm_profile(hy_pm_ctx* c, int enable, uint64_t max_calls) {
+    return guarded([&] {
+        if (!c) fail(HY_EINVAL, "null context");
+        set_device(c);
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+        c->prof_ev.clear();
+        c->prof_used.clear();
+        c->prof = enable != 0 && max_calls > 0;
+        c->prof_cap = c->prof ? max_calls : 0;
+        c->prof_calls = 0;
+        if (!c->prof) return;
+        c->prof_ev.resize(max_calls * hy_pm_ctx::kHyMarks);
+        c->prof_used.assign(c->prof_ev.size(), 0);
+        for (auto& e : c->prof_ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+    });
+}
+
+int hy_pm_profile_read(hy_pm_ctx* c, double* stage_ms, uint64_t* stage_count, uint64_t* calls) {
+    return guarded([&] {
+        if (!c || !stage_ms || !stage_count) fail(HY_EINVAL, "null argument");
+        set_device(c);
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        for (int k = 0; k < HY_STAGE_COUNT; ++k) { stage_ms[k] = 0.0; stage_count[k] = 0; }
+        for (uint64_t call = 0; call < c->prof_calls; ++call) {
+            const uint64_t base = call * hy_pm_ctx::kHyMarks;
+            for (int k = 0; k < HY_STAGE_COUNT; ++k) {
+                // stage k spans the latest recorded mark <= k to mark k+1
+                if (!c->prof_used[base + k + 1]) continue;
+                int s0 = k;
+                while (s0 >= 0 && !c->prof_used[base + s0]) --s0;
+                if (s0 < 0) continue;
+                float ms = 0.0f;
+                hip_check(hipEventElapsedTime(&ms, c->prof_ev[base + s0], c->prof_ev[base + k + 1]), "hipEventElapsedTime");
+                stage_ms[k] += ms;
+                stage_count[k] += 1;
+            }
+        }
+        if (calls) *calls = c->prof_calls;
+        c->prof_calls = 0;
+        std::fill(c->prof_used.begin(), c->prof_used.end(), 0);
     });
 }
 

@@ -1,0 +1,134 @@
+"""Synthetic 1LPT initial conditions for benchmarks and tests.
+
+Builds the workload the PM step is measured on: a corner lattice of particles
+(index (i*N+j)*N+k, q = (i,j,k)*L/N; src/ic/initial_conditions.cpp:296-312)
+displaced by the first-order LPT field of a Gaussian density with the BBKS
+spectrum P(k) = A k^n_s T(k)^2, T from Bardeen et al. (1986) with q = k/Gamma,
+normalised to sigma_8 at z=0 and scaled by the linear growth D1(a).
+Momenta p = a^2 H(a) f(a) psi (src/ic/lpt_displacement.cpp:301-380).
+
+The white noise comes from torch's seeded Gaussian generator, not from the
+reference's Philox mode-keyed generator, so the realisations differ from
+HYOWON's for the same seed; the statistics and the path being timed do not.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+from scipy import integrate
+
+
+@dataclass
+class Cosmology:
+    omega_m: float = 0.3
+    omega_lambda: float = 0.7
+    h: float = 0.7
+    sigma8: float = 0.8
+    n_s: float = 0.96
+    gamma: float = 0.21
+
+    def E(self, a):
+        return np.sqrt(self.omega_m / a ** 3 + self.omega_lambda)
+
+    def transfer(self, k):
+        q = np.asarray(k, dtype=np.float64) / self.gamma
+        out = np.ones_like(q)
+        nz = q > 0
+        qq = q[nz]
+        out[nz] = (np.log1p(2.34 * qq) / (2.34 * qq)
+                   * (1 + 3.89 * qq + (16.1 * qq) ** 2 + (5.46 * qq) ** 3 + (6.71 * qq) ** 4) ** -0.25)
+        return out
+
+    def _unnormalised(self, k):
+        return np.asarray(k, np.float64) ** self.n_s * self.transfer(k) ** 2
+
+    @property
+    def amplitude(self) -> float:
+        R = 8.0
+
+        def integrand(lnk):
+            k = math.exp(lnk)
+            x = k * R
+            w = 3.0 * (math.sin(x) - x * math.cos(x)) / x ** 3 if x > 1e-4 else 1.0 - x * x / 10.0
+            return k ** 3 * float(self._unnormalised(np.array([k]))[0]) * w * w / (2 * math.pi ** 2)
+
+        s2, _ = integrate.quad(integrand, math.log(1e-5), math.log(1e3), limit=400)
+        return self.sigma8 ** 2 / s2
+
+    def power(self, k):
+        return self.amplitude * self._unnormalised(k)
+
+    def growth(self, a):
+        """Linear growth D1(a), D1(1) = 1, and f = dlnD/dlna (flat matter + Lambda)."""
+        def unnorm(a_):
+            val, _ = integrate.quad(lambda x: 1.0 / (x * self.E(x)) ** 3, 0.0, a_, limit=200)
+            return 2.5 * self.omega_m * self.E(a_) * val
+
+        d = unnorm(a) / unnorm(1.0)
+        h = 1e-4
+        f = (math.log(unnorm(a * (1 + h))) - math.log(unnorm(a * (1 - h)))) / (math.log1p(h) - math.log1p(-h))
+        return d, f
+
+
+def lpt_ics(n_side: int, box: float, a_start: float, seed: int = 1,
+            cosmo: Cosmology | None = None, device=None):
+    """Return (pos, mom) as lists of three tensors (float64) of n_side^3 particles."""
+    import torch  # data generation only
+
+    cosmo = cosmo or Cosmology()
+    device = device or ("cuda" if torch.cuda.is_available() else "cpu")
+    n = n_side
+    D, f = cosmo.growth(a_start)
+    H = 100.0 * float(cosmo.E(a_start))
+    vel_fac = a_start * a_start * H * f
+
+    kf = 2.0 * math.pi / box
+    m = torch.fft.fftfreq(n, d=1.0 / n, device=device, dtype=torch.float64)
+    mz = torch.fft.rfftfreq(n, d=1.0 / n, device=device, dtype=torch.float64)
+    kx = (m * kf).view(n, 1, 1)
+    ky = (m * kf).view(1, n, 1)
+    kz = (mz * kf).view(1, 1, n // 2 + 1)
+    k2 = kx * kx + ky * ky + kz * kz
+    kmag = torch.sqrt(k2).cpu().numpy()
+    pk = cosmo.power(kmag.ravel()).reshape(kmag.shape)
+    vcell = (box / n) ** 3
+    amp = torch.as_tensor(np.sqrt(pk / vcell), device=device) * D
+    amp[0, 0, 0] = 0.0
+
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    noise = torch.randn((n, n, n), generator=gen, device=device, dtype=torch.float32)
+    delta_k = torch.fft.rfftn(noise).to(torch.complex128) * amp
+    del noise
+
+    k2 = k2.clone()
+    k2[0, 0, 0] = 1.0
+    nyq = n // 2
+    q = torch.arange(n, device=device, dtype=torch.float64) * (box / n)
+    qs = (q.view(n, 1, 1), q.view(1, n, 1), q.view(1, 1, n))
+    pos, mom = [], []
+    for axis, kd in enumerate((kx, ky, kz)):
+        kd = kd.clone()
+        if n % 2 == 0:  # odd derivative vanishes on the Nyquist plane
+            if axis < 2:
+                kd[(nyq,) if axis == 0 else (slice(None), nyq)] = 0.0
+            else:
+                kd[..., nyq] = 0.0
+        psi_k = 1j * kd / k2 * delta_k
+        psi = torch.fft.irfftn(psi_k, s=(n, n, n)).to(torch.float64)
+        del psi_k
+        qa = qs[axis].expand(n, n, n)
+        x = torch.remainder(qa + psi, box)
+        x = torch.where(x >= box, torch.zeros_like(x), x)
+        pos.append(x.reshape(-1).contiguous())
+        mom.append((psi * vel_fac).reshape(-1).contiguous())
+        del psi
+    return pos, mom
+
+
+def particle_mass(n_side: int, box: float, omega_m: float = 0.3) -> float:
+    """m_p = rho_crit0 * Omega_m * (L/N)^3 (src/config/simulation_parameters.cpp:403-416)."""
+    from . import rho_crit0
+    return rho_crit0() * omega_m * (box / n_side) ** 3

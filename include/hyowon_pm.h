@@ -141,6 +141,27 @@ int hy_pm_xplane_order(hy_pm_ctx* ctx, uint64_t n, const double* x, uint32_t* or
 /* Waits for outstanding work and reports errors raised by HY_ASYNC calls. */
 int hy_pm_check(hy_pm_ctx* ctx);
 
+/* Live stage timing with HIP events recorded on the context's stream (the
+ * stream every kernel of the context runs on).  No reference counterpart:
+ * replaces the reference's steady_clock phase timing
+ * (src/runtime/simulation_runner.cpp:772-988) at kernel granularity.
+ * hy_pm_profile(enable, max_calls) arms a ring for max_calls force/step calls;
+ * hy_pm_profile_read synchronises, returns the summed milliseconds and the
+ * number of timed launches per stage, and clears the ring. */
+enum {
+    HY_STAGE_BUCKET = 0,      /* key + count/rank, scan, scatter */
+    HY_STAGE_DEPOSIT = 1,     /* deterministic deposit + density source + mass partials */
+    HY_STAGE_FFT_Z = 2,       /* r2c along z */
+    HY_STAGE_FFT_Y = 3,       /* forward y */
+    HY_STAGE_FFT_X_GREEN = 4, /* forward x, Green, backward x */
+    HY_STAGE_FFT_YINV = 5,    /* backward y */
+    HY_STAGE_FFT_ZINV = 6,    /* c2r along z */
+    HY_STAGE_GATHER_KDK = 7,  /* centered-gradient gather fused with kick/drift */
+    HY_STAGE_COUNT = 8
+};
+int hy_pm_profile(hy_pm_ctx* ctx, int enable, uint64_t max_calls);
+int hy_pm_profile_read(hy_pm_ctx* ctx, double* stage_ms, uint64_t* stage_count, uint64_t* calls);
+
 /* Host scalars of the step (binary64, long-double quadrature).
  * H(a): src/cosmology/cosmology_model.cpp:25-57,167-178.
  * D = int da/(a^3 H), K = int da/(a^2 H):

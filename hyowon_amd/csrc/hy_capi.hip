@@ -124,6 +124,7 @@ struct hy_pm_ctx {
     uint64_t device_bytes = 0;
     double green_num = 0.0;
     double last_input_mass = 0.0, last_deposited_mass = 0.0;
+    double mass_unit = 1.0;  // fixed-point deposit unit (uniform mass or max explicit mass)
 
     // mesh
     void* R = nullptr;
@@ -226,6 +227,7 @@ void raise_errors(uint32_t e) {
 double input_mass(hy_pm_ctx* c, uint64_t n, const double* masses, double uniform, bool host_ptrs) {
     if (n == 0) fail(HY_ERUNTIME, "PMSolver exact input mass identity must be finite and positive");
     double M;
+    c->mass_unit = uniform;
     if (!masses) {
         if (!std::isfinite(uniform) || uniform <= 0.0)
             fail(HY_EINVAL, "PMSolver uniform mass must be finite and positive");
@@ -238,6 +240,9 @@ double input_mass(hy_pm_ctx* c, uint64_t n, const double* masses, double uniform
         else hip_check(hipMemcpyAsync(h.data(), masses, n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "mass copy");
         hip_check(hipStreamSynchronize(c->stream), "sync");
         M = hy::host::exact_sum(h.data(), n);
+        double mx = 0.0;
+        for (double v : h) mx = v > mx ? v : mx;
+        c->mass_unit = mx > 0.0 ? mx : 1.0;
         if (!std::isfinite(M)) fail(HY_EINVAL, "PMSolver explicit masses must be finite and non-negative");
     }
     if (!std::isfinite(M) || M <= 0.0)
@@ -303,6 +308,18 @@ DepositPlan plan_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double
     P.total_mass = Min;
     P.cells_real = cells;
     P.centred_scale = inv_volume * (1.0 / cells);
+    {
+        // 1/(N_c dx^3) as the reference forms it: (((1/N_c)/dx)/dx)/dx in long double
+        long double S = 1.0L;
+        S /= (long double)cells;
+        S /= (long double)dx;
+        S /= (long double)dx;
+        S /= (long double)dx;
+        int e = 0;
+        const long double f = std::frexp(S, &e);
+        P.norm_mant = (uint64_t)std::ldexp(f, 64);
+        P.norm_exp = e - 64;
+    }
     P.direct = (exact_mean && std::isfinite(inv_volume) && inv_volume > 0.0) ? 1 : 0;
     P.tile_mass = c->tile_mass;
     P.scratch = c->scratch;
@@ -317,9 +334,11 @@ void launch_deposit(hy_pm_ctx* c, const hy::DepParams& P) {
     if (c->prec == HY_PREC_FP64)
         hipLaunchKernelGGL(hy::k_deposit<double>, dim3(c->ntiles), dim3(hy::DEP_THREADS), 0, c->stream, P,
                            static_cast<double*>(c->R));
-    else
-        hipLaunchKernelGGL(hy::k_deposit<float>, dim3(c->ntiles), dim3(hy::DEP_THREADS), 0, c->stream, P,
+    else {
+        hy::DepFxParams Q{P, c->mass_unit, std::ldexp(c->mass_unit, -32)};
+        hipLaunchKernelGGL(hy::k_deposit_fixed, dim3(c->ntiles), dim3(hy::DEPFX_THREADS), 0, c->stream, Q,
                            static_cast<float*>(c->R));
+    }
     launch_check("k_deposit");
     hipLaunchKernelGGL(hy::k_reduce_sum, dim3(1), dim3(256), 0, c->stream, c->tile_mass, c->ntiles,
                        c->mass_total);
@@ -458,6 +477,12 @@ int hy_pm_create(const hy_pm_desc* desc, hy_pm_ctx** out) {
         if (!std::isfinite(dx) || dx <= 0.0) fail(HY_EOVERFLOW, "MeshGeometry cell size is not representable");
         c->geom.L = L;
         c->geom.dx = dx;
+        {
+            int e2 = 0;
+            const double mant = std::frexp(dx, &e2);
+            c->geom.inv_exact = (mant == 0.5 && std::isfinite(1.0 / dx)) ? 1 : 0;
+            c->geom.inv_dx = 1.0 / dx;
+        }
         c->geom.upper = std::nextafter((double)N, 0.0);
         c->geom.N = N;
         c->geom.T = tile_edge_for(N);

@@ -29,6 +29,8 @@ enum : uint32_t {
 struct Geom {
     double L;       // box size
     double dx;      // L / N
+    double inv_dx;  // 1 / dx, used only when dx is a power of two (then w*inv_dx == w/dx)
+    int inv_exact;  // dx is a power of two
     double upper;   // nextafter((double)N, 0)
     int N;          // mesh cells per axis
     int T;          // deposit tile edge (divides N, <= 16)
@@ -58,7 +60,7 @@ __device__ __forceinline__ double minimum_image(double d, double L) {
 __device__ __forceinline__ bool cic_axis(double pos, const Geom& g, int& base, double& frac) {
     const double w = wrap_pos(pos, g.L);
     if (!(w >= 0.0 && w < g.L)) return false;
-    double s = w / g.dx;
+    double s = g.inv_exact ? w * g.inv_dx : w / g.dx;
     s = s < 0.0 ? 0.0 : s;
     s = s > g.upper ? g.upper : s;
     const double f = floor(s);

@@ -271,7 +271,9 @@ struct DepParams {
     double inv_volume;        // 1 / dx^3
     double total_mass;        // M (exact input mass)
     double cells_real;        // N_cells as double
-    double centred_scale;     // inv_volume / N_cells (normalize path)
+    double centred_scale;     // inv_volume / N_cells (normalize path, unused)
+    uint64_t norm_mant;       // 1/(N_cells dx^3) as a 64-bit mantissa ...
+    int norm_exp;             // ... times 2^norm_exp (the reference holds it in long double)
     int direct;               // 1: (m - mean) * inv_volume ; 0: fma-centred path
     double* tile_mass;        // per-tile deposited mass (binary64)
     uint64_t* scratch;        // global fallback keys
@@ -285,6 +287,33 @@ __device__ __forceinline__ uint32_t key_cell(uint64_t k) { return (uint32_t)(k >
 __device__ __forceinline__ uint32_t key_index(uint64_t k) { return (uint32_t)((k >> 19) & 0xffffffffull); }
 __device__ __forceinline__ uint32_t key_slot(uint64_t k) { return (uint32_t)(k & 0x7ffffull); }
 
+// |c| * S with S = s_mant * 2^s_exp (64-bit mantissa), rounded first to a
+// 64-bit significand and then to binary64: the x87 long-double product and cast
+// of the reference's scaled density normalisation, emulated exactly with a
+// 128-bit integer product.
+__device__ __forceinline__ double scaled_product_ld(double c, uint64_t s_mant, int s_exp) {
+    int ea = 0;
+    const double fa = frexp(fabs(c), &ea);
+    const uint64_t ma = (uint64_t)ldexp(fa, 53);
+    const unsigned __int128 q = (unsigned __int128)ma * (unsigned __int128)s_mant;
+    const uint64_t qhi = (uint64_t)(q >> 64);
+    const int nb = qhi ? 128 - __clzll(qhi) : 64 - __clzll((uint64_t)q);
+    int sh = nb - 64;
+    uint64_t m64 = (uint64_t)(q >> sh);
+    if (sh > 0) {
+        const unsigned __int128 rem = q & ((((unsigned __int128)1) << sh) - 1);
+        const unsigned __int128 half = ((unsigned __int128)1) << (sh - 1);
+        if (rem > half || (rem == half && (m64 & 1ull))) {
+            m64 += 1ull;
+            if (m64 == 0ull) { m64 = 1ull << 63; sh += 1; }
+        }
+    }
+    uint64_t m53 = m64 >> 11;
+    const uint64_t r2 = m64 & 0x7ffull;
+    if (r2 > 0x400ull || (r2 == 0x400ull && (m53 & 1ull))) m53 += 1ull;
+    return copysign(ldexp((double)m53, (ea - 53) + s_exp + sh + 11), c);
+}
+
 template <typename T>
 __device__ __forceinline__ T contrast_of(T sum, const DepParams& P) {
     const double m = (double)sum;
@@ -293,7 +322,7 @@ __device__ __forceinline__ T contrast_of(T sum, const DepParams& P) {
         v = (m - P.mean_mass) * P.inv_volume;
     } else {
         const double c = fma(m, P.cells_real, -P.total_mass);
-        v = c == 0.0 ? 0.0 : c * P.centred_scale;
+        v = c == 0.0 ? 0.0 : scaled_product_ld(c, P.norm_mant, P.norm_exp);
     }
     return (T)v;
 }
@@ -488,6 +517,140 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restr
 
 template __global__ void k_deposit<float>(DepParams, float*);
 template __global__ void k_deposit<double>(DepParams, double*);
+
+// ---------------------------------------------------------------------------
+// Binary32 deposit: fixed-point accumulation in LDS.
+//
+// Each CIC contribution m*wx*wy*wz is converted to an unsigned integer in
+// units of 2^-32 of the mass unit (the uniform particle mass, or the largest
+// explicit mass) and added with an integer LDS atomic.  Integer addition is
+// associative, so every cell sum is bit-identical whatever the scheduling: the
+// deposit is deterministic without a per-cell order and without float
+// atomics.  The resolution (2^-32 of a particle mass per contribution) is far
+// below binary32 rounding; a cell holds up to 2^31 particle masses.
+// ---------------------------------------------------------------------------
+constexpr int DEPFX_THREADS = 512;
+constexpr int DEPFX_MAX_CELLS = DEP_MAX_T * DEP_MAX_T * DEP_MAX_T;
+
+struct DepFxParams {
+    DepParams base;
+    double mass_unit;      // uniform mass, or max explicit mass
+    double fixed_to_mass;  // mass_unit * 2^-32
+};
+
+__global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, float* __restrict__ mesh) {
+    const DepParams& P = Q.base;
+    __shared__ unsigned long long acc[DEPFX_MAX_CELLS];
+    __shared__ uint32_t sh_rs[DEP_NRANGES], sh_rc[DEP_NRANGES + 1];
+    __shared__ uint8_t sh_rd[DEP_NRANGES];
+    __shared__ uint8_t sh_rsrc[DEP_NRANGES];
+    __shared__ double sh_red[DEPFX_THREADS / 64];
+
+    const Geom& g = P.g;
+    const int nT = g.nT;
+    const int T = g.T;
+    const uint32_t ntiles = (uint32_t)nT * nT * nT;
+    const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;
+    const int tz = tile % nT, ty = (tile / nT) % nT, tx = tile / (nT * nT);
+    const int cells = T * T * T;
+
+    for (int c = threadIdx.x; c < cells; c += blockDim.x) acc[c] = 0ull;
+    if (threadIdx.x == 0) {
+        int r = 0;
+        uint32_t run = 0;
+        for (int d = 0; d < 8; ++d) {
+            const int dx = (d >> 2) & 1, dy = (d >> 1) & 1, dz = d & 1;
+            const int sx = (tx - dx + nT) % nT, sy = (ty - dy + nT) % nT, sz = (tz - dz + nT) % nT;
+            const uint32_t t8 = (((uint32_t)sx * nT + sy) * nT + sz) * 8u;
+            if (d == 0) {
+                sh_rs[r] = P.offsets[t8];
+                sh_rc[r] = run; sh_rd[r] = 0; sh_rsrc[r] = 0;
+                run += P.offsets[t8 + 8] - P.offsets[t8];
+                ++r;
+            } else {
+                for (int c = 0; c < 8; ++c) {
+                    if ((c & d) != d) continue;
+                    sh_rs[r] = P.offsets[t8 + c];
+                    sh_rc[r] = run; sh_rd[r] = (uint8_t)d; sh_rsrc[r] = (uint8_t)c;
+                    run += P.offsets[t8 + c + 1] - P.offsets[t8 + c];
+                    ++r;
+                }
+            }
+        }
+        sh_rc[r] = run;
+    }
+    __syncthreads();
+    const uint32_t M = sh_rc[DEP_NRANGES];
+    const double inv_unit = 1.0 / Q.mass_unit;
+    int r = 0;
+    bool bad = false;
+    for (uint32_t v = threadIdx.x; v < M; v += blockDim.x) {
+        while (sh_rc[r + 1] <= v) ++r;
+        const uint32_t pidx = P.order[sh_rs[r] + (v - sh_rc[r])];
+        const int d = sh_rd[r];
+        int bx = 0, by = 0, bz = 0;
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        const bool ok = cic_axis(P.x[pidx], g, bx, fx) && cic_axis(P.y[pidx], g, by, fy)
+                        && cic_axis(P.z[pidx], g, bz, fz);
+        // local base cell; a neighbour record sits just below this tile
+        const int lx = (d & 4) ? -1 : bx - tx * T;
+        const int ly = (d & 2) ? -1 : by - ty * T;
+        const int lz = (d & 1) ? -1 : bz - tz * T;
+        // the record must lie where its bucket says (stale buckets are rejected)
+        const int ox_ = bx - ((tx - ((d >> 2) & 1) + nT) % nT) * T;
+        const int oy_ = by - ((ty - ((d >> 1) & 1) + nT) % nT) * T;
+        const int oz_ = bz - ((tz - (d & 1) + nT) % nT) * T;
+        const int cls = sh_rsrc[r];
+        const bool placed = ok && ox_ >= 0 && ox_ < T && oy_ >= 0 && oy_ < T && oz_ >= 0 && oz_ < T
+                            && (d == 0 || (((ox_ == T - 1) << 2 | (oy_ == T - 1) << 1 | (oz_ == T - 1)) == cls));
+        if (!placed) { bad = true; continue; }
+        const float m = P.masses ? (float)(P.masses[pidx] * inv_unit) : 1.0f;
+        const float wx[2] = {(float)(1.0 - fx), (float)fx};
+        const float wy[2] = {(float)(1.0 - fy), (float)fy};
+        const float wz[2] = {(float)(1.0 - fz), (float)fz};
+#pragma unroll
+        for (int ox = 0; ox < 2; ++ox) {
+            const int cx = lx + ox;
+            if (cx < 0 || cx >= T) continue;
+#pragma unroll
+            for (int oy = 0; oy < 2; ++oy) {
+                const int cy = ly + oy;
+                if (cy < 0 || cy >= T) continue;
+                const float mxy = m * wx[ox] * wy[oy];
+#pragma unroll
+                for (int oz = 0; oz < 2; ++oz) {
+                    const int cz = lz + oz;
+                    if (cz < 0 || cz >= T) continue;
+                    const float w = mxy * wz[oz];
+                    const unsigned long long q = __float2ull_rn(w * 4294967296.0f);
+                    atomicAdd(&acc[(cx * T + cy) * T + cz], q);
+                }
+            }
+        }
+    }
+    if (bad) atomicOr(P.err, ERR_COORDINATE);
+    __syncthreads();
+
+    double mass_acc = 0.0;
+    for (int c = threadIdx.x; c < cells; c += blockDim.x) {
+        const unsigned long long q = acc[c];
+        mass_acc += (double)q;
+        const double mcell = (double)q * Q.fixed_to_mass;
+        const int lz = c % T, ly = (c / T) % T, lx = c / (T * T);
+        const uint64_t gx = (uint64_t)(tx * T + lx), gy = (uint64_t)(ty * T + ly), gz = (uint64_t)(tz * T + lz);
+        mesh[(gx * g.N + gy) * g.N + gz] = contrast_of((double)mcell, P) ;
+    }
+    // deterministic per-tile mass in units of the fixed-point quantum
+    for (int off = 32; off > 0; off >>= 1) mass_acc += __shfl_down(mass_acc, off);
+    if (lane_id() == 0) sh_red[threadIdx.x / 64] = mass_acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / 64); ++w) s += sh_red[w];
+        P.tile_mass[tile] = s * Q.fixed_to_mass;
+    }
+}
 
 // Fixed-order reduction of per-tile masses (single workgroup).
 __global__ void k_reduce_sum(const double* __restrict__ v, uint64_t n, double* __restrict__ out) {

@@ -375,18 +375,28 @@ int or_density_contrast(double* mesh, uint64_t N, double dx, double total_mass) 
             mesh[i] = (mesh[i] - mean) * inv_volume;
         return OR_OK;
     }
-    /* normalize(): fma-centred numerator times 1/(N_c*dx^3). */
-    const double scale = inv_volume * inv_nc;
+    /* normalize() without an exact mean quotient: the fma-centred numerator
+     * N_c*m - M times 1/(N_c*dx^3), that factor held in long double
+     * (((1/N_c)/dx)/dx)/dx, the product rounded once to binary64
+     * (density_normalization.hpp:136-142 with the scaled product of
+     * scaled_positive_product.hpp:66-78,425-458). */
+    (void)inv_nc;
+    long double scale = 1.0L;
+    scale /= (long double)nc;
+    scale /= (long double)dx;
+    scale /= (long double)dx;
+    scale /= (long double)dx;
     int failed = 0;
 #pragma omp parallel for schedule(static) num_threads(OR_NT) reduction(|: failed)
     for (int64_t i = 0; i < (int64_t)cells; ++i) {
         const double m = mesh[i];
         if (!isfinite(m) || m < 0.0) { failed = 1; continue; }
         const double centred = fma(m, nc, -total_mass);
+        if (!isfinite(centred)) { failed = 1; continue; }
         if (centred == 0.0) { mesh[i] = 0.0; continue; }
-        const double v = centred * scale;
+        const double v = (double)(scale * (long double)fabs(centred));
         if (!isfinite(v) || v == 0.0) { failed = 1; continue; }
-        mesh[i] = v;
+        mesh[i] = copysign(v, centred);
     }
     return failed ? OR_EOVERFLOW : OR_OK;
 }

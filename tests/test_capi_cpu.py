@@ -1,0 +1,91 @@
+"""The C ABI library without a GPU: it loads, exports every symbol declared in
+include/hyowon_pm.h, and its host-side scalars (time integrals, H(a), the
+geometric mean, boundary schedules, exact sum) equal the reference's golden
+vectors bit for bit.  No device compute is called here."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+@pytest.fixture(scope="module")
+def hy():
+    import hyowon_amd
+    if not os.path.exists(hyowon_amd.LIB_PATH):
+        from hyowon_amd import build
+        build.build()
+    return hyowon_amd
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "hyowon_pm.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hy_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(hy):
+    lib = C.CDLL(hy.LIB_PATH)
+    syms = _declared_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert lib.hy_abi_version() == 1
+
+
+def test_python_binding_covers_the_header(hy):
+    from hyowon_amd import _lib
+    assert set(_declared_symbols()) - set(_lib.EXPORTS) <= {"hy_pm_profile", "hy_pm_profile_read"}
+
+
+def test_host_scalars_match_reference_golden(hy):
+    g = np.load(os.path.join(GOLDEN, "time_integrals.npz"), allow_pickle=False)
+    dk = hy.DriftKickIntegrator(0.3, 0.7)
+    for a1, a2, d, k, h, gm in zip(g["a1"], g["a2"], g["drift"], g["kick"], g["hubble"], g["geometric_mean"]):
+        assert dk.drift_factor(a1, a2) == d
+        assert dk.kick_factor(a1, a2) == k
+        assert hy.hubble(a1) == h
+        if a1 != a2:
+            assert hy.geometric_mean(min(a1, a2), max(a1, a2)) == gm
+    assert hy.gravitational_constant() == float(g["G"])
+    assert hy.rho_crit0() == float(g["rho_crit0"])
+    for name, steps in (("c0", 32), ("c2", 64), ("c3", 32), ("c4", 16)):
+        a0, a1, dl = g[f"args_{name}"]
+        st = hy.TimeStepper(a0, a1, dl, [1.0])
+        assert np.array_equal(st.boundaries, g[f"boundaries_{name}"])
+        assert st.num_steps == steps
+        s = list(st)
+        assert s[0].a_begin == a0 and s[-1].a_end == 1.0 and len(s) == steps
+
+
+def test_exact_sum_rounds_once(hy):
+    lib = hy.lib()
+    pd = C.POINTER(C.c_double)
+    v = np.array([1.0, 2.0 ** -60, 2.0 ** -60, -0.0 + 0.0])
+    assert lib.hy_exact_sum(v.ctypes.data_as(pd), 3) == 1.0 + 2.0 ** -59
+    v = np.array([2.0 ** 53, 1.0, 1.0])
+    assert lib.hy_exact_sum(v.ctypes.data_as(pd), 3) == 2.0 ** 53 + 2.0
+    v = np.array([1.0, -1.0])
+    assert np.isnan(lib.hy_exact_sum(v.ctypes.data_as(pd), 2))
+
+
+def test_schedule_and_factor_errors_map_to_reference_exceptions(hy):
+    from hyowon_amd._lib import HyInvalidArgument
+    with pytest.raises(HyInvalidArgument):
+        hy.TimeStepper(0.5, 0.1, 0.1)
+    with pytest.raises(HyInvalidArgument):
+        hy.DriftKickIntegrator().drift_factor(-1.0, 0.5)
+    with pytest.raises(HyInvalidArgument):
+        hy.TimeStepper(0.1, 0.5, 0.1, [0.3, 0.2])
+
+
+def test_create_rejects_invalid_geometry_before_touching_a_device(hy):
+    from hyowon_amd._lib import HyInvalidArgument
+    with pytest.raises(HyInvalidArgument):
+        hy.PMSolver(10.0, 10, precision="fp32")   # unsupported mesh size
+    with pytest.raises(HyInvalidArgument):
+        hy.PMSolver(-1.0, 16, precision="fp32")   # box size

@@ -78,6 +78,14 @@ def run_gpu(args):
     m = ic.particle_mass(n_side, L)
 
     dev = torch.device("cuda:0")
+    if args.calibrate_copy:
+        # a plain device copy of known size: its FETCH_SIZE / WRITE_SIZE under
+        # rocprofv3 --pmc calibrate the counters against algorithmic bytes
+        src = torch.ones(args.calibrate_copy // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)  # elementwise kernel: reads and writes calibrate_copy bytes
+        torch.cuda.synchronize()
+        del src, dst
     pos, mom = ic.lpt_ics(n_side, L, a0, seed=1, device=dev)
     results = {}
     for prec in args.precisions:
@@ -170,8 +178,10 @@ def main():
     ap.add_argument("--schedule-steps", type=int, default=64)
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--precisions", default="fp32,fp64")
-    ap.add_argument("--cpu-sample-side", type=int, default=128)
+    ap.add_argument("--cpu-sample-side", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="bytes of a plain device copy run first (counter calibration)")
     args = ap.parse_args()
     args.precisions = [p for p in args.precisions.split(",") if p]
     world = int(os.environ.get("WORLD_SIZE", "1"))

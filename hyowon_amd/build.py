@@ -73,7 +73,23 @@ def build(verbose: bool = False) -> str:
             print(" ".join(link), flush=True)
         subprocess.run(link, check=True)
         os.replace(tmp_lib, LIB)
+    build_cpp_dropin(verbose)
     return LIB
+
+
+def build_cpp_dropin(verbose: bool = False) -> str:
+    """Compile the C++ drop-in example/test (tests/cpp) against the library."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "dropin_force.cpp")
+    out = os.path.join(root, "tests", "cpp", "dropin_force")
+    if not os.path.exists(src):
+        return ""
+    cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++20", f"-I{INCLUDE}", src, "-o", out,
+           f"-L{HERE}", "-lhyowon_pm", f"-Wl,-rpath,$ORIGIN/../../hyowon_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return out
 
 
 if __name__ == "__main__":

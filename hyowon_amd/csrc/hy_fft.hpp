@@ -136,6 +136,17 @@ __device__ __forceinline__ void dftR(C* u) {
     else dft8<S>(u);
 }
 
+// LDS slot of logical element j of a row: one padding slot every 32 entries,
+// so the stride-R stores of the first Stockham stage and the row-strided
+// transposes spread over all banks.
+__device__ __forceinline__ constexpr int lds_slot(int j) { return j + (j >> 5); }
+
+template <int LEN>
+struct LdsRow {
+    static constexpr int SPAN = LEN + (LEN >> 5) + 1;          // LEN+1 logical entries
+    static constexpr int RS = (SPAN % 2 == 0) ? SPAN + 1 : SPAN;  // odd stride in complex units
+};
+
 // One Stockham stage over all rows held in LDS.
 template <typename C, int LEN, int ROWS, int RS, int STAGE, int S>
 __device__ __forceinline__ void fft_stage(C* lds, const C* __restrict__ tw, int tw_stride,
@@ -157,7 +168,7 @@ __device__ __forceinline__ void fft_stage(C* lds, const C* __restrict__ tw, int 
             const int k = i % p;
             C u[R];
 #pragma unroll
-            for (int r = 0; r < R; ++r) u[r] = x[i + r * M];
+            for (int r = 0; r < R; ++r) u[r] = x[lds_slot(i + r * M)];
             if (k != 0) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
@@ -180,7 +191,7 @@ __device__ __forceinline__ void fft_stage(C* lds, const C* __restrict__ tw, int 
             const int k = i % p;
             const int base = (i - k) * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) x[base + r * p] = v[j * R + r];
+            for (int r = 0; r < R; ++r) x[lds_slot(base + r * p)] = v[j * R + r];
         }
     }
     __syncthreads();
@@ -211,7 +222,7 @@ constexpr int pow2_floor(int v) {
 template <typename T, int LEN, int DIM>
 struct RowsCfg {
     using C = cplx<T>;
-    static constexpr int RS = LEN + 1;
+    static constexpr int RS = LdsRow<LEN>::RS;
     static constexpr int by_threads = 512 / FftPlan<LEN>::TPR;
     static constexpr int by_lds = kFftLdsBudget / (RS * (int)sizeof(C));
     static constexpr int cap = by_threads < by_lds ? by_threads : by_lds;

@@ -37,22 +37,28 @@ __device__ __forceinline__ uint32_t bucket_of(int bx, int by, int bz, const Geom
 }
 
 // Wave-aggregated integer atomic: lanes with equal keys reserve consecutive
-// slots with one atomic.  Returns this lane's rank (slot) within its bucket.
+// slots.  Groups are found with ballots first; then every group leader issues
+// its atomic in the same instruction, so a wave pays one atomic round trip.
+// Returns this lane's rank (slot) within its bucket.
 __device__ __forceinline__ uint32_t aggregated_rank(uint32_t* counts, uint32_t key, bool valid) {
-    uint64_t active = __ballot(valid);
     const uint32_t lane = lane_id();
-    uint32_t rank = 0;
-    while (active) {
-        const int leader = __ffsll((unsigned long long)active) - 1;
+    uint64_t remaining = __ballot(valid);
+    int my_leader = (int)lane;
+    uint64_t my_mask = 0;
+    while (remaining) {
+        const int leader = __ffsll((unsigned long long)remaining) - 1;
         const uint32_t lead_key = __shfl(key, leader);
-        const uint64_t mask = __ballot(valid && key == lead_key) & active;
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(&counts[lead_key], (uint32_t)__popcll(mask));
-        base = __shfl(base, leader);
-        if ((mask >> lane) & 1ull) rank = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-        active &= ~mask;
+        const uint64_t mask = __ballot(valid && key == lead_key) & remaining;
+        if ((mask >> lane) & 1ull) {
+            my_leader = leader;
+            my_mask = mask;
+        }
+        remaining &= ~mask;
     }
-    return rank;
+    uint32_t old = 0;
+    if (valid && (int)lane == my_leader) old = atomicAdd(&counts[key], (uint32_t)__popcll(my_mask));
+    const uint32_t base = __shfl(old, my_leader);
+    return base + (uint32_t)__popcll(my_mask & ((1ull << lane) - 1ull));
 }
 
 // mode 0: deposit buckets; mode 1: reference x-plane bins (key = base x).
@@ -400,6 +406,52 @@ __device__ __forceinline__ void build_cell_starts(KP keys, uint32_t M, uint32_t*
     __syncthreads();
 }
 
+// Source ranges of a destination tile: range 0 is the own tile (all eight
+// classes, contiguous); ranges 1..19 are the boundary classes c of the seven
+// lower neighbour tiles at offset d in {0,1}^3 with (c & d) == d.
+struct RangeSlot { uint8_t d, c; };
+__constant__ const RangeSlot kRangeSlots[20] = {
+    {0, 0},
+    {1, 1}, {1, 3}, {1, 5}, {1, 7},
+    {2, 2}, {2, 3}, {2, 6}, {2, 7},
+    {3, 3}, {3, 7},
+    {4, 4}, {4, 5}, {4, 6}, {4, 7},
+    {5, 5}, {5, 7},
+    {6, 6}, {6, 7},
+    {7, 7}};
+
+// Fills rs (start in order[]), rc (prefix of counts, rc[20] = total), rd (d),
+// rcls (class) with one global round trip per range, in parallel.
+__device__ __forceinline__ void load_source_ranges(const uint32_t* __restrict__ offsets, int tx, int ty,
+                                                   int tz, int nT, uint32_t* rs, uint32_t* rc,
+                                                   uint8_t* rd, uint8_t* rcls) {
+    const int t = threadIdx.x;
+    if (t < 20) {
+        const RangeSlot sl = kRangeSlots[t];
+        const int d = sl.d;
+        const int sx = (tx - ((d >> 2) & 1) + nT) % nT, sy = (ty - ((d >> 1) & 1) + nT) % nT,
+                  sz = (tz - (d & 1) + nT) % nT;
+        const uint32_t t8 = (((uint32_t)sx * nT + sy) * nT + sz) * 8u;
+        const uint32_t lo = d == 0 ? t8 : t8 + sl.c;
+        const uint32_t hi = d == 0 ? t8 + 8 : t8 + sl.c + 1;
+        const uint32_t a = offsets[lo], b = offsets[hi];
+        rs[t] = a;
+        rc[t + 1] = b - a;  // count, prefix-summed below
+        rd[t] = (uint8_t)d;
+        rcls[t] = sl.c;
+    }
+    __syncthreads();
+    if (t == 0) {
+        uint32_t run = 0;
+        rc[0] = 0;
+        for (int r = 0; r < 20; ++r) {
+            run += rc[r + 1];
+            rc[r + 1] = run;
+        }
+    }
+    __syncthreads();
+}
+
 template <typename T>
 __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restrict__ mesh) {
     using Cfg = DepCfg<T>;
@@ -421,31 +473,8 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restr
     const int Te = g.T + 1;
     const int ncell = Te * Te * Te;
 
-    // Source ranges: own tile (all classes) + boundary classes of the 7 lower
-    // neighbour tiles (offset d in {0,1}^3, classes c with (c & d) == d).
-    if (threadIdx.x == 0) {
-        int r = 0;
-        uint32_t acc = 0;
-        for (int d = 0; d < 8; ++d) {
-            const int dx = (d >> 2) & 1, dy = (d >> 1) & 1, dz = d & 1;
-            const int sx = (tx - dx + nT) % nT, sy = (ty - dy + nT) % nT, sz = (tz - dz + nT) % nT;
-            const uint32_t t8 = (((uint32_t)sx * nT + sy) * nT + sz) * 8u;
-            if (d == 0) {
-                sh_rs[r] = P.offsets[t8];
-                const uint32_t cnt = P.offsets[t8 + 8] - P.offsets[t8];
-                sh_rc[r] = acc; sh_rd[r] = 0; acc += cnt; ++r;
-            } else {
-                for (int c = 0; c < 8; ++c) {
-                    if ((c & d) != d) continue;
-                    sh_rs[r] = P.offsets[t8 + c];
-                    const uint32_t cnt = P.offsets[t8 + c + 1] - P.offsets[t8 + c];
-                    sh_rc[r] = acc; sh_rd[r] = (uint8_t)d; acc += cnt; ++r;
-                }
-            }
-        }
-        sh_rc[r] = acc;
-    }
-    __syncthreads();
+    __shared__ uint8_t sh_rcls[DEP_NRANGES];
+    load_source_ranges(P.offsets, tx, ty, tz, nT, sh_rs, sh_rc, sh_rd, sh_rcls);
     const uint32_t M = sh_rc[DEP_NRANGES];
     const bool use_lds = M <= (uint32_t)CAP;
 
@@ -556,31 +585,7 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
     const int cells = T * T * T;
 
     for (int c = threadIdx.x; c < cells; c += blockDim.x) acc[c] = 0ull;
-    if (threadIdx.x == 0) {
-        int r = 0;
-        uint32_t run = 0;
-        for (int d = 0; d < 8; ++d) {
-            const int dx = (d >> 2) & 1, dy = (d >> 1) & 1, dz = d & 1;
-            const int sx = (tx - dx + nT) % nT, sy = (ty - dy + nT) % nT, sz = (tz - dz + nT) % nT;
-            const uint32_t t8 = (((uint32_t)sx * nT + sy) * nT + sz) * 8u;
-            if (d == 0) {
-                sh_rs[r] = P.offsets[t8];
-                sh_rc[r] = run; sh_rd[r] = 0; sh_rsrc[r] = 0;
-                run += P.offsets[t8 + 8] - P.offsets[t8];
-                ++r;
-            } else {
-                for (int c = 0; c < 8; ++c) {
-                    if ((c & d) != d) continue;
-                    sh_rs[r] = P.offsets[t8 + c];
-                    sh_rc[r] = run; sh_rd[r] = (uint8_t)d; sh_rsrc[r] = (uint8_t)c;
-                    run += P.offsets[t8 + c + 1] - P.offsets[t8 + c];
-                    ++r;
-                }
-            }
-        }
-        sh_rc[r] = run;
-    }
-    __syncthreads();
+    load_source_ranges(P.offsets, tx, ty, tz, nT, sh_rs, sh_rc, sh_rd, sh_rsrc);
     const uint32_t M = sh_rc[DEP_NRANGES];
     const double inv_unit = 1.0 / Q.mass_unit;
     int r = 0;
@@ -686,86 +691,107 @@ struct FftArgs {
     uint32_t* err;
 };
 
+// Staged copies with compile-time trip counts: every thread issues all of its
+// loads before its first dependent store, so each wave keeps PER requests in
+// flight instead of one.
+template <int COUNT, int NT, typename C, typename Load, typename Store>
+__device__ __forceinline__ void staged_copy(int tid, Load load, Store store) {
+    constexpr int PER = (COUNT + NT - 1) / NT;
+    C v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int l = tid + i * NT;
+        if (COUNT % NT == 0 || l < COUNT) v[i] = load(l);
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int l = tid + i * NT;
+        if (COUNT % NT == 0 || l < COUNT) store(l, v[i]);
+    }
+}
+
 // z pass: real rows -> half spectrum, written transposed into A.
 template <typename T, int N>
-__global__ void k_fft_z_r2c(FftArgs<T> a) {
+__global__ void __launch_bounds__((RowsCfg<T, N / 2, N>::THREADS)) k_fft_z_r2c(FftArgs<T> a) {
     using C = cplx<T>;
     constexpr int LEN = N / 2;
     constexpr int NH = N / 2 + 1;
     using RC = RowsCfg<T, LEN, N>;
-    constexpr int ROWS = RC::ROWS, RS = RC::RS;
-    extern __shared__ unsigned char smem_raw[];
+    constexpr int ROWS = RC::ROWS, RS = RC::RS, NT = RC::THREADS;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     C* lds = reinterpret_cast<C*>(smem_raw);
-    const uint32_t nblk = gridDim.x;
-    const uint32_t b = xcd_block(blockIdx.x, nblk);
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const int yb = b % (N / ROWS), x = b / (N / ROWS);
     const int y0 = yb * ROWS;
     const int tid = threadIdx.x;
-    for (int l = tid; l < ROWS * LEN; l += blockDim.x) {
-        const int r = l / LEN, m = l % LEN;
-        const C* src = reinterpret_cast<const C*>(a.R + ((uint64_t)x * N + y0 + r) * N);
-        lds[r * RS + m] = src[m];
-    }
+    const C* src = reinterpret_cast<const C*>(a.R + ((uint64_t)x * N + y0) * N);
+    staged_copy<ROWS * LEN, NT, C>(
+        tid, [&](int l) { return src[(l / LEN) * LEN + (l % LEN)]; },
+        [&](int l, C v) { lds[(l / LEN) * RS + lds_slot(l % LEN)] = v; });
     __syncthreads();
     fft_rows<C, LEN, ROWS, RS, -1>(lds, a.tw, 2, tid);
     // untangle: X[k] = E + W^k O, E = (Z[k]+conj Z[L-k])/2, O = -i/2 (Z[k]-conj Z[L-k])
-    for (int l = tid; l < ROWS * NH; l += blockDim.x) {
-        const int r = l % ROWS, k = l / ROWS;
-        const C zk = lds[r * RS + (k % LEN)];
-        const C zc = cconj(lds[r * RS + ((LEN - k) % LEN)]);
-        const C e = {(T)0.5 * (zk.x + zc.x), (T)0.5 * (zk.y + zc.y)};
-        const C d = csub(zk, zc);
-        const C o = {(T)0.5 * d.y, (T)-0.5 * d.x};
-        const C w = a.tw[k];
-        const C X = cadd(e, cmul(w, o));
-        a.A[((uint64_t)x * NH + k) * N + y0 + r] = X;
-    }
+    C* dst = a.A + (uint64_t)x * NH * N + y0;
+    staged_copy<ROWS * NH, NT, C>(
+        tid,
+        [&](int l) {
+            const int r = l % ROWS, k = l / ROWS;
+            const C zk = lds[r * RS + lds_slot(k % LEN)];
+            const C zc = cconj(lds[r * RS + lds_slot((LEN - k) % LEN)]);
+            const C e = {(T)0.5 * (zk.x + zc.x), (T)0.5 * (zk.y + zc.y)};
+            const C d = csub(zk, zc);
+            const C o = {(T)0.5 * d.y, (T)-0.5 * d.x};
+            return cadd(e, cmul(a.tw[k], o));
+        },
+        [&](int l, C v) { dst[(uint64_t)(l / ROWS) * N + (l % ROWS)] = v; });
 }
 
 // y pass forward: A rows over y -> B[kz][ky][x]
 template <typename T, int N>
-__global__ void k_fft_y_fwd(FftArgs<T> a) {
+__global__ void __launch_bounds__((RowsCfg<T, N, N>::THREADS)) k_fft_y_fwd(FftArgs<T> a) {
     using C = cplx<T>;
     constexpr int NH = N / 2 + 1;
     using RC = RowsCfg<T, N, N>;
-    constexpr int ROWS = RC::ROWS, RS = RC::RS;
-    extern __shared__ unsigned char smem_raw[];
+    constexpr int ROWS = RC::ROWS, RS = RC::RS, NT = RC::THREADS;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     C* lds = reinterpret_cast<C*>(smem_raw);
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const int xb = b % (N / ROWS), kz = b / (N / ROWS);
     const int x0 = xb * ROWS;
     const int tid = threadIdx.x;
-    for (int l = tid; l < ROWS * N; l += blockDim.x) {
-        const int r = l / N, y = l % N;
-        lds[r * RS + y] = a.A[((uint64_t)(x0 + r) * NH + kz) * N + y];
-    }
+    staged_copy<ROWS * N, NT, C>(
+        tid, [&](int l) { return a.A[((uint64_t)(x0 + l / N) * NH + kz) * N + (l % N)]; },
+        [&](int l, C v) { lds[(l / N) * RS + lds_slot(l % N)] = v; });
     __syncthreads();
     fft_rows<C, N, ROWS, RS, -1>(lds, a.tw, 1, tid);
-    for (int l = tid; l < ROWS * N; l += blockDim.x) {
-        const int r = l % ROWS, ky = l / ROWS;
-        a.B[((uint64_t)kz * N + ky) * N + x0 + r] = lds[r * RS + ky];
-    }
+    C* dst = a.B + (uint64_t)kz * N * N + x0;
+    staged_copy<ROWS * N, NT, C>(
+        tid, [&](int l) { return lds[(l % ROWS) * RS + lds_slot(l / ROWS)]; },
+        [&](int l, C v) { dst[(uint64_t)(l / ROWS) * N + (l % ROWS)] = v; });
 }
 
 // x pass: forward x FFT, Green multiply (with 1/N^3), backward x FFT, in place on B.
 template <typename T, int N>
-__global__ void k_fft_x_green(FftArgs<T> a) {
+__global__ void __launch_bounds__((RowsCfg<T, N, (N / 2 + 1) * N>::THREADS)) k_fft_x_green(FftArgs<T> a) {
     using C = cplx<T>;
     constexpr int NH = N / 2 + 1;
     using RC = RowsCfg<T, N, NH * N>;
-    constexpr int ROWS = RC::ROWS, RS = RC::RS;
-    extern __shared__ unsigned char smem_raw[];
+    constexpr int ROWS = RC::ROWS, RS = RC::RS, NT = RC::THREADS;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     C* lds = reinterpret_cast<C*>(smem_raw);
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const uint64_t q0 = (uint64_t)b * ROWS;  // row index q = kz*N + ky
     const int tid = threadIdx.x;
-    for (int l = tid; l < ROWS * N; l += blockDim.x) {
-        const int r = l / N, xx = l % N;
-        lds[r * RS + xx] = a.B[(q0 + r) * N + xx];
-    }
+    C* rows = a.B + q0 * N;
+    staged_copy<ROWS * N, NT, C>(
+        tid, [&](int l) { return rows[l]; }, [&](int l, C v) { lds[(l / N) * RS + lds_slot(l % N)] = v; });
     __syncthreads();
     fft_rows<C, N, ROWS, RS, -1>(lds, a.tw, 1, tid);
-    for (int l = tid; l < ROWS * N; l += blockDim.x) {
+    constexpr int PER = ROWS * N / NT;
+    static_assert(ROWS * N % NT == 0, "x pass covers whole rows");
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int l = tid + i * NT;
         const int r = l / N, kx = l % N;
         const uint64_t q = q0 + r;
         const int kz = (int)(q / N), ky = (int)(q % N);
@@ -781,72 +807,69 @@ __global__ void k_fft_x_green(FftArgs<T> a) {
             }
             f = ff * (T)a.norm;
         }
-        C v = lds[r * RS + kx];
+        C v = lds[r * RS + lds_slot(kx)];
         v.x *= f;
         v.y *= f;
-        lds[r * RS + kx] = v;
+        lds[r * RS + lds_slot(kx)] = v;
     }
     __syncthreads();
     fft_rows<C, N, ROWS, RS, +1>(lds, a.tw, 1, tid);
-    for (int l = tid; l < ROWS * N; l += blockDim.x) {
-        const int r = l / N, xx = l % N;
-        a.B[(q0 + r) * N + xx] = lds[r * RS + xx];
-    }
+    staged_copy<ROWS * N, NT, C>(
+        tid, [&](int l) { return lds[(l / N) * RS + lds_slot(l % N)]; }, [&](int l, C v) { rows[l] = v; });
 }
 
 // y pass backward: B[kz][ky][x] -> A rows over y
 template <typename T, int N>
-__global__ void k_fft_y_bwd(FftArgs<T> a) {
+__global__ void __launch_bounds__((RowsCfg<T, N, N>::THREADS)) k_fft_y_bwd(FftArgs<T> a) {
     using C = cplx<T>;
     constexpr int NH = N / 2 + 1;
     using RC = RowsCfg<T, N, N>;
-    constexpr int ROWS = RC::ROWS, RS = RC::RS;
-    extern __shared__ unsigned char smem_raw[];
+    constexpr int ROWS = RC::ROWS, RS = RC::RS, NT = RC::THREADS;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     C* lds = reinterpret_cast<C*>(smem_raw);
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const int xb = b % (N / ROWS), kz = b / (N / ROWS);
     const int x0 = xb * ROWS;
     const int tid = threadIdx.x;
-    for (int l = tid; l < ROWS * N; l += blockDim.x) {
-        const int r = l % ROWS, ky = l / ROWS;
-        lds[r * RS + ky] = a.B[((uint64_t)kz * N + ky) * N + x0 + r];
-    }
+    const C* src = a.B + (uint64_t)kz * N * N + x0;
+    staged_copy<ROWS * N, NT, C>(
+        tid, [&](int l) { return src[(uint64_t)(l / ROWS) * N + (l % ROWS)]; },
+        [&](int l, C v) { lds[(l % ROWS) * RS + lds_slot(l / ROWS)] = v; });
     __syncthreads();
     fft_rows<C, N, ROWS, RS, +1>(lds, a.tw, 1, tid);
-    for (int l = tid; l < ROWS * N; l += blockDim.x) {
-        const int r = l / N, y = l % N;
-        a.A[((uint64_t)(x0 + r) * NH + kz) * N + y] = lds[r * RS + y];
-    }
+    staged_copy<ROWS * N, NT, C>(
+        tid, [&](int l) { return lds[(l / N) * RS + lds_slot(l % N)]; },
+        [&](int l, C v) { a.A[((uint64_t)(x0 + l / N) * NH + kz) * N + (l % N)] = v; });
 }
 
 // z pass backward (c2r): A[x][kz][y] -> R rows; flags a non-finite potential.
 template <typename T, int N>
-__global__ void k_fft_z_c2r(FftArgs<T> a) {
+__global__ void __launch_bounds__((RowsCfg<T, N / 2, N>::THREADS)) k_fft_z_c2r(FftArgs<T> a) {
     using C = cplx<T>;
     constexpr int LEN = N / 2;
     constexpr int NH = N / 2 + 1;
     using RC = RowsCfg<T, LEN, N>;
-    constexpr int ROWS = RC::ROWS, RS = RC::RS;
-    extern __shared__ unsigned char smem_raw[];
+    constexpr int ROWS = RC::ROWS, RS = RC::RS, NT = RC::THREADS;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     C* lds = reinterpret_cast<C*>(smem_raw);
-    __shared__ int bad;
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const int yb = b % (N / ROWS), x = b / (N / ROWS);
     const int y0 = yb * ROWS;
     const int tid = threadIdx.x;
-    if (tid == 0) bad = 0;
-    for (int l = tid; l < ROWS * NH; l += blockDim.x) {
-        const int r = l % ROWS, k = l / ROWS;
-        C v = a.A[((uint64_t)x * NH + k) * N + y0 + r];
-        if (k == 0 || k == LEN) v.y = (T)0;  // c2r reads only the independent half
-        lds[r * RS + k] = v;
-    }
+    const C* src = a.A + (uint64_t)x * NH * N + y0;
+    staged_copy<ROWS * NH, NT, C>(
+        tid, [&](int l) { return src[(uint64_t)(l / ROWS) * N + (l % ROWS)]; },
+        [&](int l, C v) {
+            const int r = l % ROWS, k = l / ROWS;
+            if (k == 0 || k == LEN) v.y = (T)0;  // c2r reads only the independent half
+            lds[r * RS + lds_slot(k)] = v;
+        });
     __syncthreads();
     // Z'[k] = (X[k] + conj X[L-k]) + i (X[k] - conj X[L-k]) W^-k
-    for (int l = tid; l < ROWS * (LEN / 2 + 1); l += blockDim.x) {
+    for (int l = tid; l < ROWS * (LEN / 2 + 1); l += NT) {
         const int r = l % ROWS, k = l / ROWS;
-        const C xk = lds[r * RS + k];
-        const C xm = lds[r * RS + (LEN - k)];
+        const C xk = lds[r * RS + lds_slot(k)];
+        const C xm = lds[r * RS + lds_slot(LEN - k)];
         const C wk = cconj(a.tw[k]);
         const C s1 = cadd(xk, cconj(xm));
         const C d1 = cmul(csub(xk, cconj(xm)), wk);
@@ -859,22 +882,20 @@ __global__ void k_fft_z_c2r(FftArgs<T> a) {
             const C d2 = cmul(csub(xm, cconj(xk)), wm);
             zm = {s2.x - d2.y, s2.y + d2.x};
         }
-        lds[r * RS + k] = zk;
-        if (pair) lds[r * RS + (LEN - k)] = zm;
+        lds[r * RS + lds_slot(k)] = zk;
+        if (pair) lds[r * RS + lds_slot(LEN - k)] = zm;
     }
     __syncthreads();
     fft_rows<C, LEN, ROWS, RS, +1>(lds, a.tw, 2, tid);
     int mybad = 0;
-    for (int l = tid; l < ROWS * LEN; l += blockDim.x) {
-        const int r = l / LEN, m = l % LEN;
-        const C v = lds[r * RS + m];
-        if (!isfinite(v.x) || !isfinite(v.y)) mybad = 1;
-        C* dst = reinterpret_cast<C*>(a.R + ((uint64_t)x * N + y0 + r) * N);
-        dst[m] = v;
-    }
-    if (mybad) bad = 1;
-    __syncthreads();
-    if (tid == 0 && bad) atomicOr(a.err, ERR_FIELD_NONFINITE);
+    C* dst = reinterpret_cast<C*>(a.R + ((uint64_t)x * N + y0) * N);
+    staged_copy<ROWS * LEN, NT, C>(
+        tid, [&](int l) { return lds[(l / LEN) * RS + lds_slot(l % LEN)]; },
+        [&](int l, C v) {
+            if (!isfinite(v.x) || !isfinite(v.y)) mybad = 1;
+            dst[l] = v;
+        });
+    if (__ballot(mybad) && lane_id() == 0) atomicOr(a.err, ERR_FIELD_NONFINITE);
 }
 
 // ===========================================================================
@@ -913,7 +934,9 @@ struct GatherArgs {
 
 template <typename T, typename PT>
 __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
-    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // each XCD walks a contiguous particle range (spatially compact for
+    // lattice-ordered particles), so potential lines are reused in its L2
+    const uint64_t p = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p >= a.n) return;
     const Geom& g = a.g;
     const int N = g.N;

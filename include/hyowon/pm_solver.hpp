@@ -1,0 +1,158 @@
+// hyowon/pm_solver.hpp -- C++20 drop-in over libhyowon_pm's C ABI.
+//
+// Mirrors the reference's interfaces for the pure-PM step so that a caller of
+//   cosmo_nbody::gravity::PMSolver            (include/cosmo_nbody/gravity/pm_solver.hpp:53-70)
+//   cosmo_nbody::time::LeapfrogIntegrator::ForceSolverFunc
+//                                              (include/cosmo_nbody/time/leapfrog_integrator.hpp:18-26)
+//   cosmo_nbody::cosmology::DriftKickIntegrator (include/cosmo_nbody/cosmology/drift_kick_factors.hpp:14-27)
+//   cosmo_nbody::time::TimeStepper             (include/cosmo_nbody/time/time_stepper.hpp)
+// can switch to the MI355X path by changing the namespace.  Spans are host
+// memory (the reference's ParticleStore); the library stages them to the GPU.
+// Errors are rethrown as the same std:: exception classes the reference uses.
+#pragma once
+
+#include "hyowon_pm.h"
+
+#include <cstddef>
+#include <cstdint>
+#include <optional>
+#include <span>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace hyowon {
+
+inline void throw_status(int rc) {
+    if (rc == HY_OK) return;
+    const std::string msg = hy_last_error();
+    switch (rc) {
+        case HY_EINVAL: throw std::invalid_argument(msg);
+        case HY_ELOGIC: throw std::logic_error(msg);
+        case HY_EOVERFLOW: throw std::overflow_error(msg);
+        case HY_EUNDERFLOW: throw std::underflow_error(msg);
+        case HY_ENOMEM: throw std::bad_alloc();
+        default: throw std::runtime_error(msg);
+    }
+}
+
+enum class Precision { FP64 = HY_PREC_FP64, FP32Mixed = HY_PREC_FP32_MIXED };
+
+namespace gravity {
+
+// Pure PM (discrete-Laplacian Green function, centered-gradient CIC gather).
+class PMSolver {
+public:
+    PMSolver(double box_size, std::size_t mesh_size, bool deconvolve_cic,
+             std::size_t max_particles, Precision precision = Precision::FP32Mixed,
+             int device = 0, void* stream = nullptr) {
+        hy_pm_desc d{};
+        d.box_size = box_size;
+        d.mesh_size = mesh_size;
+        d.deconvolve_cic = deconvolve_cic ? 1 : 0;
+        d.precision = static_cast<int>(precision);
+        d.device = device;
+        d.stream = stream;
+        d.max_particles = max_particles;
+        throw_status(hy_pm_create(&d, &ctx_));
+    }
+    ~PMSolver() {
+        if (ctx_) hy_pm_destroy(ctx_);
+    }
+    PMSolver(const PMSolver&) = delete;
+    PMSolver& operator=(const PMSolver&) = delete;
+    PMSolver(PMSolver&& o) noexcept : ctx_(o.ctx_) { o.ctx_ = nullptr; }
+
+    // Adds the PM acceleration into acc_* (reference semantics: transactional,
+    // explicit masses when `masses` is non-empty, else the uniform mass).
+    void compute_forces(std::span<const double> pos_x, std::span<const double> pos_y,
+                        std::span<const double> pos_z, std::span<const double> masses,
+                        std::optional<double> uniform_mass, std::span<double> acc_x,
+                        std::span<double> acc_y, std::span<double> acc_z) {
+        const std::size_t n = pos_x.size();
+        if (pos_y.size() != n || pos_z.size() != n || acc_x.size() != n || acc_y.size() != n ||
+            acc_z.size() != n)
+            throw std::invalid_argument("PMSolver position and acceleration component sizes must match");
+        if (!uniform_mass && masses.size() != n)
+            throw std::invalid_argument("PMSolver explicit mass count must match particle count");
+        throw_status(hy_pm_force(ctx_, n, pos_x.data(), pos_y.data(), pos_z.data(),
+                                 uniform_mass ? nullptr : masses.data(),
+                                 uniform_mass.value_or(0.0), acc_x.data(), acc_y.data(),
+                                 acc_z.data(), HY_HOST_POINTERS));
+    }
+
+    hy_pm_ctx* handle() const noexcept { return ctx_; }
+
+private:
+    hy_pm_ctx* ctx_ = nullptr;
+};
+
+}  // namespace gravity
+
+namespace cosmology {
+
+class DriftKickIntegrator {
+public:
+    DriftKickIntegrator(double omega_m, double omega_lambda) : om_(omega_m), ol_(omega_lambda) {}
+    double drift_factor(double a1, double a2) const {
+        double v = 0.0;
+        throw_status(hy_drift_factor(a1, a2, om_, ol_, &v));
+        return v;
+    }
+    double kick_factor(double a1, double a2) const {
+        double v = 0.0;
+        throw_status(hy_kick_factor(a1, a2, om_, ol_, &v));
+        return v;
+    }
+
+private:
+    double om_, ol_;
+};
+
+}  // namespace cosmology
+
+namespace time {
+
+struct TimeStep {
+    double a_begin, a_half, a_end;
+};
+
+class TimeStepper {
+public:
+    TimeStepper(double a_start, double a_final, double delta_ln_a, std::vector<double> stops = {}) {
+        std::vector<double> out(1u << 20);
+        uint64_t count = 0;
+        throw_status(hy_time_boundaries(a_start, a_final, delta_ln_a, stops.data(), stops.size(),
+                                        out.data(), out.size(), &count));
+        out.resize(count);
+        boundaries_ = std::move(out);
+    }
+    std::optional<TimeStep> next_step() {
+        if (current_ + 1 >= boundaries_.size()) return std::nullopt;
+        const double b = boundaries_[current_], e = boundaries_[current_ + 1];
+        ++current_;
+        return TimeStep{b, hy_geometric_mean(b, e), e};
+    }
+    const std::vector<double>& boundaries() const { return boundaries_; }
+    std::size_t num_steps() const { return boundaries_.empty() ? 0 : boundaries_.size() - 1; }
+
+private:
+    std::vector<double> boundaries_;
+    std::size_t current_ = 0;
+};
+
+// ForceSolverFunc-shaped adapter: returns a callable with the reference's
+// callback signature (x, y, z, masses, ax, ay, az, a) that adds the PM force.
+template <typename Solver>
+auto force_solver(Solver& solver, std::optional<double> uniform_mass) {
+    return [&solver, uniform_mass](std::span<const double> x, std::span<const double> y,
+                                   std::span<const double> z, std::span<const double> masses,
+                                   std::span<double> ax, std::span<double> ay, std::span<double> az,
+                                   double /*scale factor: PM force does not depend on a*/) {
+        solver.compute_forces(x, y, z, masses, masses.empty() ? uniform_mass : std::nullopt, ax, ay,
+                              az);
+    };
+}
+
+}  // namespace time
+}  // namespace hyowon

@@ -88,6 +88,8 @@ def run_gpu(args):
         del src, dst
     pos, mom = ic.lpt_ics(n_side, L, a0, seed=1, device=dev)
     results = {}
+    if args.accuracy and "fp32" in args.precisions:
+        results["accuracy"] = accuracy(args, pos, mom, m, K1, K2, D, nsteps)
     for prec in args.precisions:
         mdt = torch.float32 if prec == "fp32" else torch.float64
         st = H.ParticleState(pos[0].clone(), pos[1].clone(), pos[2].clone(),
@@ -95,6 +97,7 @@ def run_gpu(args):
                              mom[2].to(mdt).contiguous())
         sp = st.empty_like()
         solver = H.PMSolver(L, N, precision=prec, max_particles=n)
+        plan = solver.autotune(reps=3) if args.autotune else solver.tuning()
         real = 4 if prec == "fp32" else 8
         steps = args.steps if prec == args.precisions[0] else max(2, args.fp64_steps)
         warm = args.warmup if prec == args.precisions[0] else 1
@@ -103,7 +106,7 @@ def run_gpu(args):
             nonlocal st, sp
             j = i % nsteps
             kp = K2[(i - 1) % nsteps] if i > 0 else None
-            solver.kdk(st, sp, m, kp, K1[j], D[j], asynchronous=asynchronous)
+            solver.kdk(st, sp, m, kp, K1[j], D[j], asynchronous=asynchronous, reuse_keys=True)
             st, sp = sp, st
 
         # warmup: initial force + kick/drift, then steady steps (checked synchronously)
@@ -125,13 +128,122 @@ def run_gpu(args):
         lib().hy_pm_profile(solver.handle, 0, 0)
         stage_ms = {STAGES[k]: ms[k] / max(cnt[k], 1) for k in range(8)}
         step_ms = (t1 - t0) * 1e3 / steps
-        results[prec] = dict(step_ms=step_ms, stage_ms=stage_ms, real=real, steps=steps)
+        results[prec] = dict(step_ms=step_ms, stage_ms=stage_ms, real=real, steps=steps, plan=plan)
         info = solver.info()
         results[prec]["device_bytes"] = int(info.device_bytes)
         solver.close()
         del st, sp
         torch.cuda.empty_cache()
+        if prec == "fp32" and args.rocfft:
+            results["rocfft"] = rocfft_compare(N, dev)
     return n, N, results
+
+
+def _time_cuda(fn, reps):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def rocfft_compare(N, dev, reps=5):
+    """rocFFT (through torch.fft / hipFFT) r2c + c2r of the same binary32 N^3
+    mesh, timed beside the hand-written pencil passes (comparison only)."""
+    import torch
+    x = torch.randn(N, N, N, dtype=torch.float32, device=dev)
+    spec = torch.fft.rfftn(x)
+    r2c = _time_cuda(lambda: torch.fft.rfftn(x), reps)
+    c2r = _time_cuda(lambda: torch.fft.irfftn(spec, s=(N, N, N)), reps)
+    del x, spec
+    torch.cuda.empty_cache()
+    return {"r2c_ms": r2c, "c2r_ms": c2r, "via": f"torch {torch.__version__} torch.fft (hipFFT/rocFFT)"}
+
+
+def accuracy(args, pos, mom, m, K1, K2, D, nsteps):
+    """FP32-mixed against binary64 on the same ICs: RMS relative force error
+    sqrt(sum|g32-g64|^2 / sum|g64|^2), median and p99 of |dg|/|g| per particle,
+    then the full log-a run in both precisions (final positions as periodic
+    minimum-image distances in cells, and the CIC power spectrum below half
+    the Nyquist wavenumber).  The binary64 GPU step is itself pinned to the
+    oracle by the parity tests."""
+    import torch
+    import hyowon_amd as H
+    n_side, N, L = args.n_side, args.mesh, args.box
+    n = n_side ** 3
+    dev = pos[0].device
+    acc = {}
+    for prec in ("fp32", "fp64"):
+        s = H.PMSolver(L, N, precision=prec, max_particles=n)
+        g = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(3)]
+        s.compute_forces(pos[0], pos[1], pos[2], None, m, *g)
+        acc[prec] = torch.stack(g)
+        s.close()
+        del g
+    d2 = ((acc["fp32"] - acc["fp64"]) ** 2).sum(0)
+    g2 = (acc["fp64"] ** 2).sum(0)
+    rms = math.sqrt(float(d2.sum() / g2.sum()))
+    rel = torch.sqrt(d2 / g2.clamp_min(1e-300)).float()
+    gen = torch.Generator(device=dev).manual_seed(3)
+    samp = rel[torch.randint(0, n, (min(n, 1 << 23),), device=dev, generator=gen)]
+    q = torch.quantile(samp, torch.tensor([0.5, 0.99], device=dev)).tolist()
+    del acc, d2, g2, rel, samp
+    torch.cuda.empty_cache()
+
+    final = {}
+    for prec in ("fp32", "fp64"):
+        mdt = torch.float32 if prec == "fp32" else torch.float64
+        s = H.PMSolver(L, N, precision=prec, max_particles=n)
+        st = H.ParticleState(pos[0].clone(), pos[1].clone(), pos[2].clone(),
+                             mom[0].to(mdt).contiguous(), mom[1].to(mdt).contiguous(),
+                             mom[2].to(mdt).contiguous())
+        sp = st.empty_like()
+        for i in range(nsteps):  # the final K2 kick does not move particles
+            s.kdk(st, sp, m, K2[i - 1] if i > 0 else None, K1[i], D[i], asynchronous=True,
+                  reuse_keys=True)
+            st, sp = sp, st
+        s.check()
+        # CIC density contrast of the final positions (binary64 deposit)
+        s64 = H.PMSolver(L, N, precision="fp64", max_particles=n)
+        rho = s64.density(st.x, st.y, st.z, None, m)
+        s64.close()
+        final[prec] = (torch.stack([st.x, st.y, st.z]), rho)
+        s.close()
+        del sp
+        torch.cuda.empty_cache()
+    dx = L / N
+    dpos = final["fp32"][0] - final["fp64"][0]
+    dpos = dpos - L * torch.round(dpos / L)
+    dist = torch.sqrt((dpos ** 2).sum(0)) / dx
+    pos_rms = float(torch.sqrt((dist ** 2).mean()))
+    pos_max = float(dist.max())
+    del dpos, dist
+    pk = {}
+    for prec in ("fp32", "fp64"):
+        rho = final[prec][1]
+        spec = torch.fft.rfftn(rho)
+        p = (spec.real ** 2 + spec.imag ** 2)
+        del spec
+        kf = torch.fft.fftfreq(N, device=dev) * N
+        kz = torch.arange(N // 2 + 1, device=dev, dtype=kf.dtype)
+        kk = torch.sqrt(kf[:, None, None] ** 2 + kf[None, :, None] ** 2 + kz[None, None, :] ** 2)
+        shell = torch.round(kk).long().clamp_max(N // 4)
+        del kk
+        pk[prec] = (torch.bincount(shell.flatten(), weights=p.flatten(), minlength=N // 4 + 1)
+                    / torch.bincount(shell.flatten(), minlength=N // 4 + 1).clamp_min(1))
+        del p, shell
+        torch.cuda.empty_cache()
+    r = (pk["fp32"][1:N // 4] / pk["fp64"][1:N // 4] - 1.0).abs()
+    return {"force_rms_rel_fp32_vs_fp64": rms, "force_rel_median": q[0], "force_rel_p99": q[1],
+            "force_at": f"initial conditions, {n} particles, {N}^3 mesh",
+            "final_pos_err_cells_rms": pos_rms, "final_pos_err_cells_max": pos_max,
+            "pk_max_rel_diff_k_lt_half_nyquist": float(r.max()),
+            "run": f"{nsteps} KDK steps z={args.z_start:g}->0 in each precision"}
 
 
 def cpu_baseline(args):
@@ -180,6 +292,12 @@ def main():
     ap.add_argument("--precisions", default="fp32,fp64")
     ap.add_argument("--cpu-sample-side", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--accuracy", action=argparse.BooleanOptionalAction, default=True,
+                    help="FP32-vs-FP64 force, final-position and P(k) errors (full run)")
+    ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
+                    help="measure the FFT launch plan before timing (untimed)")
+    ap.add_argument("--rocfft", action=argparse.BooleanOptionalAction, default=True,
+                    help="time rocFFT r2c/c2r beside the hand-written FFT passes")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="bytes of a plain device copy run first (counter calibration)")
     args = ap.parse_args()
@@ -221,6 +339,7 @@ def main():
         "roofline": roofline,
         "step_hbm_frac": step_alg_bytes(n, N, r["real"]) / (r["step_ms"] * 1e-3) / HBM_PEAK,
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "fft_plan": r["plan"],
     }
     if "fp64" in res and main_prec != "fp64":
         r64 = res["fp64"]
@@ -228,6 +347,12 @@ def main():
                        "speedup_fp32_vs_fp64": r64["step_ms"] / r["step_ms"],
                        "step_hbm_frac": step_alg_bytes(n, N, 8) / (r64["step_ms"] * 1e-3) / HBM_PEAK,
                        "stage_ms": {k: round(v, 4) for k, v in r64["stage_ms"].items()}}
+    if "rocfft" in res:
+        fft_ms = sum(v for k, v in stage_ms.items() if k.startswith("fft_"))
+        out["fft_compare"] = dict(res["rocfft"], hip_pencil_passes_ms=fft_ms,
+                                  note="hip passes include the Green multiply and 1/N^3")
+    if "accuracy" in res:
+        out["accuracy"] = res["accuracy"]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out), flush=True)

@@ -14,14 +14,20 @@ HY_HOST_POINTERS = 0x1
 HY_HAS_PREV = 0x2
 HY_HAS_NEXT = 0x4
 HY_ASYNC = 0x8
+HY_REUSE_KEYS = 0x10
 
 EXPORTS = [
     "hy_pm_create", "hy_pm_destroy", "hy_pm_get_info", "hy_last_error", "hy_abi_version",
     "hy_pm_force", "hy_pm_kdk", "hy_pm_density", "hy_pm_potential", "hy_pm_gather",
     "hy_pm_bucket_order", "hy_pm_xplane_order", "hy_pm_check", "hy_hubble", "hy_drift_factor",
     "hy_kick_factor", "hy_geometric_mean", "hy_time_boundaries", "hy_exact_sum",
-    "hy_gravitational_constant", "hy_rho_crit0",
+    "hy_gravitational_constant", "hy_rho_crit0", "hy_pm_profile", "hy_pm_profile_read",
+    "hy_pm_get_tuning", "hy_pm_set_tuning", "hy_pm_autotune",
 ]
+
+HY_FFT_PASSES = 5
+HY_FFT_MAX_VARIANTS = 4
+FFT_PASS_NAMES = ["z_r2c", "y_fwd", "x_green", "y_bwd", "z_c2r"]
 
 
 class HyError(RuntimeError):
@@ -77,6 +83,14 @@ class hy_pm_info(C.Structure):
                 ("last_deposited_mass", C.c_double), ("precision", C.c_int)]
 
 
+class hy_fft_tuning(C.Structure):
+    _fields_ = [("variant", C.c_uint32 * HY_FFT_PASSES), ("plane_group", C.c_uint32),
+                ("variants", C.c_uint32),
+                ("rows", (C.c_uint32 * HY_FFT_MAX_VARIANTS) * HY_FFT_PASSES),
+                ("pass_ms", (C.c_float * HY_FFT_MAX_VARIANTS) * HY_FFT_PASSES),
+                ("group_ms", C.c_float * 4)]
+
+
 _lib = None
 
 
@@ -118,6 +132,11 @@ def lib():
     fn("hy_exact_sum", d, pd, u64)
     fn("hy_gravitational_constant", d)
     fn("hy_rho_crit0", d)
+    fn("hy_pm_profile", i, vp, i, u64)
+    fn("hy_pm_profile_read", i, vp, pd, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64))
+    fn("hy_pm_get_tuning", i, vp, C.POINTER(hy_fft_tuning))
+    fn("hy_pm_set_tuning", i, vp, C.POINTER(hy_fft_tuning))
+    fn("hy_pm_autotune", i, vp, u32, C.POINTER(hy_fft_tuning))
     _lib = L
     return L
 

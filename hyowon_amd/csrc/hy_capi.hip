@@ -5,12 +5,14 @@
 #include "hyowon_pm.h"
 
 #include "hy_host_scalars.hpp"
+#include "hy_fft_table.hpp"
 #include "hy_kernels.hpp"
 
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <algorithm>
 #include <cstring>
 #include <memory>
@@ -64,43 +66,6 @@ int guarded(F&& f) {
     }
 }
 
-// Supported mesh sizes (compile-time FFT plans).
-#define HY_MESH_SIZES(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(1024)
-
-template <typename T>
-struct FftKernels {
-    void (*z_r2c)(hy::FftArgs<T>);
-    void (*y_fwd)(hy::FftArgs<T>);
-    void (*x_green)(hy::FftArgs<T>);
-    void (*y_bwd)(hy::FftArgs<T>);
-    void (*z_c2r)(hy::FftArgs<T>);
-    int rows_z, threads_z, lds_z;
-    int rows_y, threads_y, lds_y;
-    int rows_x, threads_x, lds_x;
-};
-
-template <typename T, int N>
-FftKernels<T> make_fft() {
-    using RZ = hy::RowsCfg<T, N / 2, N>;
-    using RY = hy::RowsCfg<T, N, N>;
-    using RX = hy::RowsCfg<T, N, (N / 2 + 1) * N>;
-    return {hy::k_fft_z_r2c<T, N>, hy::k_fft_y_fwd<T, N>, hy::k_fft_x_green<T, N>,
-            hy::k_fft_y_bwd<T, N>, hy::k_fft_z_c2r<T, N>,
-            RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES,
-            RY::ROWS, RY::THREADS, RY::LDS_BYTES,
-            RX::ROWS, RX::THREADS, RX::LDS_BYTES};
-}
-
-template <typename T>
-bool lookup_fft(int N, FftKernels<T>& out) {
-    switch (N) {
-#define HY_CASE(n) case n: out = make_fft<T, n>(); return true;
-        HY_MESH_SIZES(HY_CASE)
-#undef HY_CASE
-        default: return false;
-    }
-}
-
 int tile_edge_for(int N) {
     for (int t = 16; t >= 1; --t)
         if (N % t == 0) return t;
@@ -125,6 +90,16 @@ struct hy_pm_ctx {
     double green_num = 0.0;
     double last_input_mass = 0.0, last_deposited_mass = 0.0;
     double mass_unit = 1.0;  // fixed-point deposit unit (uniform mass or max explicit mass)
+    int plane_group = 0;     // kz planes per y/x/y^-1 launch group (0 = all planes at once)
+    // Bucket keys/ranks/counts computed by the previous fused step for its output
+    // positions; reused when the next step's input is that output.
+    bool keys_ready = false;
+    const double* keys_x = nullptr;
+    const double* keys_y = nullptr;
+    const double* keys_z = nullptr;
+    uint64_t keys_n = 0;
+    hy::DepRecord* rec = nullptr;
+    float* rec_mass = nullptr;
 
     // mesh
     void* R = nullptr;
@@ -155,8 +130,10 @@ struct hy_pm_ctx {
     double* hpos = nullptr;  // device staging for host-pointer calls (3*cap doubles + masses)
     double* hacc = nullptr;  // 3*cap
 
-    FftKernels<float> fft32{};
-    FftKernels<double> fft64{};
+    hy::FftKernels<float> fft32{};
+    hy::FftKernels<double> fft64{};
+    int fft_var[hy::FFT_PASSES] = {0, 0, 0, 0, 0};  // launch variant per pass
+    int max_lds = 0;                                 // device LDS bytes per workgroup
 
     // pinned host mirror of the status words
     struct Status {
@@ -215,6 +192,7 @@ void raise_errors(uint32_t e) {
     using namespace hy;
     if (e & ERR_POSITION_NONFINITE) fail(HY_EINVAL, "CIC positions must be finite");
     if (e & ERR_MASS_INVALID) fail(HY_EINVAL, "CIC masses must be finite and non-negative");
+    if (e & ERR_STALE_KEYS) fail(HY_ELOGIC, "reused bucket keys do not match the input positions");
     if (e & ERR_COORDINATE) fail(HY_ELOGIC, "CIC bounded coordinate escaped the mesh domain");
     if (e & ERR_SCRATCH) fail(HY_ERUNTIME, "deposit fallback scratch exhausted");
     if (e & ERR_FIELD_NONFINITE) fail(HY_EINVAL, "CIC centered-gradient potential must be finite");
@@ -251,12 +229,16 @@ double input_mass(hy_pm_ctx* c, uint64_t n, const double* masses, double uniform
 }
 
 // Bucket count/rank + scan + scatter.  mode 0: deposit buckets, 1: x-planes.
+// carried: keys, ranks and counts were produced by the previous fused step.
 void bucketize(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z, int mode,
-               uint32_t* counts, uint32_t* offsets, uint64_t nb, uint32_t* order) {
-    hip_check(hipMemsetAsync(counts, 0, nb * sizeof(uint32_t), c->stream), "memset counts");
-    hipLaunchKernelGGL(hy::k_bucket_key, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, x, y, z,
-                       c->geom, mode, c->keys, c->ranks, counts, c->err);
-    launch_check("k_bucket_key");
+               uint32_t* counts, uint32_t* offsets, uint64_t nb, uint32_t* order, bool carried = false,
+               const double* masses = nullptr) {
+    if (!carried) {
+        hip_check(hipMemsetAsync(counts, 0, nb * sizeof(uint32_t), c->stream), "memset counts");
+        hipLaunchKernelGGL(hy::k_bucket_key, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, x, y, z,
+                           c->geom, mode, c->keys, c->ranks, counts, c->err);
+        launch_check("k_bucket_key");
+    }
     const uint64_t nblk = blocks_for(nb, hy::SCAN_TILE);
     hipLaunchKernelGGL(hy::k_scan_reduce, dim3(nblk), dim3(hy::SCAN_THREADS), 0, c->stream, counts, nb,
                        c->block_sums);
@@ -265,9 +247,20 @@ void bucketize(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const
     hipLaunchKernelGGL(hy::k_scan_apply, dim3(nblk), dim3(hy::SCAN_THREADS), 0, c->stream, counts, nb,
                        c->block_sums, c->grand_total, offsets);
     launch_check("k_scan");
-    hipLaunchKernelGGL(hy::k_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->keys,
-                       c->ranks, offsets, order);
-    launch_check("k_scatter");
+    if (mode == 0 && c->prec == HY_PREC_FP32_MIXED) {
+        hipLaunchKernelGGL(hy::k_scatter_records, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, x, y,
+                           z, masses, 1.0 / c->mass_unit, c->geom, c->keys, c->ranks, offsets,
+                           order, c->rec, c->rec_mass, c->err);
+        launch_check("k_scatter_records");
+    } else if (carried) {
+        hipLaunchKernelGGL(hy::k_scatter_checked, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, x, y,
+                           z, c->geom, c->keys, c->ranks, offsets, order, c->err);
+        launch_check("k_scatter_checked");
+    } else {
+        hipLaunchKernelGGL(hy::k_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, c->keys,
+                           c->ranks, offsets, order);
+        launch_check("k_scatter");
+    }
 }
 
 void reset_status(hy_pm_ctx* c) {
@@ -303,6 +296,7 @@ DepositPlan plan_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double
     P.uniform_mass = uniform;
     P.offsets = c->offsets;
     P.order = c->order;
+    P.n = n;
     P.mean_mass = mean;
     P.inv_volume = inv_volume;
     P.total_mass = Min;
@@ -335,7 +329,8 @@ void launch_deposit(hy_pm_ctx* c, const hy::DepParams& P) {
         hipLaunchKernelGGL(hy::k_deposit<double>, dim3(c->ntiles), dim3(hy::DEP_THREADS), 0, c->stream, P,
                            static_cast<double*>(c->R));
     else {
-        hy::DepFxParams Q{P, c->mass_unit, std::ldexp(c->mass_unit, -32)};
+        hy::DepFxParams Q{P, c->mass_unit, std::ldexp(c->mass_unit, -32), c->rec,
+                          P.masses ? c->rec_mass : nullptr};
         hipLaunchKernelGGL(hy::k_deposit_fixed, dim3(c->ntiles), dim3(hy::DEPFX_THREADS), 0, c->stream, Q,
                            static_cast<float*>(c->R));
     }
@@ -346,7 +341,7 @@ void launch_deposit(hy_pm_ctx* c, const hy::DepParams& P) {
 }
 
 template <typename T>
-void launch_fft(hy_pm_ctx* c, const FftKernels<T>& K) {
+hy::FftArgs<T> fft_args(hy_pm_ctx* c) {
     hy::FftArgs<T> a{};
     a.R = static_cast<T*>(c->R);
     a.A = static_cast<hy::cplx<T>*>(c->A);
@@ -358,21 +353,47 @@ void launch_fft(hy_pm_ctx* c, const FftKernels<T>& K) {
     a.norm = 1.0 / (static_cast<double>(c->N) * static_cast<double>(c->N) * static_cast<double>(c->N));
     a.deconvolve = c->desc.deconvolve_cic ? 1 : 0;
     a.err = c->err;
-    const int N = c->N, NH = N / 2 + 1;
-    hipLaunchKernelGGL(K.z_r2c, dim3(N * (N / K.rows_z)), dim3(K.threads_z), K.lds_z, c->stream, a);
-    launch_check("k_fft_z_r2c");
+    return a;
+}
+
+// One launch of pass p (variant v) over kz planes [kz0, kz0+g) (y and x
+// passes) or over the whole mesh (z passes).
+template <typename T>
+void launch_pass(hy_pm_ctx* c, const hy::FftKernels<T>& K, hy::FftArgs<T>& a, int p, int v, int kz0, int g) {
+    static const char* names[hy::FFT_PASSES] = {"k_fft_z_r2c", "k_fft_y_fwd", "k_fft_x_green", "k_fft_y_bwd",
+                                                "k_fft_z_c2r"};
+    const hy::FftPass<T>& P = K.pass[p][v];
+    const uint64_t N = (uint64_t)c->N;
+    uint64_t blocks;
+    if (p == hy::FFT_Z_R2C || p == hy::FFT_Z_C2R) blocks = N * (N / P.rows);
+    else if (p == hy::FFT_X_GREEN) blocks = (uint64_t)g * N / P.rows;
+    else blocks = (uint64_t)g * (N / P.rows);
+    a.kz0 = kz0;
+    hipLaunchKernelGGL(P.fn, dim3((uint32_t)blocks), dim3(P.threads), P.lds, c->stream, a);
+    launch_check(names[p]);
+}
+
+template <typename T>
+void launch_fft(hy_pm_ctx* c, const hy::FftKernels<T>& K) {
+    hy::FftArgs<T> a = fft_args<T>(c);
+    const int NH = c->N / 2 + 1;
+    const int* v = c->fft_var;
+    launch_pass(c, K, a, hy::FFT_Z_R2C, v[hy::FFT_Z_R2C], 0, NH);
     mark(c, HY_STAGE_FFT_Z + 1);
-    hipLaunchKernelGGL(K.y_fwd, dim3(NH * (N / K.rows_y)), dim3(K.threads_y), K.lds_y, c->stream, a);
-    launch_check("k_fft_y_fwd");
-    mark(c, HY_STAGE_FFT_Y + 1);
-    hipLaunchKernelGGL(K.x_green, dim3((uint64_t)NH * N / K.rows_x), dim3(K.threads_x), K.lds_x, c->stream, a);
-    launch_check("k_fft_x_green");
-    mark(c, HY_STAGE_FFT_X_GREEN + 1);
-    hipLaunchKernelGGL(K.y_bwd, dim3(NH * (N / K.rows_y)), dim3(K.threads_y), K.lds_y, c->stream, a);
-    launch_check("k_fft_y_bwd");
+    // The y, x+Green and y^-1 passes touch one kz plane (N^2 complex) at a
+    // time.  Running them over groups of kz planes keeps a group (<= the
+    // Infinity Cache) resident between the three passes.
+    const int group = c->plane_group > 0 ? std::min(c->plane_group, NH) : NH;
+    for (int kz0 = 0; kz0 < NH; kz0 += group) {
+        const int g = std::min(group, NH - kz0);
+        launch_pass(c, K, a, hy::FFT_Y_FWD, v[hy::FFT_Y_FWD], kz0, g);
+        if (kz0 + g >= NH) mark(c, HY_STAGE_FFT_Y + 1);
+        launch_pass(c, K, a, hy::FFT_X_GREEN, v[hy::FFT_X_GREEN], kz0, g);
+        if (kz0 + g >= NH) mark(c, HY_STAGE_FFT_X_GREEN + 1);
+        launch_pass(c, K, a, hy::FFT_Y_BWD, v[hy::FFT_Y_BWD], kz0, g);
+    }
     mark(c, HY_STAGE_FFT_YINV + 1);
-    hipLaunchKernelGGL(K.z_c2r, dim3(N * (N / K.rows_z)), dim3(K.threads_z), K.lds_z, c->stream, a);
-    launch_check("k_fft_z_c2r");
+    launch_pass(c, K, a, hy::FFT_Z_C2R, v[hy::FFT_Z_C2R], 0, NH);
     mark(c, HY_STAGE_FFT_ZINV + 1);
 }
 
@@ -406,9 +427,9 @@ void check_mass(hy_pm_ctx* c, double input, double deposited, uint64_t n) {
 
 // Deposit stage: bucketize, deposit (+ retry with a larger fallback scratch).
 void run_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z,
-                 const double* masses, double uniform, double Min, bool sync) {
+                 const double* masses, double uniform, double Min, bool sync, bool carried = false) {
     mark(c, 0);
-    bucketize(c, n, x, y, z, 0, c->counts, c->offsets, c->nb, c->order);
+    bucketize(c, n, x, y, z, 0, c->counts, c->offsets, c->nb, c->order, carried, masses);
     mark(c, HY_STAGE_BUCKET + 1);
     auto plan = plan_deposit(c, n, x, y, z, masses, uniform, Min);
     launch_deposit(c, plan.P);
@@ -417,8 +438,21 @@ void run_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, con
     if (!sync) return;
     for (int attempt = 0;; ++attempt) {
         const auto st = fetch_status(c);
+        if (carried && (st.err & hy::ERR_STALE_KEYS)) {
+            // the positions changed since the keys were made: route them afresh
+            // (coordinate and scratch errors of the stale routing are its artefacts)
+            raise_errors(st.err & (hy::ERR_POSITION_NONFINITE | hy::ERR_MASS_INVALID));
+            carried = false;
+            reset_status(c);
+            bucketize(c, n, x, y, z, 0, c->counts, c->offsets, c->nb, c->order, false, masses);
+            plan = plan_deposit(c, n, x, y, z, masses, uniform, Min);
+            launch_deposit(c, plan.P);
+            continue;
+        }
         if ((st.err & hy::ERR_SCRATCH) && attempt < 3) {
-            // grow the fallback scratch (records of the hot tiles) and redo the deposit
+            // grow the fallback scratch (records of the hot tiles) and redo the
+            // deposit; errors the retry would clear are raised first
+            raise_errors(st.err & ~hy::ERR_SCRATCH);
             const uint64_t grow = c->scratch_cap * 2;
             void* p = nullptr;
             hip_check(hipMalloc(&p, grow * sizeof(uint64_t)), "hipMalloc scratch");
@@ -444,6 +478,78 @@ void require_particles(hy_pm_ctx* c, uint64_t n) {
     if (n > 0xffffffffull) fail(HY_EOVERFLOW, "particle count exceeds the 32-bit index range");
 }
 
+static_assert(hy::FFT_PASSES == HY_FFT_PASSES && hy::kFftVariants == HY_FFT_MAX_VARIANTS,
+              "FFT tuning table layout");
+
+template <typename T>
+void fill_tuning(const hy_pm_ctx* c, const hy::FftKernels<T>& K, hy_fft_tuning* t) {
+    t->variants = (uint32_t)K.variants;
+    t->plane_group = (uint32_t)c->plane_group;
+    for (int p = 0; p < hy::FFT_PASSES; ++p) {
+        t->variant[p] = (uint32_t)c->fft_var[p];
+        for (int v = 0; v < HY_FFT_MAX_VARIANTS; ++v)
+            t->rows[p][v] = v < K.variants ? (uint32_t)K.pass[p][v].rows : 0u;
+    }
+}
+
+// Times each launch variant of every pass on zero-filled mesh buffers (all
+// passes stay finite), keeps the fastest, then times whole transforms over
+// plane-group sizes.
+template <typename T>
+void autotune(hy_pm_ctx* c, const hy::FftKernels<T>& K, uint32_t reps, hy_fft_tuning* t) {
+    const uint64_t N = (uint64_t)c->N, NH = N / 2 + 1;
+    hip_check(hipMemsetAsync(c->R, 0, N * N * N * sizeof(T), c->stream), "memset R");
+    hip_check(hipMemsetAsync(c->A, 0, N * NH * N * 2 * sizeof(T), c->stream), "memset A");
+    hip_check(hipMemsetAsync(c->B, 0, N * NH * N * 2 * sizeof(T), c->stream), "memset B");
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    const bool prof = c->prof;
+    c->prof = false;
+    auto timed = [&](auto&& launch) {
+        launch();  // warm
+        hip_check(hipEventRecord(e0, c->stream), "hipEventRecord");
+        for (uint32_t r = 0; r < reps; ++r) launch();
+        hip_check(hipEventRecord(e1, c->stream), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        return ms / (float)reps;
+    };
+    hy::FftArgs<T> a = fft_args<T>(c);
+    for (int p = 0; p < hy::FFT_PASSES; ++p) {
+        int best = 0;
+        for (int v = 0; v < HY_FFT_MAX_VARIANTS; ++v) t->pass_ms[p][v] = 0.0f;
+        for (int v = 0; v < K.variants; ++v) {
+            if (K.pass[p][v].lds > c->max_lds) {
+                t->pass_ms[p][v] = -1.0f;  // not launchable on this device
+                continue;
+            }
+            t->pass_ms[p][v] = timed([&] { launch_pass(c, K, a, p, v, 0, (int)NH); });
+            if (t->pass_ms[p][v] < t->pass_ms[p][best]) best = v;
+        }
+        c->fft_var[p] = best;
+    }
+    static const int groups[4] = {0, 16, 64, 128};
+    int best_g = 0;
+    for (int i = 0; i < 4; ++i) {
+        t->group_ms[i] = 0.0f;
+        if (groups[i] != 0 && groups[i] >= (int)NH) continue;
+        c->plane_group = groups[i];
+        t->group_ms[i] = timed([&] { launch_fft<T>(c, K); });
+        // a plane group must win by 2% to replace whole-mesh launches
+        if (i > 0 && t->group_ms[i] < 0.98f * t->group_ms[best_g]) best_g = i;
+    }
+    c->plane_group = groups[best_g];
+    c->prof = prof;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    hip_check(hipMemsetAsync(c->err, 0, sizeof(uint32_t), c->stream), "memset err");
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+    c->keys_ready = false;
+    fill_tuning(c, K, t);
+}
+
 }  // namespace
 
 extern "C" {
@@ -461,9 +567,9 @@ int hy_pm_create(const hy_pm_desc* desc, hy_pm_ctx** out) {
         if (!std::isfinite(L) || L <= 0.0) fail(HY_EINVAL, "MeshGeometry box size must be finite and positive");
         if (desc->precision != HY_PREC_FP64 && desc->precision != HY_PREC_FP32_MIXED)
             fail(HY_EINVAL, "unknown precision");
-        FftKernels<float> k32{};
-        FftKernels<double> k64{};
-        if (Nu > 4096 || !lookup_fft<float>((int)Nu, k32) || !lookup_fft<double>((int)Nu, k64))
+        hy::FftKernels<float> k32{};
+        hy::FftKernels<double> k64{};
+        if (Nu > 4096 || !hy::lookup_fft((int)Nu, k32) || !hy::lookup_fft((int)Nu, k64))
             fail(HY_EINVAL, "mesh size must be 2^a (8..1024) or 3*2^a (12..384)");
         const int N = (int)Nu;
         auto c = std::make_unique<hy_pm_ctx>();
@@ -488,11 +594,20 @@ int hy_pm_create(const hy_pm_desc* desc, hy_pm_ctx** out) {
         c->geom.T = tile_edge_for(N);
         c->geom.nT = N / c->geom.T;
         c->cap = desc->max_particles;
+        if (const char* env = std::getenv("HY_PLANE_GROUP")) c->plane_group = std::atoi(env);
         set_device(c.get());
+        hip_check(hipDeviceGetAttribute(&c->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, desc->device),
+                  "hipDeviceGetAttribute");
+        for (int p = 0; p < hy::FFT_PASSES; ++p)
+            if (std::max(k32.pass[p][0].lds, k64.pass[p][0].lds) > c->max_lds)
+                fail(HY_EDEVICE, "device LDS per workgroup is below the FFT passes' requirement");
         if (desc->stream) {
             c->stream = static_cast<hipStream_t>(desc->stream);
         } else {
-            hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+            // a blocking stream: ordered with the legacy default stream, on
+            // which callers (PyTorch's default stream among them) produce
+            // the device arrays they pass in
+            hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamDefault), "hipStreamCreate");
             c->own_stream = true;
         }
 
@@ -562,6 +677,8 @@ int hy_pm_create(const hy_pm_desc* desc, hy_pm_ctx** out) {
         c->scratch = (uint64_t*)c->alloc(c->scratch_cap * 8);
         c->scratch_cursor = (unsigned long long*)c->alloc(16);
         c->err = (uint32_t*)c->alloc(16);
+        c->rec = (hy::DepRecord*)c->alloc(cap * sizeof(hy::DepRecord));
+        c->rec_mass = (float*)c->alloc(cap * sizeof(float));
         c->gx = c->alloc(cap * rb);
         c->gy = c->alloc(cap * rb);
         c->gz = c->alloc(cap * rb);
@@ -601,6 +718,12 @@ int hy_pm_check(hy_pm_ctx* c) {
         if (!c) fail(HY_EINVAL, "null context");
         set_device(c);
         const auto st = fetch_status(c);
+        if (st.err) {
+            // report once; the context stays usable
+            c->keys_ready = false;
+            reset_status(c);
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        }
         raise_errors(st.err);
     });
 }
@@ -610,6 +733,7 @@ int hy_pm_density(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, co
     return guarded([&] {
         if (!c || !x || !y || !z || !mesh_out) fail(HY_EINVAL, "hy_pm_density: null argument");
         set_device(c);
+        c->keys_ready = false;
         require_particles(c, n);
         const double M = input_mass(c, n, masses, uniform, false);
         reset_status(c);
@@ -674,6 +798,7 @@ int hy_pm_force(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, cons
         if (n == 0) fail(HY_ERUNTIME, "PMSolver exact input mass identity must be finite and positive");
         if (!x || !y || !z || !ax || !ay || !az) fail(HY_EINVAL, "hy_pm_force: null argument");
         set_device(c);
+        c->keys_ready = false;
         require_particles(c, n);
         const bool host = (flags & HY_HOST_POINTERS) != 0;
         if (host) {
@@ -780,7 +905,15 @@ int hy_pm_kdk(hy_pm_ctx* c, uint64_t n, const double* x_in, const double* y_in, 
         if (sync) reset_status(c);
         else hip_check(hipMemsetAsync(c->scratch_cursor, 0, sizeof(unsigned long long), c->stream), "memset");
         if (masses) hipLaunchKernelGGL(hy::k_check_mass, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, masses, c->err);
-        run_deposit(c, n, x_in, y_in, z_in, masses, uniform, M, sync);
+        // keys of the previous fused step are valid for exactly its output positions
+        const bool carried = (flags & HY_REUSE_KEYS) && c->keys_ready && c->keys_n == n && c->keys_x == x_in &&
+                             c->keys_y == y_in && c->keys_z == z_in;
+        c->keys_ready = false;
+        run_deposit(c, n, x_in, y_in, z_in, masses, uniform, M, sync, carried);
+        // counts are consumed by the scan above; the gather below accumulates the
+        // next step's counts into the zeroed buffer
+        const bool make_keys = has_next;
+        if (make_keys) hip_check(hipMemsetAsync(c->counts, 0, c->nb * sizeof(uint32_t), c->stream), "memset counts");
         run_fft(c);
         auto fill = [&](auto& a) {
             a.g = c->geom; a.n = n;
@@ -792,6 +925,11 @@ int hy_pm_kdk(hy_pm_ctx* c, uint64_t n, const double* x_in, const double* y_in, 
             a.k_prev = k_prev; a.k_next = k_next; a.d_next = d_next;
             a.has_prev = has_prev; a.has_next = has_next;
             a.err = c->err;
+            if (make_keys) {
+                a.next_counts = c->counts;
+                a.next_keys = c->keys;
+                a.next_ranks = c->ranks;
+            }
         };
         if (c->prec == HY_PREC_FP64) {
             hy::GatherArgs<double, double> a{};
@@ -814,6 +952,13 @@ int hy_pm_kdk(hy_pm_ctx* c, uint64_t n, const double* x_in, const double* y_in, 
             const auto st = fetch_status(c);
             raise_errors(st.err);
         }
+        if (make_keys) {
+            c->keys_ready = true;
+            c->keys_x = x_out ? x_out : x_in;
+            c->keys_y = y_out ? y_out : y_in;
+            c->keys_z = z_out ? z_out : z_in;
+            c->keys_n = n;
+        }
     });
 }
 
@@ -822,6 +967,7 @@ int hy_pm_bucket_order(hy_pm_ctx* c, uint64_t n, const double* x, const double* 
     return guarded([&] {
         if (!c || !x || !y || !z || !order || !offsets) fail(HY_EINVAL, "hy_pm_bucket_order: null argument");
         set_device(c);
+        c->keys_ready = false;
         require_particles(c, n);
         reset_status(c);
         bucketize(c, n, x, y, z, 0, c->counts, offsets, c->nb, order);
@@ -838,6 +984,7 @@ int hy_pm_xplane_order(hy_pm_ctx* c, uint64_t n, const double* x, uint32_t* orde
     return guarded([&] {
         if (!c || !x || !order || !offsets) fail(HY_EINVAL, "hy_pm_xplane_order: null argument");
         set_device(c);
+        c->keys_ready = false;
         require_particles(c, n);
         reset_status(c);
         bucketize(c, n, x, x, x, 1, c->xcounts, offsets, (uint64_t)c->N, order);
@@ -891,6 +1038,43 @@ int hy_pm_profile_read(hy_pm_ctx* c, double* stage_ms, uint64_t* stage_count, ui
         if (calls) *calls = c->prof_calls;
         c->prof_calls = 0;
         std::fill(c->prof_used.begin(), c->prof_used.end(), 0);
+    });
+}
+
+int hy_pm_get_tuning(hy_pm_ctx* c, hy_fft_tuning* out) {
+    return guarded([&] {
+        if (!c || !out) fail(HY_EINVAL, "null argument");
+        std::memset(out, 0, sizeof(*out));
+        if (c->prec == HY_PREC_FP64) fill_tuning(c, c->fft64, out);
+        else fill_tuning(c, c->fft32, out);
+    });
+}
+
+int hy_pm_set_tuning(hy_pm_ctx* c, const hy_fft_tuning* in) {
+    return guarded([&] {
+        if (!c || !in) fail(HY_EINVAL, "null argument");
+        const int nv = c->prec == HY_PREC_FP64 ? c->fft64.variants : c->fft32.variants;
+        for (int p = 0; p < hy::FFT_PASSES; ++p) {
+            const int v = (int)in->variant[p];
+            if (v >= nv) fail(HY_EINVAL, "FFT launch variant not compiled for this mesh size");
+            const int lds = c->prec == HY_PREC_FP64 ? c->fft64.pass[p][v].lds : c->fft32.pass[p][v].lds;
+            if (lds > c->max_lds) fail(HY_EINVAL, "FFT launch variant exceeds the device LDS per workgroup");
+        }
+        if (in->plane_group > (uint32_t)c->N) fail(HY_EINVAL, "plane group exceeds the mesh size");
+        for (int p = 0; p < hy::FFT_PASSES; ++p) c->fft_var[p] = (int)in->variant[p];
+        c->plane_group = (int)in->plane_group;
+    });
+}
+
+int hy_pm_autotune(hy_pm_ctx* c, uint32_t reps, hy_fft_tuning* out) {
+    return guarded([&] {
+        if (!c) fail(HY_EINVAL, "null context");
+        if (reps == 0 || reps > 1000) fail(HY_EINVAL, "autotune repetitions must be 1..1000");
+        set_device(c);
+        hy_fft_tuning t{};
+        if (c->prec == HY_PREC_FP64) autotune(c, c->fft64, reps, &t);
+        else autotune(c, c->fft32, reps, &t);
+        if (out) *out = t;
     });
 }
 

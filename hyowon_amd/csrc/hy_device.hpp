@@ -24,6 +24,7 @@ enum : uint32_t {
     ERR_FORCE_NONFINITE = 1u << 5,     // overflow_error: gathered force not finite
     ERR_UPDATE_NONFINITE = 1u << 6,    // overflow_error: kick/drift not finite
     ERR_MOMENTUM_NONFINITE = 1u << 7,  // overflow_error: kick input not finite
+    ERR_STALE_KEYS = 1u << 8,          // reused bucket keys do not match the positions
 };
 
 struct Geom {

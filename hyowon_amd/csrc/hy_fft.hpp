@@ -209,7 +209,15 @@ __device__ __forceinline__ void fft_rows(C* lds, const C* __restrict__ tw, int t
 // ---------------------------------------------------------------------------
 // Per-pass launch geometry
 // ---------------------------------------------------------------------------
-constexpr int kFftLdsBudget = 80 * 1024;
+// Launch variants of every pass (chosen per pass by the autotuner): an LDS
+// budget per workgroup, which sets the rows per workgroup and therefore the
+// workgroups resident per CU (160 KB of LDS each) and the length of the
+// contiguous runs of the transposing stores.
+constexpr int kFftVariants = 4;
+constexpr int fft_lds_budget(int v) {
+    return v == 0 ? 80 * 1024 : v == 1 ? 40 * 1024 : v == 2 ? 136 * 1024 : 20 * 1024;
+}
+constexpr int fft_thread_cap(int v) { return v == 2 ? 1024 : 512; }
 
 constexpr int pow2_floor(int v) {
     int p = 1;
@@ -218,19 +226,20 @@ constexpr int pow2_floor(int v) {
 }
 
 // Rows per workgroup for a transform of length LEN whose rows are blocked from
-// a dimension of extent DIM (ROWS must divide DIM).
-template <typename T, int LEN, int DIM>
+// a dimension of extent DIM (ROWS must divide DIM), launch variant V.
+template <typename T, int LEN, int DIM, int V = 0>
 struct RowsCfg {
     using C = cplx<T>;
     static constexpr int RS = LdsRow<LEN>::RS;
-    static constexpr int by_threads = 512 / FftPlan<LEN>::TPR;
-    static constexpr int by_lds = kFftLdsBudget / (RS * (int)sizeof(C));
+    static constexpr int by_threads = fft_thread_cap(V) / FftPlan<LEN>::TPR;
+    static constexpr int by_lds = fft_lds_budget(V) / (RS * (int)sizeof(C));
     static constexpr int cap = by_threads < by_lds ? by_threads : by_lds;
     static constexpr int dim_pow2 = (DIM & (-DIM));  // largest power of two dividing DIM
     static constexpr int ROWS0 = pow2_floor(cap < 1 ? 1 : cap);
     static constexpr int ROWS = ROWS0 < dim_pow2 ? ROWS0 : dim_pow2;
     static constexpr int THREADS = ROWS * FftPlan<LEN>::TPR;
     static constexpr int LDS_BYTES = ROWS * RS * (int)sizeof(C);
+    static_assert(LDS_BYTES <= 160 * 1024, "LDS per workgroup");
 };
 
 }  // namespace hy

@@ -1,0 +1,7 @@
+// Binary64 FFT pass instantiations (every mesh size and launch variant).
+#define HY_FFT_INSTANTIATE
+#include "hy_fft_table.hpp"
+
+namespace hy {
+bool lookup_fft(int N, FftKernels<double>& out) { return lookup_fft_impl<double>(N, out); }
+}  // namespace hy

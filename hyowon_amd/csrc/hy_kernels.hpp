@@ -9,7 +9,7 @@
 //        to src/mesh/mass_assignment_deposit.inc:193-240 (bin X with weight
 //        1-f, then bin X-1 with weight f, particles in index order, oy outer,
 //        oz inner), density contrast as src/gravity/pm_solver.cpp:700-759.
-//   k_fft_*      r2c / Green / c2r Poisson solve (src/mesh/fft_backend.cpp,
+//   k_fft_*      (hy_fft_passes.hpp) r2c / Green / c2r Poisson solve (src/mesh/fft_backend.cpp,
 //        src/mesh/green_function.cpp:165-241).
 //   k_gather_kdk fused centered-gradient CIC gather
 //        (src/mesh/mass_assignment_gradient.cpp:117-206) with the K2 kick,
@@ -168,6 +168,71 @@ __global__ void k_scatter(uint64_t n, const uint32_t* __restrict__ keys,
     }
 }
 
+// Scatter with keys made by an earlier call: each key is checked against the
+// bucket of the current position first (ERR_STALE_KEYS on a mismatch).
+__global__ void k_scatter_checked(uint64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                                  const double* __restrict__ z, Geom g, const uint32_t* __restrict__ keys,
+                                  const uint32_t* __restrict__ ranks, const uint32_t* __restrict__ offsets,
+                                  uint32_t* __restrict__ order, uint32_t* __restrict__ err) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t k = keys[p];
+    int bx, by, bz;
+    double f;
+    const bool ok = cic_axis(x[p], g, bx, f) && cic_axis(y[p], g, by, f) && cic_axis(z[p], g, bz, f);
+    if (!ok) {
+        atomicOr(err, ERR_POSITION_NONFINITE);
+        return;
+    }
+    if (k == 0xffffffffu || bucket_of(bx, by, bz, g) != k) {
+        atomicOr(err, ERR_STALE_KEYS);
+        return;
+    }
+    order[offsets[k] + ranks[p]] = (uint32_t)p;
+}
+
+// Deposit record in bucket order: tile-local base cell (lx*T+ly)*T+lz and the
+// three CIC fractions (computed in binary64, stored in binary32).
+struct DepRecord {
+    uint32_t cell;
+    float fx, fy, fz;
+};
+
+// Scatter into bucket order, writing the particle index and its deposit
+// record.  The bucket is recomputed from the current position and must equal
+// the stored key (keys carried over from the previous step are checked here).
+__global__ void k_scatter_records(uint64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                                  const double* __restrict__ z, const double* __restrict__ masses,
+                                  double inv_mass_unit, Geom g, const uint32_t* __restrict__ keys,
+                                  const uint32_t* __restrict__ ranks, const uint32_t* __restrict__ offsets,
+                                  uint32_t* __restrict__ order, DepRecord* __restrict__ rec,
+                                  float* __restrict__ rec_mass, uint32_t* __restrict__ err) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t k = keys[p];
+    int bx, by, bz;
+    double fx, fy, fz;
+    const bool ok = cic_axis(x[p], g, bx, fx) && cic_axis(y[p], g, by, fy) && cic_axis(z[p], g, bz, fz);
+    if (!ok) {
+        atomicOr(err, ERR_POSITION_NONFINITE);
+        return;
+    }
+    if (k == 0xffffffffu || bucket_of(bx, by, bz, g) != k) {
+        atomicOr(err, ERR_STALE_KEYS);
+        return;
+    }
+    const uint32_t dst = offsets[k] + ranks[p];
+    order[dst] = (uint32_t)p;
+    const int lx = bx % g.T, ly = by % g.T, lz = bz % g.T;
+    DepRecord r;
+    r.cell = (uint32_t)((lx * g.T + ly) * g.T + lz);
+    r.fx = (float)fx;
+    r.fy = (float)fy;
+    r.fz = (float)fz;
+    rec[dst] = r;
+    if (masses) rec_mass[dst] = (float)(masses[p] * inv_mass_unit);
+}
+
 // ===========================================================================
 // Workgroup sort (ascending u64), arbitrary n, any address space.
 // Bitonic network in the "flip" form: every comparator is ascending, so
@@ -273,6 +338,7 @@ struct DepParams {
     double uniform_mass;
     const uint32_t* offsets;  // bucket offsets (nb+1)
     const uint32_t* order;    // bucketed particle indices
+    uint64_t n;               // particle count (bound of every order entry)
     double mean_mass;         // M / N_cells
     double inv_volume;        // 1 / dx^3
     double total_mass;        // M (exact input mass)
@@ -493,11 +559,13 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restr
     for (uint32_t v = threadIdx.x; v < M; v += blockDim.x) {
         int r = 0;
         while (sh_rc[r + 1] <= v) ++r;
-        const uint32_t pidx = P.order[sh_rs[r] + (v - sh_rc[r])];
+        uint32_t pidx = P.order[sh_rs[r] + (v - sh_rc[r])];
+        bool routed = pidx < P.n;  // an order slot left unwritten by a failed scatter
+        if (!routed) pidx = 0;
         const int d = sh_rd[r];
         int bx = 0, by = 0, bz = 0;
         double fx = 0.0, fy = 0.0, fz = 0.0;
-        const bool ok = cic_axis(P.x[pidx], g, bx, fx) && cic_axis(P.y[pidx], g, by, fy)
+        const bool ok = routed && cic_axis(P.x[pidx], g, bx, fx) && cic_axis(P.y[pidx], g, by, fy)
                         && cic_axis(P.z[pidx], g, bz, fz);
         int ex = (d & 4) ? 0 : bx - tx * g.T + 1;
         int ey = (d & 2) ? 0 : by - ty * g.T + 1;
@@ -563,8 +631,10 @@ constexpr int DEPFX_MAX_CELLS = DEP_MAX_T * DEP_MAX_T * DEP_MAX_T;
 
 struct DepFxParams {
     DepParams base;
-    double mass_unit;      // uniform mass, or max explicit mass
-    double fixed_to_mass;  // mass_unit * 2^-32
+    double mass_unit;               // uniform mass, or max explicit mass
+    double fixed_to_mass;           // mass_unit * 2^-32
+    const DepRecord* rec;           // deposit records in bucket order
+    const float* rec_mass;          // explicit masses / mass_unit in bucket order (or null)
 };
 
 __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, float* __restrict__ mesh) {
@@ -587,33 +657,27 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
     for (int c = threadIdx.x; c < cells; c += blockDim.x) acc[c] = 0ull;
     load_source_ranges(P.offsets, tx, ty, tz, nT, sh_rs, sh_rc, sh_rd, sh_rsrc);
     const uint32_t M = sh_rc[DEP_NRANGES];
-    const double inv_unit = 1.0 / Q.mass_unit;
     int r = 0;
     bool bad = false;
     for (uint32_t v = threadIdx.x; v < M; v += blockDim.x) {
         while (sh_rc[r + 1] <= v) ++r;
-        const uint32_t pidx = P.order[sh_rs[r] + (v - sh_rc[r])];
+        const uint32_t slot = sh_rs[r] + (v - sh_rc[r]);
+        const DepRecord rec = Q.rec[slot];
         const int d = sh_rd[r];
-        int bx = 0, by = 0, bz = 0;
-        double fx = 0.0, fy = 0.0, fz = 0.0;
-        const bool ok = cic_axis(P.x[pidx], g, bx, fx) && cic_axis(P.y[pidx], g, by, fy)
-                        && cic_axis(P.z[pidx], g, bz, fz);
-        // local base cell; a neighbour record sits just below this tile
-        const int lx = (d & 4) ? -1 : bx - tx * T;
-        const int ly = (d & 2) ? -1 : by - ty * T;
-        const int lz = (d & 1) ? -1 : bz - tz * T;
-        // the record must lie where its bucket says (stale buckets are rejected)
-        const int ox_ = bx - ((tx - ((d >> 2) & 1) + nT) % nT) * T;
-        const int oy_ = by - ((ty - ((d >> 1) & 1) + nT) % nT) * T;
-        const int oz_ = bz - ((tz - (d & 1) + nT) % nT) * T;
-        const int cls = sh_rsrc[r];
-        const bool placed = ok && ox_ >= 0 && ox_ < T && oy_ >= 0 && oy_ < T && oz_ >= 0 && oz_ < T
-                            && (d == 0 || (((ox_ == T - 1) << 2 | (oy_ == T - 1) << 1 | (oz_ == T - 1)) == cls));
-        if (!placed) { bad = true; continue; }
-        const float m = P.masses ? (float)(P.masses[pidx] * inv_unit) : 1.0f;
-        const float wx[2] = {(float)(1.0 - fx), (float)fx};
-        const float wy[2] = {(float)(1.0 - fy), (float)fy};
-        const float wz[2] = {(float)(1.0 - fz), (float)fz};
+        const int c = (int)rec.cell;
+        const int cz_ = c % T, cy_ = (c / T) % T, cx_ = c / (T * T);
+        // a neighbour record sits just below this tile along its shifted axes
+        const int lx = (d & 4) ? -1 : cx_;
+        const int ly = (d & 2) ? -1 : cy_;
+        const int lz = (d & 1) ? -1 : cz_;
+        if (((d & 4) && cx_ != T - 1) || ((d & 2) && cy_ != T - 1) || ((d & 1) && cz_ != T - 1)) {
+            bad = true;
+            continue;
+        }
+        const float m = Q.rec_mass ? Q.rec_mass[slot] : 1.0f;
+        const float wx[2] = {1.0f - rec.fx, rec.fx};
+        const float wy[2] = {1.0f - rec.fy, rec.fy};
+        const float wz[2] = {1.0f - rec.fz, rec.fz};
 #pragma unroll
         for (int ox = 0; ox < 2; ++ox) {
             const int cx = lx + ox;
@@ -672,233 +736,6 @@ __global__ void k_reduce_sum(const double* __restrict__ v, uint64_t n, double* _
 }
 
 // ===========================================================================
-// FFT passes.  Layouts (NH = N/2+1):
-//   R  [x][y][z]   real, N^3            (density source in, potential out)
-//   A  [x][kz][y]  complex, N*NH*N      (after the z pass)
-//   B  [kz][ky][x] complex, NH*N*N      (after the y pass; x pass in place)
-// ===========================================================================
-template <typename T>
-struct FftArgs {
-    T* R;
-    cplx<T>* A;
-    cplx<T>* B;
-    const cplx<T>* tw;   // exp(-2 pi i m / N), m < N
-    const T* green_lap;  // discrete Laplacian symbol per index (binary64 derived)
-    const T* green_win;  // CIC window per index (deconvolution)
-    double green_num;    // 4 pi G
-    double norm;         // 1 / N^3
-    int deconvolve;
-    uint32_t* err;
-};
-
-// Staged copies with compile-time trip counts: every thread issues all of its
-// loads before its first dependent store, so each wave keeps PER requests in
-// flight instead of one.
-template <int COUNT, int NT, typename C, typename Load, typename Store>
-__device__ __forceinline__ void staged_copy(int tid, Load load, Store store) {
-    constexpr int PER = (COUNT + NT - 1) / NT;
-    C v[PER];
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-        const int l = tid + i * NT;
-        if (COUNT % NT == 0 || l < COUNT) v[i] = load(l);
-    }
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-        const int l = tid + i * NT;
-        if (COUNT % NT == 0 || l < COUNT) store(l, v[i]);
-    }
-}
-
-// z pass: real rows -> half spectrum, written transposed into A.
-template <typename T, int N>
-__global__ void __launch_bounds__((RowsCfg<T, N / 2, N>::THREADS)) k_fft_z_r2c(FftArgs<T> a) {
-    using C = cplx<T>;
-    constexpr int LEN = N / 2;
-    constexpr int NH = N / 2 + 1;
-    using RC = RowsCfg<T, LEN, N>;
-    constexpr int ROWS = RC::ROWS, RS = RC::RS, NT = RC::THREADS;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    C* lds = reinterpret_cast<C*>(smem_raw);
-    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-    const int yb = b % (N / ROWS), x = b / (N / ROWS);
-    const int y0 = yb * ROWS;
-    const int tid = threadIdx.x;
-    const C* src = reinterpret_cast<const C*>(a.R + ((uint64_t)x * N + y0) * N);
-    staged_copy<ROWS * LEN, NT, C>(
-        tid, [&](int l) { return src[(l / LEN) * LEN + (l % LEN)]; },
-        [&](int l, C v) { lds[(l / LEN) * RS + lds_slot(l % LEN)] = v; });
-    __syncthreads();
-    fft_rows<C, LEN, ROWS, RS, -1>(lds, a.tw, 2, tid);
-    // untangle: X[k] = E + W^k O, E = (Z[k]+conj Z[L-k])/2, O = -i/2 (Z[k]-conj Z[L-k])
-    C* dst = a.A + (uint64_t)x * NH * N + y0;
-    staged_copy<ROWS * NH, NT, C>(
-        tid,
-        [&](int l) {
-            const int r = l % ROWS, k = l / ROWS;
-            const C zk = lds[r * RS + lds_slot(k % LEN)];
-            const C zc = cconj(lds[r * RS + lds_slot((LEN - k) % LEN)]);
-            const C e = {(T)0.5 * (zk.x + zc.x), (T)0.5 * (zk.y + zc.y)};
-            const C d = csub(zk, zc);
-            const C o = {(T)0.5 * d.y, (T)-0.5 * d.x};
-            return cadd(e, cmul(a.tw[k], o));
-        },
-        [&](int l, C v) { dst[(uint64_t)(l / ROWS) * N + (l % ROWS)] = v; });
-}
-
-// y pass forward: A rows over y -> B[kz][ky][x]
-template <typename T, int N>
-__global__ void __launch_bounds__((RowsCfg<T, N, N>::THREADS)) k_fft_y_fwd(FftArgs<T> a) {
-    using C = cplx<T>;
-    constexpr int NH = N / 2 + 1;
-    using RC = RowsCfg<T, N, N>;
-    constexpr int ROWS = RC::ROWS, RS = RC::RS, NT = RC::THREADS;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    C* lds = reinterpret_cast<C*>(smem_raw);
-    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-    const int xb = b % (N / ROWS), kz = b / (N / ROWS);
-    const int x0 = xb * ROWS;
-    const int tid = threadIdx.x;
-    staged_copy<ROWS * N, NT, C>(
-        tid, [&](int l) { return a.A[((uint64_t)(x0 + l / N) * NH + kz) * N + (l % N)]; },
-        [&](int l, C v) { lds[(l / N) * RS + lds_slot(l % N)] = v; });
-    __syncthreads();
-    fft_rows<C, N, ROWS, RS, -1>(lds, a.tw, 1, tid);
-    C* dst = a.B + (uint64_t)kz * N * N + x0;
-    staged_copy<ROWS * N, NT, C>(
-        tid, [&](int l) { return lds[(l % ROWS) * RS + lds_slot(l / ROWS)]; },
-        [&](int l, C v) { dst[(uint64_t)(l / ROWS) * N + (l % ROWS)] = v; });
-}
-
-// x pass: forward x FFT, Green multiply (with 1/N^3), backward x FFT, in place on B.
-template <typename T, int N>
-__global__ void __launch_bounds__((RowsCfg<T, N, (N / 2 + 1) * N>::THREADS)) k_fft_x_green(FftArgs<T> a) {
-    using C = cplx<T>;
-    constexpr int NH = N / 2 + 1;
-    using RC = RowsCfg<T, N, NH * N>;
-    constexpr int ROWS = RC::ROWS, RS = RC::RS, NT = RC::THREADS;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    C* lds = reinterpret_cast<C*>(smem_raw);
-    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-    const uint64_t q0 = (uint64_t)b * ROWS;  // row index q = kz*N + ky
-    const int tid = threadIdx.x;
-    C* rows = a.B + q0 * N;
-    staged_copy<ROWS * N, NT, C>(
-        tid, [&](int l) { return rows[l]; }, [&](int l, C v) { lds[(l / N) * RS + lds_slot(l % N)] = v; });
-    __syncthreads();
-    fft_rows<C, N, ROWS, RS, -1>(lds, a.tw, 1, tid);
-    constexpr int PER = ROWS * N / NT;
-    static_assert(ROWS * N % NT == 0, "x pass covers whole rows");
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-        const int l = tid + i * NT;
-        const int r = l / N, kx = l % N;
-        const uint64_t q = q0 + r;
-        const int kz = (int)(q / N), ky = (int)(q % N);
-        T f;
-        if (kx == 0 && ky == 0 && kz == 0) {
-            f = (T)0;
-        } else {
-            const T D = a.green_lap[kx] + a.green_lap[ky] + a.green_lap[kz];
-            T ff = (T)a.green_num / D;
-            if (a.deconvolve) {
-                const T w = a.green_win[kx] * a.green_win[ky] * a.green_win[kz];
-                ff /= (w * w);
-            }
-            f = ff * (T)a.norm;
-        }
-        C v = lds[r * RS + lds_slot(kx)];
-        v.x *= f;
-        v.y *= f;
-        lds[r * RS + lds_slot(kx)] = v;
-    }
-    __syncthreads();
-    fft_rows<C, N, ROWS, RS, +1>(lds, a.tw, 1, tid);
-    staged_copy<ROWS * N, NT, C>(
-        tid, [&](int l) { return lds[(l / N) * RS + lds_slot(l % N)]; }, [&](int l, C v) { rows[l] = v; });
-}
-
-// y pass backward: B[kz][ky][x] -> A rows over y
-template <typename T, int N>
-__global__ void __launch_bounds__((RowsCfg<T, N, N>::THREADS)) k_fft_y_bwd(FftArgs<T> a) {
-    using C = cplx<T>;
-    constexpr int NH = N / 2 + 1;
-    using RC = RowsCfg<T, N, N>;
-    constexpr int ROWS = RC::ROWS, RS = RC::RS, NT = RC::THREADS;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    C* lds = reinterpret_cast<C*>(smem_raw);
-    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-    const int xb = b % (N / ROWS), kz = b / (N / ROWS);
-    const int x0 = xb * ROWS;
-    const int tid = threadIdx.x;
-    const C* src = a.B + (uint64_t)kz * N * N + x0;
-    staged_copy<ROWS * N, NT, C>(
-        tid, [&](int l) { return src[(uint64_t)(l / ROWS) * N + (l % ROWS)]; },
-        [&](int l, C v) { lds[(l % ROWS) * RS + lds_slot(l / ROWS)] = v; });
-    __syncthreads();
-    fft_rows<C, N, ROWS, RS, +1>(lds, a.tw, 1, tid);
-    staged_copy<ROWS * N, NT, C>(
-        tid, [&](int l) { return lds[(l / N) * RS + lds_slot(l % N)]; },
-        [&](int l, C v) { a.A[((uint64_t)(x0 + l / N) * NH + kz) * N + (l % N)] = v; });
-}
-
-// z pass backward (c2r): A[x][kz][y] -> R rows; flags a non-finite potential.
-template <typename T, int N>
-__global__ void __launch_bounds__((RowsCfg<T, N / 2, N>::THREADS)) k_fft_z_c2r(FftArgs<T> a) {
-    using C = cplx<T>;
-    constexpr int LEN = N / 2;
-    constexpr int NH = N / 2 + 1;
-    using RC = RowsCfg<T, LEN, N>;
-    constexpr int ROWS = RC::ROWS, RS = RC::RS, NT = RC::THREADS;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    C* lds = reinterpret_cast<C*>(smem_raw);
-    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-    const int yb = b % (N / ROWS), x = b / (N / ROWS);
-    const int y0 = yb * ROWS;
-    const int tid = threadIdx.x;
-    const C* src = a.A + (uint64_t)x * NH * N + y0;
-    staged_copy<ROWS * NH, NT, C>(
-        tid, [&](int l) { return src[(uint64_t)(l / ROWS) * N + (l % ROWS)]; },
-        [&](int l, C v) {
-            const int r = l % ROWS, k = l / ROWS;
-            if (k == 0 || k == LEN) v.y = (T)0;  // c2r reads only the independent half
-            lds[r * RS + lds_slot(k)] = v;
-        });
-    __syncthreads();
-    // Z'[k] = (X[k] + conj X[L-k]) + i (X[k] - conj X[L-k]) W^-k
-    for (int l = tid; l < ROWS * (LEN / 2 + 1); l += NT) {
-        const int r = l % ROWS, k = l / ROWS;
-        const C xk = lds[r * RS + lds_slot(k)];
-        const C xm = lds[r * RS + lds_slot(LEN - k)];
-        const C wk = cconj(a.tw[k]);
-        const C s1 = cadd(xk, cconj(xm));
-        const C d1 = cmul(csub(xk, cconj(xm)), wk);
-        const C zk = {s1.x - d1.y, s1.y + d1.x};
-        C zm = {(T)0, (T)0};
-        const bool pair = k > 0 && (LEN - k) != k;
-        if (pair) {
-            const C wm = cconj(a.tw[LEN - k]);
-            const C s2 = cadd(xm, cconj(xk));
-            const C d2 = cmul(csub(xm, cconj(xk)), wm);
-            zm = {s2.x - d2.y, s2.y + d2.x};
-        }
-        lds[r * RS + lds_slot(k)] = zk;
-        if (pair) lds[r * RS + lds_slot(LEN - k)] = zm;
-    }
-    __syncthreads();
-    fft_rows<C, LEN, ROWS, RS, +1>(lds, a.tw, 2, tid);
-    int mybad = 0;
-    C* dst = reinterpret_cast<C*>(a.R + ((uint64_t)x * N + y0) * N);
-    staged_copy<ROWS * LEN, NT, C>(
-        tid, [&](int l) { return lds[(l / LEN) * RS + lds_slot(l % LEN)]; },
-        [&](int l, C v) {
-            if (!isfinite(v.x) || !isfinite(v.y)) mybad = 1;
-            dst[l] = v;
-        });
-    if (__ballot(mybad) && lane_id() == 0) atomicOr(a.err, ERR_FIELD_NONFINITE);
-}
-
-// ===========================================================================
 // Fused centered-gradient gather + kick/drift
 // ===========================================================================
 template <typename T, typename PT>
@@ -924,6 +761,9 @@ struct GatherArgs {
     T* g_x;                     // optional: g written (transactional force mode)
     T* g_y;
     T* g_z;
+    uint32_t* next_counts;      // optional: bucket counts/ranks of the drifted positions
+    uint32_t* next_keys;
+    uint32_t* next_ranks;
     double k_prev;              // K2 of the previous step
     double k_next;              // K1 of the next step
     double d_next;              // drift factor of the next step
@@ -937,98 +777,120 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
     // each XCD walks a contiguous particle range (spatially compact for
     // lattice-ordered particles), so potential lines are reused in its L2
     const uint64_t p = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (p >= a.n) return;
     const Geom& g = a.g;
     const int N = g.N;
-    int ix, iy, iz;
-    double fx, fy, fz;
-    const double xp = a.x_in[p], yp = a.y_in[p], zp = a.z_in[p];
-    if (!(cic_axis(xp, g, ix, fx) && cic_axis(yp, g, iy, fy) && cic_axis(zp, g, iz, fz))) {
-        atomicOr(a.err, ERR_POSITION_NONFINITE);
-        return;
-    }
-    const T wx[2] = {(T)(1.0 - fx), (T)fx};
-    const T wy[2] = {(T)(1.0 - fy), (T)fy};
-    const T wz[2] = {(T)(1.0 - fz), (T)fz};
-    const T* phi = a.phi;
-    const uint64_t NN = (uint64_t)N * N;
-    auto at = [&](int cx, int cy, int cz) -> T {
-        return phi[(uint64_t)cx * NN + (uint64_t)cy * N + cz];
-    };
-    const T two_dx = (T)(2.0 * g.dx);
-    T gx = 0, gy = 0, gz = 0;
+    uint32_t fail = 0;
+    bool live = p < a.n;
+    uint32_t next_key = 0;
+    bool key_ok = false;
+    if (live) {
+        int ix = 0, iy = 0, iz = 0;
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        const double xp = a.x_in[p], yp = a.y_in[p], zp = a.z_in[p];
+        if (!(cic_axis(xp, g, ix, fx) && cic_axis(yp, g, iy, fy) && cic_axis(zp, g, iz, fz))) {
+            fail = ERR_POSITION_NONFINITE;
+        } else {
+            const T wx[2] = {(T)(1.0 - fx), (T)fx};
+            const T wy[2] = {(T)(1.0 - fy), (T)fy};
+            const T wz[2] = {(T)(1.0 - fz), (T)fz};
+            const T* phi = a.phi;
+            const uint64_t NN = (uint64_t)N * N;
+            auto at = [&](int cx, int cy, int cz) -> T {
+                return phi[(uint64_t)cx * NN + (uint64_t)cy * N + cz];
+            };
+            const T two_dx = (T)(2.0 * g.dx);
+            T gx = 0, gy = 0, gz = 0;
 #pragma unroll
-    for (int ox = 0; ox < 2; ++ox) {
-        const int cx = wrap_cell(ix + ox, N);
-        const int cxp = wrap_cell(cx + 1, N), cxm = wrap_cell(cx - 1, N);
+            for (int ox = 0; ox < 2; ++ox) {
+                const int cx = wrap_cell(ix + ox, N);
+                const int cxp = wrap_cell(cx + 1, N), cxm = wrap_cell(cx - 1, N);
 #pragma unroll
-        for (int oy = 0; oy < 2; ++oy) {
-            const int cy = wrap_cell(iy + oy, N);
-            const int cyp = wrap_cell(cy + 1, N), cym = wrap_cell(cy - 1, N);
+                for (int oy = 0; oy < 2; ++oy) {
+                    const int cy = wrap_cell(iy + oy, N);
+                    const int cyp = wrap_cell(cy + 1, N), cym = wrap_cell(cy - 1, N);
 #pragma unroll
-            for (int oz = 0; oz < 2; ++oz) {
-                const int cz = wrap_cell(iz + oz, N);
-                const int czp = wrap_cell(cz + 1, N), czm = wrap_cell(cz - 1, N);
-                const T w = wx[ox] * wy[oy] * wz[oz];
-                const T dxv = at(cxp, cy, cz) - at(cxm, cy, cz);
-                const T dyv = at(cx, cyp, cz) - at(cx, cym, cz);
-                const T dzv = at(cx, cy, czp) - at(cx, cy, czm);
-                gx += w * (-dxv / two_dx);
-                gy += w * (-dyv / two_dx);
-                gz += w * (-dzv / two_dx);
+                    for (int oz = 0; oz < 2; ++oz) {
+                        const int cz = wrap_cell(iz + oz, N);
+                        const int czp = wrap_cell(cz + 1, N), czm = wrap_cell(cz - 1, N);
+                        const T w = wx[ox] * wy[oy] * wz[oz];
+                        const T dxv = at(cxp, cy, cz) - at(cxm, cy, cz);
+                        const T dyv = at(cx, cyp, cz) - at(cx, cym, cz);
+                        const T dzv = at(cx, cy, czp) - at(cx, cy, czm);
+                        gx += w * (-dxv / two_dx);
+                        gy += w * (-dyv / two_dx);
+                        gz += w * (-dzv / two_dx);
+                    }
+                }
+            }
+            if (!isfinite(gx) || !isfinite(gy) || !isfinite(gz)) {
+                fail = ERR_FORCE_NONFINITE;
+            } else {
+                if (a.acc_x) {
+                    a.acc_x[p] += (double)gx;
+                    a.acc_y[p] += (double)gy;
+                    a.acc_z[p] += (double)gz;
+                }
+                if (a.g_x) {
+                    a.g_x[p] = gx;
+                    a.g_y[p] = gy;
+                    a.g_z[p] = gz;
+                }
+                if (a.has_prev || a.has_next) {
+                    PT qx = a.px_in[p], qy = a.py_in[p], qz = a.pz_in[p];
+                    if (!isfinite(qx) || !isfinite(qy) || !isfinite(qz)) {
+                        fail = ERR_MOMENTUM_NONFINITE;
+                    } else {
+                        if (a.has_prev) {
+                            const PT k = (PT)a.k_prev;
+                            qx = qx + (PT)gx * k;
+                            qy = qy + (PT)gy * k;
+                            qz = qz + (PT)gz * k;
+                        }
+                        double nx = xp, ny = yp, nz = zp;
+                        if (a.has_next) {
+                            const PT k = (PT)a.k_next;
+                            qx = qx + (PT)gx * k;
+                            qy = qy + (PT)gy * k;
+                            qz = qz + (PT)gz * k;
+                            const double D = a.d_next;
+                            const double L = g.L;
+                            const double ddx = (double)qx * D, ddy = (double)qy * D, ddz = (double)qz * D;
+                            nx = wrap_pos(wrap_pos(xp, L) + minimum_image(ddx, L), L);
+                            ny = wrap_pos(wrap_pos(yp, L) + minimum_image(ddy, L), L);
+                            nz = wrap_pos(wrap_pos(zp, L) + minimum_image(ddz, L), L);
+                        }
+                        if (!isfinite(qx) || !isfinite(qy) || !isfinite(qz) || !isfinite(nx)
+                            || !isfinite(ny) || !isfinite(nz)) {
+                            fail = ERR_UPDATE_NONFINITE;
+                        } else {
+                            a.px_out[p] = qx;
+                            a.py_out[p] = qy;
+                            a.pz_out[p] = qz;
+                            a.x_out[p] = nx;
+                            a.y_out[p] = ny;
+                            a.z_out[p] = nz;
+                            if (a.next_counts) {
+                                // bucket of the drifted position for the next deposit
+                                int bx, by, bz;
+                                double f;
+                                key_ok = cic_axis(nx, g, bx, f) && cic_axis(ny, g, by, f)
+                                         && cic_axis(nz, g, bz, f);
+                                if (key_ok) next_key = bucket_of(bx, by, bz, g);
+                            }
+                        }
+                    }
+                }
             }
         }
+        if (fail) atomicOr(a.err, fail);
     }
-    if (!isfinite(gx) || !isfinite(gy) || !isfinite(gz)) {
-        atomicOr(a.err, ERR_FORCE_NONFINITE);
-        return;
+    if (a.next_counts) {  // uniform across the grid: every lane reaches the ballots
+        const uint32_t rank = aggregated_rank(a.next_counts, next_key, live && key_ok);
+        if (live) {
+            a.next_keys[p] = key_ok ? next_key : 0xffffffffu;
+            a.next_ranks[p] = rank;
+        }
     }
-    if (a.acc_x) {
-        a.acc_x[p] += (double)gx;
-        a.acc_y[p] += (double)gy;
-        a.acc_z[p] += (double)gz;
-    }
-    if (a.g_x) {
-        a.g_x[p] = gx;
-        a.g_y[p] = gy;
-        a.g_z[p] = gz;
-    }
-    if (!a.has_prev && !a.has_next) return;
-    PT qx = a.px_in[p], qy = a.py_in[p], qz = a.pz_in[p];
-    if (!isfinite(qx) || !isfinite(qy) || !isfinite(qz)) {
-        atomicOr(a.err, ERR_MOMENTUM_NONFINITE);
-        return;
-    }
-    if (a.has_prev) {
-        const PT k = (PT)a.k_prev;
-        qx = qx + (PT)gx * k;
-        qy = qy + (PT)gy * k;
-        qz = qz + (PT)gz * k;
-    }
-    double nx = xp, ny = yp, nz = zp;
-    if (a.has_next) {
-        const PT k = (PT)a.k_next;
-        qx = qx + (PT)gx * k;
-        qy = qy + (PT)gy * k;
-        qz = qz + (PT)gz * k;
-        const double D = a.d_next;
-        const double L = g.L;
-        const double ddx = (double)qx * D, ddy = (double)qy * D, ddz = (double)qz * D;
-        nx = wrap_pos(wrap_pos(xp, L) + minimum_image(ddx, L), L);
-        ny = wrap_pos(wrap_pos(yp, L) + minimum_image(ddy, L), L);
-        nz = wrap_pos(wrap_pos(zp, L) + minimum_image(ddz, L), L);
-    }
-    if (!isfinite(qx) || !isfinite(qy) || !isfinite(qz) || !isfinite(nx) || !isfinite(ny)
-        || !isfinite(nz)) {
-        atomicOr(a.err, ERR_UPDATE_NONFINITE);
-        return;
-    }
-    a.px_out[p] = qx;
-    a.py_out[p] = qy;
-    a.pz_out[p] = qz;
-    a.x_out[p] = nx;
-    a.y_out[p] = ny;
-    a.z_out[p] = nz;
 }
 
 // Transactional commit for the reference-shaped force entry: acc += g.

@@ -13,8 +13,9 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from ._lib import (HY_ASYNC, HY_HAS_NEXT, HY_HAS_PREV, HY_HOST_POINTERS, HY_PREC_FP32_MIXED,
-                   HY_PREC_FP64, check, hy_pm_desc, hy_pm_info, lib)
+from ._lib import (FFT_PASS_NAMES, HY_ASYNC, HY_FFT_PASSES, HY_HAS_NEXT, HY_HAS_PREV,
+                   HY_HOST_POINTERS, HY_PREC_FP32_MIXED, HY_PREC_FP64, HY_REUSE_KEYS, check,
+                   hy_fft_tuning, hy_pm_desc, hy_pm_info, lib)
 
 PRECISIONS = {"fp64": HY_PREC_FP64, "fp32": HY_PREC_FP32_MIXED, "fp32_mixed": HY_PREC_FP32_MIXED}
 
@@ -126,6 +127,17 @@ class TimeStepper:
 
 # --------------------------------------------------------------------------
 # PM solver
+def _tuning_dict(t) -> dict:
+    nv = int(t.variants)
+    return {"variant": {FFT_PASS_NAMES[p]: int(t.variant[p]) for p in range(HY_FFT_PASSES)},
+            "plane_group": int(t.plane_group),
+            "rows": {FFT_PASS_NAMES[p]: [int(t.rows[p][v]) for v in range(nv)]
+                     for p in range(HY_FFT_PASSES)},
+            "pass_ms": {FFT_PASS_NAMES[p]: [round(float(t.pass_ms[p][v]), 4) for v in range(nv)]
+                        for p in range(HY_FFT_PASSES)},
+            "group_ms": {g: round(float(t.group_ms[i]), 4) for i, g in enumerate((0, 16, 64, 128))}}
+
+
 # --------------------------------------------------------------------------
 class PMSolver:
     """gravity::PMSolver (include/cosmo_nbody/gravity/pm_solver.hpp:53-70),
@@ -140,6 +152,16 @@ class PMSolver:
         self.box_size = float(box_size)
         self.mesh_size = int(mesh_size)
         self.precision = PRECISIONS[precision]
+        if stream is None:
+            # run on the caller's current torch stream when it is not the
+            # default stream (the private stream is ordered with the default one)
+            try:
+                torch = _torch()
+                if torch.cuda.is_available():
+                    cs = torch.cuda.current_stream(device).cuda_stream
+                    stream = cs or None
+            except ImportError:
+                pass
         d = hy_pm_desc(box_size=self.box_size, mesh_size=self.mesh_size,
                        deconvolve_cic=int(bool(deconvolve_cic)), precision=self.precision,
                        device=device, stream=stream, max_particles=int(max_particles))
@@ -174,6 +196,27 @@ class PMSolver:
         inf = hy_pm_info()
         check(lib().hy_pm_get_info(self._h, C.byref(inf)))
         return inf
+
+    # -- FFT launch plan (FFTW_MEASURE counterpart)
+    def tuning(self) -> dict:
+        t = hy_fft_tuning()
+        check(lib().hy_pm_get_tuning(self._h, C.byref(t)))
+        return _tuning_dict(t)
+
+    def set_tuning(self, variants, plane_group: int = 0) -> None:
+        t = hy_fft_tuning()
+        check(lib().hy_pm_get_tuning(self._h, C.byref(t)))
+        for p, v in enumerate(variants):
+            t.variant[p] = int(v)
+        t.plane_group = int(plane_group)
+        check(lib().hy_pm_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, reps: int = 3) -> dict:
+        """Times every compiled launch variant of each FFT pass on this
+        context's buffers and keeps the fastest (results are unchanged)."""
+        t = hy_fft_tuning()
+        check(lib().hy_pm_autotune(self._h, int(reps), C.byref(t)))
+        return _tuning_dict(t)
 
     @property
     def mesh_dtype(self):
@@ -232,10 +275,15 @@ class PMSolver:
 
     def kdk(self, src: "ParticleState", dst: "ParticleState", uniform_mass: float,
             k_prev: Optional[float], k_next: Optional[float], d_next: Optional[float],
-            masses=None, acc=None, asynchronous: bool = False) -> None:
+            masses=None, acc=None, asynchronous: bool = False, reuse_keys: bool = False) -> None:
         """Force at src positions fused with the K2 kick of the previous step
-        (k_prev), the K1 kick of the next step (k_next) and its drift (d_next)."""
-        flags = 0
+        (k_prev), the K1 kick of the next step (k_next) and its drift (d_next).
+
+        reuse_keys: src holds the previous kdk call's output positions, so the
+        deposit bucket keys made by that call are reused (they are checked
+        against the positions, and recomputed on a mismatch unless
+        asynchronous)."""
+        flags = HY_REUSE_KEYS if reuse_keys else 0
         if k_prev is not None:
             flags |= HY_HAS_PREV
         if k_next is not None:
@@ -299,6 +347,7 @@ class LeapfrogIntegrator:
         self.masses = masses
         self._pending_k2: Optional[float] = None
         self._spare: Optional[ParticleState] = None
+        self._last_out: Optional[ParticleState] = None
 
     def step(self, state: ParticleState, step: TimeStep, asynchronous: bool = False) -> ParticleState:
         D = self.factors.drift_factor(step.a_begin, step.a_end)
@@ -306,17 +355,20 @@ class LeapfrogIntegrator:
         K2 = self.factors.kick_factor(step.a_half, step.a_end)
         out = self._spare if self._spare is not None else state.empty_like()
         self.solver.kdk(state, out, self.uniform_mass, self._pending_k2, K1, D,
-                        masses=self.masses, asynchronous=asynchronous)
+                        masses=self.masses, asynchronous=asynchronous,
+                        reuse_keys=state is self._last_out)
         self._spare = state
         self._pending_k2 = K2
+        self._last_out = out
         return out
 
     def finalize(self, state: ParticleState, acc=None) -> ParticleState:
         out = self._spare if self._spare is not None else state.empty_like()
         self.solver.kdk(state, out, self.uniform_mass, self._pending_k2, None, None,
-                        masses=self.masses, acc=acc)
+                        masses=self.masses, acc=acc, reuse_keys=state is self._last_out)
         self._pending_k2 = None
         self._spare = state
+        self._last_out = None
         # positions are unchanged by a pure kick; copy them across
         out.x.copy_(state.x)
         out.y.copy_(state.y)

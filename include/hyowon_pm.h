@@ -14,6 +14,9 @@
  *  - Particle arrays are SoA, binary64 positions in [0, L) (any finite value is
  *    wrapped as the reference does).  Pointers are device pointers unless
  *    HY_HOST_POINTERS is set.
+ *  - Device inputs are read in stream order on the context's stream: work on
+ *    the legacy default stream is ordered before the private stream's; work on
+ *    other streams must be ordered by the caller (or pass that stream).
  *  - One context owns one HIP stream and its mesh workspace; calls on a
  *    context must be serialised (as PMSolver, include/cosmo_nbody/mesh/
  *    mass_assignment.hpp:44-45).  All device memory is allocated by
@@ -53,6 +56,8 @@ typedef enum {
 #define HY_HAS_PREV       0x2u  /* hy_pm_kdk: apply K2 of the previous step */
 #define HY_HAS_NEXT       0x4u  /* hy_pm_kdk: apply K1 of the next step and drift */
 #define HY_ASYNC          0x8u  /* do not synchronise; errors are reported by hy_pm_check */
+#define HY_REUSE_KEYS     0x10u /* hy_pm_kdk: x_in/y_in/z_in are the previous hy_pm_kdk call's
+                                   outputs on this context; reuse the bucket keys it made */
 
 typedef struct {
     double box_size;        /* L (Mpc/h) */
@@ -60,7 +65,8 @@ typedef struct {
     int deconvolve_cic;     /* divide the Green function by W(k)^2 */
     int precision;          /* hy_precision */
     int device;             /* HIP device ordinal */
-    void* stream;           /* hipStream_t to use, or NULL for a private stream */
+    void* stream;           /* hipStream_t to use, or NULL for a private blocking stream
+                               (ordered with the legacy default stream) */
     uint64_t max_particles; /* capacity; particle counts above it are rejected */
 } hy_pm_desc;
 
@@ -101,7 +107,12 @@ int hy_pm_force(hy_pm_ctx* ctx, uint64_t n, const double* x, const double* y, co
  *   HY_HAS_NEXT: p += g * k_next          (K1 of the step that starts here)
  *                x = wrap(wrap(x) + minimum_image(p * d_next))
  * Momenta are binary64 (HY_PREC_FP64) or binary32 (HY_PREC_FP32_MIXED).
- * Outputs may alias inputs.  acc_x/y/z (optional, binary64) receive g. */
+ * Outputs may alias inputs.  acc_x/y/z (optional, binary64) receive g.
+ * With HY_HAS_NEXT the call also computes the deposit bucket keys of the
+ * drifted positions; HY_REUSE_KEYS on the next call skips recomputing them.
+ * Reused keys are checked against the positions: synchronous calls fall back
+ * to fresh keys when they differ, HY_ASYNC calls report logic_error through
+ * hy_pm_check (outputs of that call are then undefined). */
 int hy_pm_kdk(hy_pm_ctx* ctx, uint64_t n, const double* x_in, const double* y_in,
               const double* z_in, const void* px_in, const void* py_in, const void* pz_in,
               double* x_out, double* y_out, double* z_out, void* px_out, void* py_out,
@@ -138,7 +149,8 @@ int hy_pm_bucket_order(hy_pm_ctx* ctx, uint64_t n, const double* x, const double
 int hy_pm_xplane_order(hy_pm_ctx* ctx, uint64_t n, const double* x, uint32_t* order,
                        uint32_t* offsets);
 
-/* Waits for outstanding work and reports errors raised by HY_ASYNC calls. */
+/* Waits for outstanding work and reports errors raised by HY_ASYNC calls
+ * (each error once: the context's error word is cleared when reported). */
 int hy_pm_check(hy_pm_ctx* ctx);
 
 /* Live stage timing with HIP events recorded on the context's stream (the
@@ -161,6 +173,27 @@ enum {
 };
 int hy_pm_profile(hy_pm_ctx* ctx, int enable, uint64_t max_calls);
 int hy_pm_profile_read(hy_pm_ctx* ctx, double* stage_ms, uint64_t* stage_count, uint64_t* calls);
+
+/* Measured launch plan of the FFT passes.  Counterpart of the reference's
+ * FFTW_MEASURE planning (src/mesh/fft_backend.cpp:967-1003): each pass has
+ * compiled launch variants (rows per workgroup, hence workgroups per CU and
+ * the run length of its transposing stores); hy_pm_autotune times every
+ * variant on the context's own buffers and keeps the fastest, then picks the
+ * kz plane-group size of the y / x+Green / y^-1 passes.  Results do not depend
+ * on the plan: every variant computes the same values bit for bit. */
+#define HY_FFT_PASSES 5        /* z r2c, y, x+Green+x^-1, y^-1, z c2r */
+#define HY_FFT_MAX_VARIANTS 4
+typedef struct {
+    uint32_t variant[HY_FFT_PASSES];  /* launch variant per pass */
+    uint32_t plane_group;             /* kz planes per y/x/y^-1 launch group, 0 = all at once */
+    uint32_t variants;                /* variants compiled for this mesh size (1 or 4) */
+    uint32_t rows[HY_FFT_PASSES][HY_FFT_MAX_VARIANTS];  /* rows per workgroup of each variant */
+    float pass_ms[HY_FFT_PASSES][HY_FFT_MAX_VARIANTS];  /* last autotune timings, 0 = not timed */
+    float group_ms[4];                /* last autotune: plane groups 0, 16, 64, 128 */
+} hy_fft_tuning;
+int hy_pm_get_tuning(hy_pm_ctx* ctx, hy_fft_tuning* out);
+int hy_pm_set_tuning(hy_pm_ctx* ctx, const hy_fft_tuning* in);
+int hy_pm_autotune(hy_pm_ctx* ctx, uint32_t reps, hy_fft_tuning* out);
 
 /* Host scalars of the step (binary64, long-double quadrature).
  * H(a): src/cosmology/cosmology_model.cpp:25-57,167-178.

@@ -55,6 +55,29 @@ def step_alg_bytes(n_p: int, N: int, real: int) -> float:
     return (96.0 * n_p + 24.0 * M) if real == 4 else (120.0 * n_p + 48.0 * M)
 
 
+STAGE_KERNEL = {"bucket": "hy::k_scatter_records", "deposit": "hy::k_deposit_fixed",
+                "gather_kdk": "hy::k_gather_kdk<float, float>"}
+FFT_KERNEL = {"fft_z_r2c": "k_fft_z_r2c", "fft_y_fwd": "k_fft_y_fwd", "fft_x_green": "k_fft_x_green",
+              "fft_y_bwd": "k_fft_y_bwd", "fft_z_c2r": "k_fft_z_c2r"}
+
+
+def measured_traffic(stage: str, n_side: int, N: int, prec: str, plan: dict):
+    """HBM bytes per launch of the stage's kernel from the committed calibrated
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload, if present."""
+    path = os.path.join(ROOT, "profiles", f"pm_kdk_step_{n_side}p_{N}m_{prec}_traffic.json")
+    if not os.path.exists(path) or prec != "fp32":
+        return None, None
+    with open(path) as f:
+        prof = json.load(f)
+    if stage in FFT_KERNEL:
+        v = plan["variant"][stage[4:]]
+        name = f"void hy::{FFT_KERNEL[stage]}<float, {N}, {v}>"
+    else:
+        name = STAGE_KERNEL.get(stage)
+    k = prof.get("kernels", {}).get(name)
+    return (k["traffic_bytes"] if k else None), os.path.relpath(path, ROOT)
+
+
 def run_gpu(args):
     import numpy as np
     import torch
@@ -88,8 +111,6 @@ def run_gpu(args):
         del src, dst
     pos, mom = ic.lpt_ics(n_side, L, a0, seed=1, device=dev)
     results = {}
-    if args.accuracy and "fp32" in args.precisions:
-        results["accuracy"] = accuracy(args, pos, mom, m, K1, K2, D, nsteps)
     for prec in args.precisions:
         mdt = torch.float32 if prec == "fp32" else torch.float64
         st = H.ParticleState(pos[0].clone(), pos[1].clone(), pos[2].clone(),
@@ -97,7 +118,12 @@ def run_gpu(args):
                              mom[2].to(mdt).contiguous())
         sp = st.empty_like()
         solver = H.PMSolver(L, N, precision=prec, max_particles=n)
-        plan = solver.autotune(reps=3) if args.autotune else solver.tuning()
+        if args.fft_plan:
+            var, _, grp = args.fft_plan.partition(":")
+            solver.set_tuning([int(v) for v in var.split(",")], plane_group=int(grp or 0))
+            plan = solver.tuning()
+        else:
+            plan = solver.autotune(reps=3) if args.autotune else solver.tuning()
         real = 4 if prec == "fp32" else 8
         steps = args.steps if prec == args.precisions[0] else max(2, args.fp64_steps)
         warm = args.warmup if prec == args.precisions[0] else 1
@@ -134,8 +160,12 @@ def run_gpu(args):
         solver.close()
         del st, sp
         torch.cuda.empty_cache()
-        if prec == "fp32" and args.rocfft:
-            results["rocfft"] = rocfft_compare(N, dev)
+    # comparison and accuracy runs come after the timed loops: their large
+    # transient allocations must not shape the timed contexts' memory
+    if args.rocfft and "fp32" in args.precisions:
+        results["rocfft"] = rocfft_compare(N, dev)
+    if args.accuracy and "fp32" in args.precisions:
+        results["accuracy"] = accuracy(args, pos, mom, m, K1, K2, D, nsteps)
     return n, N, results
 
 
@@ -296,6 +326,8 @@ def main():
                     help="FP32-vs-FP64 force, final-position and P(k) errors (full run)")
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
                     help="measure the FFT launch plan before timing (untimed)")
+    ap.add_argument("--fft-plan", default="",
+                    help="pin the FFT launch plan 'v0,v1,v2,v3,v4[:plane_group]' instead of autotuning")
     ap.add_argument("--rocfft", action=argparse.BooleanOptionalAction, default=True,
                     help="time rocFFT r2c/c2r beside the hand-written FFT passes")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -316,8 +348,9 @@ def main():
     stage_ms = r["stage_ms"]
     dom = max(stage_ms, key=stage_ms.get)
     ach = stage_bytes(dom, n, N, r["real"]) / (stage_ms[dom] * 1e-3)
+    traffic, traffic_src = measured_traffic(dom, args.n_side, N, main_prec, r["plan"])
     roofline = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": ach / HBM_PEAK, "traffic": None, "kernel": dom,
+                "frac": ach / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_src, "kernel": dom,
                 "kernel_ms": stage_ms[dom],
                 "alg_bytes_per_launch": stage_bytes(dom, n, N, r["real"])}
     out = {

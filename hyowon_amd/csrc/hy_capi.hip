@@ -120,6 +120,7 @@ struct hy_pm_ctx {
     uint32_t* xoffsets = nullptr;  // N+1
     double* tile_mass = nullptr;
     double* mass_total = nullptr;
+    double* mass_partials = nullptr;  // REDUCE_BLOCKS partial tile-mass sums
     uint64_t* scratch = nullptr;
     uint64_t scratch_cap = 0;
     unsigned long long* scratch_cursor = nullptr;
@@ -335,8 +336,10 @@ void launch_deposit(hy_pm_ctx* c, const hy::DepParams& P) {
                            static_cast<float*>(c->R));
     }
     launch_check("k_deposit");
-    hipLaunchKernelGGL(hy::k_reduce_sum, dim3(1), dim3(256), 0, c->stream, c->tile_mass, c->ntiles,
-                       c->mass_total);
+    hipLaunchKernelGGL(hy::k_reduce_partials, dim3(hy::REDUCE_BLOCKS), dim3(256), 0, c->stream, c->tile_mass,
+                       c->ntiles, c->mass_partials);
+    hipLaunchKernelGGL(hy::k_reduce_sum, dim3(1), dim3(256), 0, c->stream, c->mass_partials,
+                       (uint64_t)hy::REDUCE_BLOCKS, c->mass_total);
     launch_check("k_reduce_sum");
 }
 
@@ -402,8 +405,32 @@ void run_fft(hy_pm_ctx* c) {
     else launch_fft<float>(c, c->fft32);
 }
 
+// Lattice block schedule for n = S^3 particles when a 256-particle block
+// tiles z-rows exactly (S a multiple of 256); about 64K particles (the
+// blocks resident on one XCD) per column tile.  Identity otherwise.
+hy::LatticeTiling lattice_tiling(uint64_t n, uint32_t block) {
+    hy::LatticeTiling t{0, 0, 0, 0};
+    if (std::getenv("HY_NO_LATTICE_SCHEDULE")) return t;
+    const uint64_t S = (uint64_t)std::llround(std::cbrt((double)n));
+    if (S == 0 || S * S * S != n || S % block != 0 || S > 2048) return t;
+    const uint32_t ZC = (uint32_t)(S / block);
+    uint32_t rows = std::max<uint32_t>(1, 256 / ZC);  // TX*TY rows per tile
+    uint32_t TX = 1, TY = 1;
+    while (TX * TY < rows) {
+        if (TY <= TX && S % (TY * 2) == 0) TY *= 2;
+        else if (S % (TX * 2) == 0) TX *= 2;
+        else break;
+    }
+    t.S = (uint32_t)S;
+    t.TX = TX;
+    t.TY = TY;
+    t.ZC = ZC;
+    return t;
+}
+
 template <typename T, typename PT>
-void launch_gather(hy_pm_ctx* c, const hy::GatherArgs<T, PT>& a) {
+void launch_gather(hy_pm_ctx* c, hy::GatherArgs<T, PT> a) {
+    a.lat = lattice_tiling(a.n, 256);
     hipLaunchKernelGGL((hy::k_gather_kdk<T, PT>), dim3(blocks_for(a.n, 256)), dim3(256), 0, c->stream, a);
     launch_check("k_gather_kdk");
 }
@@ -429,7 +456,9 @@ void check_mass(hy_pm_ctx* c, double input, double deposited, uint64_t n) {
 void run_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z,
                  const double* masses, double uniform, double Min, bool sync, bool carried = false) {
     mark(c, 0);
-    bucketize(c, n, x, y, z, 0, c->counts, c->offsets, c->nb, c->order, carried, masses);
+    // the binary32 deposit reads records only: no particle order is written
+    uint32_t* order = c->prec == HY_PREC_FP64 ? c->order : nullptr;
+    bucketize(c, n, x, y, z, 0, c->counts, c->offsets, c->nb, order, carried, masses);
     mark(c, HY_STAGE_BUCKET + 1);
     auto plan = plan_deposit(c, n, x, y, z, masses, uniform, Min);
     launch_deposit(c, plan.P);
@@ -444,7 +473,7 @@ void run_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, con
             raise_errors(st.err & (hy::ERR_POSITION_NONFINITE | hy::ERR_MASS_INVALID));
             carried = false;
             reset_status(c);
-            bucketize(c, n, x, y, z, 0, c->counts, c->offsets, c->nb, c->order, false, masses);
+            bucketize(c, n, x, y, z, 0, c->counts, c->offsets, c->nb, order, false, masses);
             plan = plan_deposit(c, n, x, y, z, masses, uniform, Min);
             launch_deposit(c, plan.P);
             continue;
@@ -673,6 +702,7 @@ int hy_pm_create(const hy_pm_desc* desc, hy_pm_ctx** out) {
         c->xoffsets = (uint32_t*)c->alloc(((size_t)N + 1) * 4);
         c->tile_mass = (double*)c->alloc(c->ntiles * 8);
         c->mass_total = (double*)c->alloc(16);
+        c->mass_partials = (double*)c->alloc(hy::REDUCE_BLOCKS * sizeof(double));
         c->scratch_cap = 2 * cap + (1u << 16);
         c->scratch = (uint64_t*)c->alloc(c->scratch_cap * 8);
         c->scratch_cursor = (unsigned long long*)c->alloc(16);

@@ -222,7 +222,7 @@ __global__ void k_scatter_records(uint64_t n, const double* __restrict__ x, cons
         return;
     }
     const uint32_t dst = offsets[k] + ranks[p];
-    order[dst] = (uint32_t)p;
+    if (order) order[dst] = (uint32_t)p;
     const int lx = bx % g.T, ly = by % g.T, lz = bz % g.T;
     DepRecord r;
     r.cell = (uint32_t)((lx * g.T + ly) * g.T + lz);
@@ -721,7 +721,27 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
     }
 }
 
-// Fixed-order reduction of per-tile masses (single workgroup).
+// Fixed-order two-level reduction of per-tile masses: REDUCE_BLOCKS
+// workgroups sum fixed contiguous chunks into partials, then k_reduce_sum
+// adds the partials (the same order on every run).
+constexpr int REDUCE_BLOCKS = 256;
+__global__ void k_reduce_partials(const double* __restrict__ v, uint64_t n, double* __restrict__ partials) {
+    __shared__ double sh[256];
+    const uint64_t chunk = (n + REDUCE_BLOCKS - 1) / REDUCE_BLOCKS;
+    const uint64_t lo = (uint64_t)blockIdx.x * chunk;
+    const uint64_t hi = lo + chunk < n ? lo + chunk : n;
+    double s = 0.0;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += 256) s += v[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+// Fixed-order reduction (single workgroup).
 __global__ void k_reduce_sum(const double* __restrict__ v, uint64_t n, double* __restrict__ out) {
     __shared__ double sh[256];
     double s = 0.0;
@@ -738,6 +758,28 @@ __global__ void k_reduce_sum(const double* __restrict__ v, uint64_t n, double* _
 // ===========================================================================
 // Fused centered-gradient gather + kick/drift
 // ===========================================================================
+// Block schedule for particle arrays stored in Lagrangian lattice order
+// p = (ix*S + iy)*S + iz (the order of lattice initial conditions, which
+// particles keep for the whole run).  Logical block L covers 256-particle
+// chunks of z-rows; consecutive L walk a TX x TY column of lattice rows
+// (all z) before the next column, so the blocks resident on one XCD touch a
+// compact region of the potential that stays in its L2.  S == 0: identity.
+struct LatticeTiling {
+    uint32_t S, TX, TY, ZC;  // lattice side, column tile, 256-chunks per z-row
+};
+
+__device__ __forceinline__ uint32_t lattice_block(uint32_t L, const LatticeTiling& t) {
+    if (t.S == 0) return L;
+    // every quantity is below S^2 * ZC = n / 256 < 2^32
+    const uint32_t per_tile = t.TX * t.TY * t.ZC;
+    const uint32_t tile = L / per_tile, w = L % per_tile;
+    const uint32_t tiles_y = t.S / t.TY;
+    const uint32_t tx = tile / tiles_y, ty = tile % tiles_y;
+    const uint32_t zc = w % t.ZC, dy = (w / t.ZC) % t.TY, dx = w / (t.ZC * t.TY);
+    const uint32_t ix = tx * t.TX + dx, iy = ty * t.TY + dy;
+    return (ix * t.S + iy) * t.ZC + zc;
+}
+
 template <typename T, typename PT>
 struct GatherArgs {
     Geom g;
@@ -761,6 +803,7 @@ struct GatherArgs {
     T* g_x;                     // optional: g written (transactional force mode)
     T* g_y;
     T* g_z;
+    LatticeTiling lat;          // block schedule over the Lagrangian lattice (or identity)
     uint32_t* next_counts;      // optional: bucket counts/ranks of the drifted positions
     uint32_t* next_keys;
     uint32_t* next_ranks;
@@ -776,7 +819,7 @@ template <typename T, typename PT>
 __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
     // each XCD walks a contiguous particle range (spatially compact for
     // lattice-ordered particles), so potential lines are reused in its L2
-    const uint64_t p = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const uint64_t p = (uint64_t)lattice_block(xcd_block(blockIdx.x, gridDim.x), a.lat) * blockDim.x + threadIdx.x;
     const Geom& g = a.g;
     const int N = g.N;
     uint32_t fail = 0;
@@ -786,36 +829,68 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
     if (live) {
         int ix = 0, iy = 0, iz = 0;
         double fx = 0.0, fy = 0.0, fz = 0.0;
+        // the whole particle state is loaded up front: one memory round trip
+        // for it, then one for the potential stencil
         const double xp = a.x_in[p], yp = a.y_in[p], zp = a.z_in[p];
+        const bool kick = a.has_prev || a.has_next;
+        PT qx0 = 0, qy0 = 0, qz0 = 0;
+        if (kick) {
+            qx0 = a.px_in[p];
+            qy0 = a.py_in[p];
+            qz0 = a.pz_in[p];
+        }
         if (!(cic_axis(xp, g, ix, fx) && cic_axis(yp, g, iy, fy) && cic_axis(zp, g, iz, fz))) {
             fail = ERR_POSITION_NONFINITE;
         } else {
             const T wx[2] = {(T)(1.0 - fx), (T)fx};
             const T wy[2] = {(T)(1.0 - fy), (T)fy};
             const T wz[2] = {(T)(1.0 - fz), (T)fz};
+            // The 8 corners' centred differences touch 32 distinct potential
+            // values: the x stencil X[k][b][c] at (ix-1+k, iy+b, iz+c), and the
+            // y / z stencil ends Y[j][a][c] at (ix+a, iy-1 / iy+2, iz+c) and
+            // Z[j][a][b] at (ix+a, iy+b, iz-1 / iz+2).  All 32 loads are
+            // issued before any arithmetic.
             const T* phi = a.phi;
             const uint64_t NN = (uint64_t)N * N;
-            auto at = [&](int cx, int cy, int cz) -> T {
-                return phi[(uint64_t)cx * NN + (uint64_t)cy * N + cz];
-            };
+            uint64_t xo[4], yo[4], zo[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                xo[k] = (uint64_t)wrap_cell(ix - 1 + k, N) * NN;
+                yo[k] = (uint64_t)wrap_cell(iy - 1 + k, N) * (uint64_t)N;
+                zo[k] = (uint64_t)wrap_cell(iz - 1 + k, N);
+            }
+            T X[4][2][2], Y[2][2][2], Z[2][2][2];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) X[k][b][c] = phi[xo[k] + yo[b + 1] + zo[c + 1]];
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                    for (int v = 0; v < 2; ++v) {
+                        Y[j][u][v] = phi[xo[u + 1] + yo[j * 3] + zo[v + 1]];
+                        Z[j][u][v] = phi[xo[u + 1] + yo[v + 1] + zo[j * 3]];
+                    }
             const T two_dx = (T)(2.0 * g.dx);
             T gx = 0, gy = 0, gz = 0;
 #pragma unroll
             for (int ox = 0; ox < 2; ++ox) {
-                const int cx = wrap_cell(ix + ox, N);
-                const int cxp = wrap_cell(cx + 1, N), cxm = wrap_cell(cx - 1, N);
 #pragma unroll
                 for (int oy = 0; oy < 2; ++oy) {
-                    const int cy = wrap_cell(iy + oy, N);
-                    const int cyp = wrap_cell(cy + 1, N), cym = wrap_cell(cy - 1, N);
 #pragma unroll
                     for (int oz = 0; oz < 2; ++oz) {
-                        const int cz = wrap_cell(iz + oz, N);
-                        const int czp = wrap_cell(cz + 1, N), czm = wrap_cell(cz - 1, N);
                         const T w = wx[ox] * wy[oy] * wz[oz];
-                        const T dxv = at(cxp, cy, cz) - at(cxm, cy, cz);
-                        const T dyv = at(cx, cyp, cz) - at(cx, cym, cz);
-                        const T dzv = at(cx, cy, czp) - at(cx, cy, czm);
+                        const T ypl = oy == 0 ? X[ox + 1][1][oz] : Y[1][ox][oz];
+                        const T ymi = oy == 0 ? Y[0][ox][oz] : X[ox + 1][0][oz];
+                        const T zpl = oz == 0 ? X[ox + 1][oy][1] : Z[1][ox][oy];
+                        const T zmi = oz == 0 ? Z[0][ox][oy] : X[ox + 1][oy][0];
+                        const T dxv = X[ox + 2][oy][oz] - X[ox][oy][oz];
+                        const T dyv = ypl - ymi;
+                        const T dzv = zpl - zmi;
                         gx += w * (-dxv / two_dx);
                         gy += w * (-dyv / two_dx);
                         gz += w * (-dzv / two_dx);
@@ -835,8 +910,8 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
                     a.g_y[p] = gy;
                     a.g_z[p] = gz;
                 }
-                if (a.has_prev || a.has_next) {
-                    PT qx = a.px_in[p], qy = a.py_in[p], qz = a.pz_in[p];
+                if (kick) {
+                    PT qx = qx0, qy = qy0, qz = qz0;
                     if (!isfinite(qx) || !isfinite(qy) || !isfinite(qz)) {
                         fail = ERR_MOMENTUM_NONFINITE;
                     } else {

@@ -355,3 +355,35 @@ def test_fft_tuning_rejects_uncompiled_variant(gpu):
     assert len(s.tuning()["rows"]["z_r2c"]) == 1
     with pytest.raises(HyInvalidArgument):
         s.set_tuning([1, 0, 0, 0, 0])
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_lattice_block_schedule_bit_identical(gpu, prec, monkeypatch):
+    """256^3 particles in lattice order take the Lagrangian-tiled gather
+    schedule; results equal the identity schedule bit for bit."""
+    import hyowon_amd as H
+    S, N, L = 256, 64, 64.0
+    n = S ** 3
+    rng = np.random.default_rng(9)
+    q = (np.arange(S) + 0.5) * (L / S)
+    X, Y, Z = np.meshgrid(q, q, q, indexing="ij")
+    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()]) + rng.normal(0, 0.05, (3, n))
+    pos = np.mod(pos, L)
+    mom = rng.normal(0, 5.0, (3, n))
+
+    def run():
+        s = _solver(L, N, prec, n)
+        a = H.ParticleState.from_numpy(pos, mom, precision=prec)
+        b = a.empty_like()
+        for i in range(2):
+            s.kdk(a, b, 1.0, 0.01 if i else None, 0.01, 0.02, reuse_keys=True)
+            a, b = b, a
+        out = [t.cpu().numpy() for t in (a.x, a.y, a.z, a.px, a.py, a.pz)]
+        s.close()
+        return out
+
+    tiled = run()
+    monkeypatch.setenv("HY_NO_LATTICE_SCHEDULE", "1")
+    plain = run()
+    for u, v in zip(tiled, plain):
+        assert np.array_equal(u, v)

@@ -11,6 +11,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// helpers usable from host code too (CPU checks of the index arithmetic)
+#define HY_HD __host__ __device__ __forceinline__
+
 namespace hy {
 
 // Error bits raised by kernels into a device word; the host maps them to the

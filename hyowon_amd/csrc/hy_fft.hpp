@@ -56,32 +56,32 @@ struct FftPlan {
 // Complex helpers
 // ---------------------------------------------------------------------------
 template <typename C>
-__device__ __forceinline__ C cadd(C a, C b) { return {a.x + b.x, a.y + b.y}; }
+HY_HD C cadd(C a, C b) { return {a.x + b.x, a.y + b.y}; }
 template <typename C>
-__device__ __forceinline__ C csub(C a, C b) { return {a.x - b.x, a.y - b.y}; }
+HY_HD C csub(C a, C b) { return {a.x - b.x, a.y - b.y}; }
 template <typename C>
-__device__ __forceinline__ C cmul(C a, C b) {
+HY_HD C cmul(C a, C b) {
     return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 template <typename C>
-__device__ __forceinline__ C cconj(C a) { return {a.x, -a.y}; }
+HY_HD C cconj(C a) { return {a.x, -a.y}; }
 // multiply by s*i, s = +1 or -1
 template <int S, typename C>
-__device__ __forceinline__ C cmuli(C a) {
+HY_HD C cmuli(C a) {
     if constexpr (S > 0) return {-a.y, a.x};
     else return {a.y, -a.x};
 }
 
 // In-register DFTs of radix 2, 3, 4, 8.  S = -1 forward, +1 backward.
 template <int S, typename C>
-__device__ __forceinline__ void dft2(C* u) {
+HY_HD void dft2(C* u) {
     const C a = u[0], b = u[1];
     u[0] = cadd(a, b);
     u[1] = csub(a, b);
 }
 
 template <int S, typename C>
-__device__ __forceinline__ void dft3(C* u) {
+HY_HD void dft3(C* u) {
     using T = decltype(u[0].x);
     const T h = (T)0.86602540378443864676;  // sqrt(3)/2
     const C t1 = cadd(u[1], u[2]);
@@ -95,7 +95,7 @@ __device__ __forceinline__ void dft3(C* u) {
 }
 
 template <int S, typename C>
-__device__ __forceinline__ void dft4(C* u) {
+HY_HD void dft4(C* u) {
     const C t0 = cadd(u[0], u[2]);
     const C t1 = csub(u[0], u[2]);
     const C t2 = cadd(u[1], u[3]);
@@ -107,7 +107,7 @@ __device__ __forceinline__ void dft4(C* u) {
 }
 
 template <int S, typename C>
-__device__ __forceinline__ void dft8(C* u) {
+HY_HD void dft8(C* u) {
     using T = decltype(u[0].x);
     C e[4] = {u[0], u[2], u[4], u[6]};
     C o[4] = {u[1], u[3], u[5], u[7]};
@@ -129,11 +129,125 @@ __device__ __forceinline__ void dft8(C* u) {
 }
 
 template <int R, int S, typename C>
-__device__ __forceinline__ void dftR(C* u) {
+HY_HD void dftR(C* u) {
     if constexpr (R == 2) dft2<S>(u);
     else if constexpr (R == 3) dft3<S>(u);
     else if constexpr (R == 4) dft4<S>(u);
     else dft8<S>(u);
+}
+
+// ---------------------------------------------------------------------------
+// Register DFTs of length 2..32 (natural order in and out) and the two-level
+// row FFT built on them.
+// ---------------------------------------------------------------------------
+// exp(2 pi i m / 32), m = 0..31 (cos, sin), evaluated in binary64.
+constexpr double kW32[32][2] = {
+    {                     1.0,                      0.0},
+    {      0.9807852804032304,      0.19509032201612825},
+    {      0.9238795325112867,       0.3826834323650898},
+    {      0.8314696123025452,       0.5555702330196022},
+    {      0.7071067811865476,       0.7071067811865475},
+    {      0.5555702330196023,       0.8314696123025452},
+    {     0.38268343236508984,       0.9238795325112867},
+    {     0.19509032201612833,       0.9807852804032304},
+    {   6.123233995736766e-17,                      1.0},
+    {     -0.1950903220161282,       0.9807852804032304},
+    {     -0.3826834323650897,       0.9238795325112867},
+    {      -0.555570233019602,       0.8314696123025455},
+    {     -0.7071067811865475,       0.7071067811865476},
+    {     -0.8314696123025453,       0.5555702330196022},
+    {     -0.9238795325112867,       0.3826834323650899},
+    {     -0.9807852804032304,       0.1950903220161286},
+    {                    -1.0,   1.2246467991473532e-16},
+    {     -0.9807852804032304,     -0.19509032201612836},
+    {     -0.9238795325112868,     -0.38268343236508967},
+    {     -0.8314696123025455,       -0.555570233019602},
+    {     -0.7071067811865477,      -0.7071067811865475},
+    {     -0.5555702330196022,      -0.8314696123025452},
+    {    -0.38268343236509034,      -0.9238795325112865},
+    {    -0.19509032201612866,      -0.9807852804032303},
+    { -1.8369701987210297e-16,                     -1.0},
+    {      0.1950903220161283,      -0.9807852804032304},
+    {        0.38268343236509,      -0.9238795325112866},
+    {      0.5555702330196018,      -0.8314696123025455},
+    {      0.7071067811865474,      -0.7071067811865477},
+    {      0.8314696123025452,      -0.5555702330196022},
+    {      0.9238795325112865,      -0.3826834323650904},
+    {      0.9807852804032303,     -0.19509032201612872},
+};
+
+// exp(S 2 pi i m / R) for R | 32, as a complex constant of type C.
+template <int R, int S, typename C>
+HY_HD C wconst(int m) {
+    using T = decltype(C{}.x);
+    const int q = ((m % R) + R) % R * (32 / R);
+    return C{(T)kW32[q][0], (T)(S * kW32[q][1])};
+}
+
+// In-register DFT of length R (power of two <= 32), natural order in place:
+// R = A * B with a length-A DFT over each stride-B column, the twiddle
+// exp(S 2 pi i n2 k1 / R), then a length-B DFT over each row.
+template <int R, int S, typename C>
+HY_HD void dft_reg(C* u) {
+    if constexpr (R <= 8) {
+        dftR<R, S>(u);
+    } else {
+        constexpr int A = 4, B = R / 4;
+        C y[R];
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) {
+            C col[A];
+#pragma unroll
+            for (int n1 = 0; n1 < A; ++n1) col[n1] = u[B * n1 + n2];
+            dft4<S>(col);
+#pragma unroll
+            for (int k1 = 0; k1 < A; ++k1)
+                y[k1 * B + n2] = (k1 * n2 == 0) ? col[k1] : cmul(col[k1], wconst<R, S, C>(k1 * n2));
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) {
+            C row[B];
+#pragma unroll
+            for (int n2 = 0; n2 < B; ++n2) row[n2] = y[k1 * B + n2];
+            dft_reg<B, S>(row);
+#pragma unroll
+            for (int k2 = 0; k2 < B; ++k2) u[k1 + A * k2] = row[k2];
+        }
+    }
+}
+
+// Two-level row FFT of a power-of-two length LEN = A * B (A, B <= 32):
+//   level 1: thread n2 < B holds the stride-B column x[n2 + B n1] (n1 < A),
+//            transforms it (length A) and multiplies by W_LEN^(n2 k1);
+//   exchange: Y[k1][n2] through LDS (row stride B + 1);
+//   level 2: thread k1 < A holds row Y[k1][*], transforms it (length B) and
+//            then holds X[k1 + A k2], k2 < B.
+// The inverse of a spectrum held as in level 2 is the same procedure with
+// A and B exchanged.
+template <int LEN>
+struct TwoLevel {
+    static constexpr int lg(int v) { return v <= 1 ? 0 : 1 + lg(v / 2); }
+    static constexpr int A = 1 << (lg(LEN) / 2);
+    static constexpr int B = LEN / A;
+    static constexpr int TPR = A > B ? A : B;
+    static constexpr int EMAX = A > B ? A : B;
+    // exchange region per row (complex): A rows of B+1 or B rows of A+1
+    static constexpr int XS0 = (A * (B + 1) > B * (A + 1) ? A * (B + 1) : B * (A + 1));
+    static constexpr int XS = XS0 % 2 == 0 ? XS0 + 1 : XS0;
+    static_assert(A * B == LEN && A <= 32 && B <= 32 && A >= 2, "two-level FFT length");
+};
+
+// Level-1 twiddle: v[k1] *= W_LEN^(S col k1), twiddles from the table
+// tw[m * stride] = exp(-2 pi i m / LEN).
+template <int A, int LEN, int S, typename C>
+HY_HD void tl_twiddle(C* v, int col, const C* tw, int stride) {
+#pragma unroll
+    for (int k1 = 1; k1 < A; ++k1) {
+        const int m = (col * k1) % LEN;
+        C w = tw[m * stride];
+        if constexpr (S > 0) w.y = -w.y;
+        v[k1] = cmul(v[k1], w);
+    }
 }
 
 // LDS slot of logical element j of a row: one padding slot every 32 entries,
@@ -239,6 +353,32 @@ struct RowsCfg {
     static constexpr int ROWS = ROWS0 < dim_pow2 ? ROWS0 : dim_pow2;
     static constexpr int THREADS = ROWS * FftPlan<LEN>::TPR;
     static constexpr int LDS_BYTES = ROWS * RS * (int)sizeof(C);
+    static_assert(LDS_BYTES <= 160 * 1024, "LDS per workgroup");
+};
+
+// Launch geometry of the two-level passes: ROWS rows per workgroup, TPR
+// threads per row; the LDS holds either the per-row exchange (XS complex per
+// row), a natural-order copy of the rows (LEN + 1 per row) or the transposed
+// tile of the y passes (LEN x (ROWS + 1)).
+template <typename T, int LEN, int DIM, int V = 0>
+struct TLCfg {
+    using C = cplx<T>;
+    using TL = TwoLevel<LEN>;
+    static constexpr int TPR = TL::TPR;
+    static constexpr int PER_ROW = (TL::XS > LEN + 1 ? TL::XS : LEN + 1) + LEN / 8 + 1;
+    static constexpr int by_threads = fft_thread_cap(V) / TPR;
+    static constexpr int by_lds = fft_lds_budget(V) / (PER_ROW * (int)sizeof(C));
+    static constexpr int cap = by_threads < by_lds ? by_threads : by_lds;
+    static constexpr int dim_pow2 = (DIM & (-DIM));
+    static constexpr int ROWS0 = pow2_floor(cap < 1 ? 1 : cap);
+    static constexpr int ROWS = ROWS0 < dim_pow2 ? ROWS0 : dim_pow2;
+    static constexpr int THREADS = ROWS * TPR;
+    static constexpr int NAT = LEN + 1;  // natural-order row stride (odd)
+    static constexpr int TS = ROWS + 1;  // transposed tile row stride
+    static constexpr int ELEMS = (ROWS * TL::XS > ROWS * NAT ? ROWS * TL::XS : ROWS * NAT) > LEN * TS
+                                     ? (ROWS * TL::XS > ROWS * NAT ? ROWS * TL::XS : ROWS * NAT)
+                                     : LEN * TS;
+    static constexpr int LDS_BYTES = ELEMS * (int)sizeof(C);
     static_assert(LDS_BYTES <= 160 * 1024, "LDS per workgroup");
 };
 

@@ -236,4 +236,280 @@ __global__ void __launch_bounds__((RowsCfg<T, N / 2, N, V>::THREADS)) k_fft_z_c2
     if (__ballot(mybad) && lane_id() == 0) atomicOr(a.err, ERR_FIELD_NONFINITE);
 }
 
+// ===========================================================================
+// Two-level passes (power-of-two meshes).  Each row transform is a length-A
+// DFT per stride-B column read straight from HBM into registers, one LDS
+// exchange, and a length-B DFT per row written straight back (or through a
+// transposing LDS tile where the pass changes the memory layout).
+// ===========================================================================
+template <typename T>
+__device__ __forceinline__ T green_factor(const FftArgs<T>& a, int kx, int ky, int kz) {
+    if (kx == 0 && ky == 0 && kz == 0) return (T)0;
+    const T D = a.green_lap[kx] + a.green_lap[ky] + a.green_lap[kz];
+    T ff = (T)a.green_num / D;
+    if (a.deconvolve) {
+        const T w = a.green_win[kx] * a.green_win[ky] * a.green_win[kz];
+        ff /= (w * w);
+    }
+    return ff * (T)a.norm;
+}
+
+// z pass: real rows (as LEN = N/2 complex) -> half spectrum, transposed into A.
+template <typename T, int N, int V>
+__global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_r2c(FftArgs<T> a) {
+    using C = cplx<T>;
+    constexpr int LEN = N / 2, NH = N / 2 + 1;
+    using Cfg = TLCfg<T, LEN, N, V>;
+    using TL = TwoLevel<LEN>;
+    constexpr int A = TL::A, B = TL::B, XS = TL::XS, ROWS = Cfg::ROWS, TPR = Cfg::TPR, NT = Cfg::THREADS;
+    constexpr int NAT = Cfg::NAT;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    const int tid = threadIdx.x, r = tid / TPR, t = tid % TPR;
+    C* xr = lds + r * XS;
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const int yb = b % (N / ROWS), x = b / (N / ROWS);
+    const int y0 = yb * ROWS;
+    const C* zrow = reinterpret_cast<const C*>(a.R + ((uint64_t)x * N + y0 + r) * N);
+    C v[TL::EMAX];
+    if (t < B) {
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) v[n1] = zrow[t + B * n1];
+        dft_reg<A, -1>(v);
+        tl_twiddle<A, LEN, -1>(v, t, a.tw, 2);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) xr[k1 * (B + 1) + t] = v[k1];
+    }
+    __syncthreads();
+    if (t < A) {
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) v[n2] = xr[t * (B + 1) + n2];
+    }
+    __syncthreads();
+    if (t < A) {
+        dft_reg<B, -1>(v);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) lds[r * NAT + t + A * k2] = v[k2];
+    }
+    __syncthreads();
+    // untangle: X[k] = E + W^k O, E = (Z[k]+conj Z[L-k])/2, O = -i/2 (Z[k]-conj Z[L-k])
+    C* dst = a.A + (uint64_t)x * NH * N + y0;
+    staged_copy<ROWS * NH, NT, C>(
+        tid,
+        [&](int l) {
+            const int rr = l % ROWS, k = l / ROWS;
+            const C zk = lds[rr * NAT + (k % LEN)];
+            const C zc = cconj(lds[rr * NAT + (LEN - k) % LEN]);
+            const C e = {(T)0.5 * (zk.x + zc.x), (T)0.5 * (zk.y + zc.y)};
+            const C d = csub(zk, zc);
+            const C o = {(T)0.5 * d.y, (T)-0.5 * d.x};
+            return cadd(e, cmul(a.tw[k], o));
+        },
+        [&](int l, C val) { dst[(uint64_t)(l / ROWS) * N + (l % ROWS)] = val; });
+}
+
+// y pass forward: A rows over y -> B[kz][ky][x] through a transposing tile.
+template <typename T, int N, int V>
+__global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_fwd(FftArgs<T> a) {
+    using C = cplx<T>;
+    constexpr int LEN = N, NH = N / 2 + 1;
+    using Cfg = TLCfg<T, LEN, N, V>;
+    using TL = TwoLevel<LEN>;
+    constexpr int A = TL::A, B = TL::B, XS = TL::XS, ROWS = Cfg::ROWS, TPR = Cfg::TPR, NT = Cfg::THREADS;
+    constexpr int TS = Cfg::TS;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    const int tid = threadIdx.x, r = tid / TPR, t = tid % TPR;
+    C* xr = lds + r * XS;
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const int xb = b % (N / ROWS), kz = a.kz0 + (int)(b / (N / ROWS));
+    const int x0 = xb * ROWS;
+    const C* src = a.A + ((uint64_t)(x0 + r) * NH + kz) * N;
+    C v[TL::EMAX];
+    if (t < B) {
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) v[n1] = src[t + B * n1];
+        dft_reg<A, -1>(v);
+        tl_twiddle<A, LEN, -1>(v, t, a.tw, 1);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) xr[k1 * (B + 1) + t] = v[k1];
+    }
+    __syncthreads();
+    if (t < A) {
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) v[n2] = xr[t * (B + 1) + n2];
+    }
+    __syncthreads();
+    if (t < A) {
+        dft_reg<B, -1>(v);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) lds[(t + A * k2) * TS + r] = v[k2];
+    }
+    __syncthreads();
+    C* dst = a.B + (uint64_t)kz * N * N + x0;
+    staged_copy<ROWS * N, NT, C>(
+        tid, [&](int l) { return lds[(l / ROWS) * TS + (l % ROWS)]; },
+        [&](int l, C val) { dst[(uint64_t)(l / ROWS) * N + (l % ROWS)] = val; });
+}
+
+// x pass: forward x FFT, Green multiply (with 1/N^3), backward x FFT, in place on B.
+template <typename T, int N, int V>
+__global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_tl_x_green(FftArgs<T> a) {
+    using C = cplx<T>;
+    constexpr int LEN = N, NH = N / 2 + 1;
+    using Cfg = TLCfg<T, LEN, NH * N, V>;
+    using TL = TwoLevel<LEN>;
+    constexpr int A = TL::A, B = TL::B, XS = TL::XS, ROWS = Cfg::ROWS, TPR = Cfg::TPR;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    const int tid = threadIdx.x, r = tid / TPR, t = tid % TPR;
+    C* xr = lds + r * XS;
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const uint64_t q = (uint64_t)a.kz0 * N + (uint64_t)b * ROWS + r;  // row q = kz*N + ky
+    const int kz = (int)(q / N), ky = (int)(q % N);
+    C* row = a.B + q * N;
+    C v[TL::EMAX];
+    if (t < B) {
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) v[n1] = row[t + B * n1];
+        dft_reg<A, -1>(v);
+        tl_twiddle<A, LEN, -1>(v, t, a.tw, 1);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) xr[k1 * (B + 1) + t] = v[k1];
+    }
+    __syncthreads();
+    if (t < A) {
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) v[n2] = xr[t * (B + 1) + n2];
+    }
+    __syncthreads();
+    if (t < A) {
+        dft_reg<B, -1>(v);  // v[k2] = X[t + A k2]
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) {
+            const T f = green_factor(a, t + A * k2, ky, kz);
+            v[k2].x *= f;
+            v[k2].y *= f;
+        }
+        // backward transform of the held stride-A column (A and B exchanged)
+        dft_reg<B, +1>(v);
+        tl_twiddle<B, LEN, +1>(v, t, a.tw, 1);
+#pragma unroll
+        for (int k1 = 0; k1 < B; ++k1) xr[k1 * (A + 1) + t] = v[k1];
+    }
+    __syncthreads();
+    if (t < B) {
+#pragma unroll
+        for (int n = 0; n < A; ++n) v[n] = xr[t * (A + 1) + n];
+        dft_reg<A, +1>(v);
+#pragma unroll
+        for (int k2 = 0; k2 < A; ++k2) row[t + B * k2] = v[k2];
+    }
+}
+
+// y pass backward: B[kz][ky][x] -> A rows over y through a transposing tile.
+template <typename T, int N, int V>
+__global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_bwd(FftArgs<T> a) {
+    using C = cplx<T>;
+    constexpr int LEN = N, NH = N / 2 + 1;
+    using Cfg = TLCfg<T, LEN, N, V>;
+    using TL = TwoLevel<LEN>;
+    constexpr int A = TL::A, B = TL::B, XS = TL::XS, ROWS = Cfg::ROWS, TPR = Cfg::TPR, NT = Cfg::THREADS;
+    constexpr int TS = Cfg::TS;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    const int tid = threadIdx.x, r = tid / TPR, t = tid % TPR;
+    C* xr = lds + r * XS;
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const int xb = b % (N / ROWS), kz = a.kz0 + (int)(b / (N / ROWS));
+    const int x0 = xb * ROWS;
+    const C* src = a.B + (uint64_t)kz * N * N + x0;
+    staged_copy<ROWS * N, NT, C>(
+        tid, [&](int l) { return src[(uint64_t)(l / ROWS) * N + (l % ROWS)]; },
+        [&](int l, C val) { lds[(l / ROWS) * TS + (l % ROWS)] = val; });
+    __syncthreads();
+    C v[TL::EMAX];
+    if (t < B) {
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) v[n1] = lds[(t + B * n1) * TS + r];
+    }
+    __syncthreads();
+    if (t < B) {
+        dft_reg<A, +1>(v);
+        tl_twiddle<A, LEN, +1>(v, t, a.tw, 1);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) xr[k1 * (B + 1) + t] = v[k1];
+    }
+    __syncthreads();
+    if (t < A) {
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) v[n2] = xr[t * (B + 1) + n2];
+        dft_reg<B, +1>(v);
+        C* dst = a.A + ((uint64_t)(x0 + r) * NH + kz) * N;
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) dst[t + A * k2] = v[k2];
+    }
+}
+
+// z pass backward (c2r): A[x][kz][y] -> R rows; flags a non-finite potential.
+template <typename T, int N, int V>
+__global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_c2r(FftArgs<T> a) {
+    using C = cplx<T>;
+    constexpr int LEN = N / 2, NH = N / 2 + 1;
+    using Cfg = TLCfg<T, LEN, N, V>;
+    using TL = TwoLevel<LEN>;
+    constexpr int A = TL::A, B = TL::B, XS = TL::XS, ROWS = Cfg::ROWS, TPR = Cfg::TPR, NT = Cfg::THREADS;
+    constexpr int NAT = Cfg::NAT;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    const int tid = threadIdx.x, r = tid / TPR, t = tid % TPR;
+    C* xr = lds + r * XS;
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const int yb = b % (N / ROWS), x = b / (N / ROWS);
+    const int y0 = yb * ROWS;
+    const C* src = a.A + (uint64_t)x * NH * N + y0;
+    staged_copy<ROWS * NH, NT, C>(
+        tid, [&](int l) { return src[(uint64_t)(l / ROWS) * N + (l % ROWS)]; },
+        [&](int l, C val) {
+            const int rr = l % ROWS, k = l / ROWS;
+            if (k == 0 || k == LEN) val.y = (T)0;  // c2r reads only the independent half
+            lds[rr * NAT + k] = val;
+        });
+    __syncthreads();
+    C v[TL::EMAX];
+    if (t < B) {
+        // Z'[k] = (X[k] + conj X[L-k]) + i (X[k] - conj X[L-k]) W^-k
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) {
+            const int k = t + B * n1;
+            const C xk = lds[r * NAT + k];
+            const C xm = lds[r * NAT + LEN - k];
+            const C s1 = cadd(xk, cconj(xm));
+            const C d1 = cmul(csub(xk, cconj(xm)), cconj(a.tw[k]));
+            v[n1] = C{s1.x - d1.y, s1.y + d1.x};
+        }
+    }
+    __syncthreads();
+    if (t < B) {
+        dft_reg<A, +1>(v);
+        tl_twiddle<A, LEN, +1>(v, t, a.tw, 2);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) xr[k1 * (B + 1) + t] = v[k1];
+    }
+    __syncthreads();
+    int mybad = 0;
+    if (t < A) {
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) v[n2] = xr[t * (B + 1) + n2];
+        dft_reg<B, +1>(v);
+        C* dst = reinterpret_cast<C*>(a.R + ((uint64_t)x * N + y0 + r) * N);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) {
+            if (!isfinite(v[k2].x) || !isfinite(v[k2].y)) mybad = 1;
+            dst[t + A * k2] = v[k2];
+        }
+    }
+    if (__ballot(mybad) && lane_id() == 0) atomicOr(a.err, ERR_FIELD_NONFINITE);
+}
+
 }  // namespace hy

@@ -32,17 +32,32 @@ constexpr int kFftTunedFrom = 192;
 bool lookup_fft(int N, FftKernels<float>& out);
 bool lookup_fft(int N, FftKernels<double>& out);
 
+// Power-of-two meshes from 128 up use the two-level register passes; the
+// others (3*2^a, small meshes) use the LDS Stockham passes.
+constexpr bool fft_two_level(int N) { return N >= 128 && (N & (N - 1)) == 0; }
+
 #ifdef HY_FFT_INSTANTIATE
 template <typename T, int N, int V>
 void fill_variant(FftKernels<T>& k) {
-    using RZ = RowsCfg<T, N / 2, N, V>;
-    using RY = RowsCfg<T, N, N, V>;
-    using RX = RowsCfg<T, N, (N / 2 + 1) * N, V>;
-    k.pass[FFT_Z_R2C][V] = {k_fft_z_r2c<T, N, V>, RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES};
-    k.pass[FFT_Y_FWD][V] = {k_fft_y_fwd<T, N, V>, RY::ROWS, RY::THREADS, RY::LDS_BYTES};
-    k.pass[FFT_X_GREEN][V] = {k_fft_x_green<T, N, V>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
-    k.pass[FFT_Y_BWD][V] = {k_fft_y_bwd<T, N, V>, RY::ROWS, RY::THREADS, RY::LDS_BYTES};
-    k.pass[FFT_Z_C2R][V] = {k_fft_z_c2r<T, N, V>, RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES};
+    if constexpr (fft_two_level(N)) {
+        using RZ = TLCfg<T, N / 2, N, V>;
+        using RY = TLCfg<T, N, N, V>;
+        using RX = TLCfg<T, N, (N / 2 + 1) * N, V>;
+        k.pass[FFT_Z_R2C][V] = {k_tl_z_r2c<T, N, V>, RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES};
+        k.pass[FFT_Y_FWD][V] = {k_tl_y_fwd<T, N, V>, RY::ROWS, RY::THREADS, RY::LDS_BYTES};
+        k.pass[FFT_X_GREEN][V] = {k_tl_x_green<T, N, V>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
+        k.pass[FFT_Y_BWD][V] = {k_tl_y_bwd<T, N, V>, RY::ROWS, RY::THREADS, RY::LDS_BYTES};
+        k.pass[FFT_Z_C2R][V] = {k_tl_z_c2r<T, N, V>, RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES};
+    } else {
+        using RZ = RowsCfg<T, N / 2, N, V>;
+        using RY = RowsCfg<T, N, N, V>;
+        using RX = RowsCfg<T, N, (N / 2 + 1) * N, V>;
+        k.pass[FFT_Z_R2C][V] = {k_fft_z_r2c<T, N, V>, RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES};
+        k.pass[FFT_Y_FWD][V] = {k_fft_y_fwd<T, N, V>, RY::ROWS, RY::THREADS, RY::LDS_BYTES};
+        k.pass[FFT_X_GREEN][V] = {k_fft_x_green<T, N, V>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
+        k.pass[FFT_Y_BWD][V] = {k_fft_y_bwd<T, N, V>, RY::ROWS, RY::THREADS, RY::LDS_BYTES};
+        k.pass[FFT_Z_C2R][V] = {k_fft_z_c2r<T, N, V>, RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES};
+    }
 }
 
 template <typename T, int N>

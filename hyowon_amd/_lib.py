@@ -22,7 +22,8 @@ EXPORTS = [
     "hy_pm_bucket_order", "hy_pm_xplane_order", "hy_pm_check", "hy_hubble", "hy_drift_factor",
     "hy_kick_factor", "hy_geometric_mean", "hy_time_boundaries", "hy_exact_sum",
     "hy_gravitational_constant", "hy_rho_crit0", "hy_pm_profile", "hy_pm_profile_read",
-    "hy_pm_get_tuning", "hy_pm_set_tuning", "hy_pm_autotune",
+    "hy_pm_get_tuning", "hy_pm_set_tuning", "hy_pm_autotune", "hy_ensemble_create", "hy_ensemble_step",
+    "hy_ensemble_check", "hy_ensemble_destroy",
 ]
 
 HY_FFT_PASSES = 5
@@ -74,13 +75,13 @@ _BY_CODE = {1: HyInvalidArgument, 2: HyLogicError, 3: HyOverflowError, 4: HyUnde
 class hy_pm_desc(C.Structure):
     _fields_ = [("box_size", C.c_double), ("mesh_size", C.c_uint64), ("deconvolve_cic", C.c_int),
                 ("precision", C.c_int), ("device", C.c_int), ("stream", C.c_void_p),
-                ("max_particles", C.c_uint64)]
+                ("max_particles", C.c_uint64), ("boxes", C.c_uint32)]
 
 
 class hy_pm_info(C.Structure):
     _fields_ = [("mesh_size", C.c_uint64), ("tile_edge", C.c_uint64), ("bucket_count", C.c_uint64),
                 ("device_bytes", C.c_uint64), ("last_input_mass", C.c_double),
-                ("last_deposited_mass", C.c_double), ("precision", C.c_int)]
+                ("last_deposited_mass", C.c_double), ("precision", C.c_int), ("boxes", C.c_uint32)]
 
 
 class hy_fft_tuning(C.Structure):
@@ -89,6 +90,12 @@ class hy_fft_tuning(C.Structure):
                 ("rows", (C.c_uint32 * HY_FFT_MAX_VARIANTS) * HY_FFT_PASSES),
                 ("pass_ms", (C.c_float * HY_FFT_MAX_VARIANTS) * HY_FFT_PASSES),
                 ("group_ms", C.c_float * 4)]
+
+
+class hy_ensemble_member(C.Structure):
+    _fields_ = [("ctx", C.c_void_p), ("n", C.c_uint64), ("x", C.c_void_p), ("y", C.c_void_p),
+                ("z", C.c_void_p), ("px", C.c_void_p), ("py", C.c_void_p), ("pz", C.c_void_p),
+                ("uniform_mass", C.c_double)]
 
 
 _lib = None
@@ -137,6 +144,10 @@ def lib():
     fn("hy_pm_get_tuning", i, vp, C.POINTER(hy_fft_tuning))
     fn("hy_pm_set_tuning", i, vp, C.POINTER(hy_fft_tuning))
     fn("hy_pm_autotune", i, vp, u32, C.POINTER(hy_fft_tuning))
+    fn("hy_ensemble_create", i, C.POINTER(hy_ensemble_member), u32, C.POINTER(vp))
+    fn("hy_ensemble_step", i, vp, d, d, d, u32)
+    fn("hy_ensemble_check", i, vp)
+    fn("hy_ensemble_destroy", i, vp)
     _lib = L
     return L
 

@@ -135,6 +135,10 @@ struct hy_pm_ctx {
     hy::FftKernels<double> fft64{};
     int fft_var[hy::FFT_PASSES] = {0, 0, 0, 0, 0};  // launch variant per pass
     int max_lds = 0;                                 // device LDS bytes per workgroup
+    uint32_t nbox = 1;                               // boxes of this geometry (box-major arrays)
+    uint64_t nball = 0;                              // deposit buckets over all boxes
+    const double* dfac = nullptr;                    // device step factors (graph capture)
+    double* hmass = nullptr;                         // pinned per-box deposited masses
 
     // pinned host mirror of the status words
     struct Status {
@@ -165,6 +169,7 @@ struct hy_pm_ctx {
         for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
         for (void* p : allocations) (void)hipFree(p);
         if (hstatus) (void)hipHostFree(hstatus);
+        if (hmass) (void)hipHostFree(hmass);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -174,6 +179,27 @@ namespace {
 void set_device(const hy_pm_ctx* c) { hip_check(hipSetDevice(c->desc.device), "hipSetDevice"); }
 
 void launch_check(const char* what) { hip_check(hipGetLastError(), what); }
+
+// Clears `words` 32-bit words with a kernel (no memset nodes: callers may
+// capture these paths into a HIP graph).
+void fill_u32(hy_pm_ctx* c, void* p, uint64_t words) {
+    const uint64_t blocks = std::min<uint64_t>(std::max<uint64_t>(blocks_for(words, 256), 1), 4096);
+    hipLaunchKernelGGL(hy::k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, c->stream, static_cast<uint32_t*>(p),
+                       words, 0u);
+    launch_check("k_fill_u32");
+}
+
+// Particles per box of a call (particle arrays concatenate the boxes).
+uint64_t set_boxes(hy_pm_ctx* c, uint64_t n) {
+    if (n % c->nbox != 0) fail(HY_EINVAL, "particle count must be a multiple of the context's box count");
+    const uint64_t nper = n / c->nbox;
+    c->geom.nper = (uint32_t)nper;
+    return nper;
+}
+
+void require_single_box(const hy_pm_ctx* c, const char* what) {
+    if (c->nbox != 1) fail(HY_EINVAL, std::string(what) + " takes a single-box context");
+}
 
 // Record stage boundary `mark` (0 = call start, k = end of stage k-1) of the
 // current profiled call.  No-op when profiling is off or the ring is full.
@@ -235,7 +261,7 @@ void bucketize(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const
                uint32_t* counts, uint32_t* offsets, uint64_t nb, uint32_t* order, bool carried = false,
                const double* masses = nullptr) {
     if (!carried) {
-        hip_check(hipMemsetAsync(counts, 0, nb * sizeof(uint32_t), c->stream), "memset counts");
+        fill_u32(c, counts, nb);
         hipLaunchKernelGGL(hy::k_bucket_key, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, x, y, z,
                            c->geom, mode, c->keys, c->ranks, counts, c->err);
         launch_check("k_bucket_key");
@@ -265,8 +291,8 @@ void bucketize(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const
 }
 
 void reset_status(hy_pm_ctx* c) {
-    hip_check(hipMemsetAsync(c->err, 0, sizeof(uint32_t), c->stream), "memset err");
-    hip_check(hipMemsetAsync(c->scratch_cursor, 0, sizeof(unsigned long long), c->stream), "memset cursor");
+    fill_u32(c, c->err, 1);
+    fill_u32(c, c->scratch_cursor, 2);
 }
 
 // Copy err / grand total / deposited mass to the host and synchronise.
@@ -327,18 +353,18 @@ DepositPlan plan_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double
 
 void launch_deposit(hy_pm_ctx* c, const hy::DepParams& P) {
     if (c->prec == HY_PREC_FP64)
-        hipLaunchKernelGGL(hy::k_deposit<double>, dim3(c->ntiles), dim3(hy::DEP_THREADS), 0, c->stream, P,
+        hipLaunchKernelGGL(hy::k_deposit<double>, dim3(c->ntiles * c->nbox), dim3(hy::DEP_THREADS), 0, c->stream, P,
                            static_cast<double*>(c->R));
     else {
         hy::DepFxParams Q{P, c->mass_unit, std::ldexp(c->mass_unit, -32), c->rec,
                           P.masses ? c->rec_mass : nullptr};
-        hipLaunchKernelGGL(hy::k_deposit_fixed, dim3(c->ntiles), dim3(hy::DEPFX_THREADS), 0, c->stream, Q,
+        hipLaunchKernelGGL(hy::k_deposit_fixed, dim3(c->ntiles * c->nbox), dim3(hy::DEPFX_THREADS), 0, c->stream, Q,
                            static_cast<float*>(c->R));
     }
     launch_check("k_deposit");
-    hipLaunchKernelGGL(hy::k_reduce_partials, dim3(hy::REDUCE_BLOCKS), dim3(256), 0, c->stream, c->tile_mass,
-                       c->ntiles, c->mass_partials);
-    hipLaunchKernelGGL(hy::k_reduce_sum, dim3(1), dim3(256), 0, c->stream, c->mass_partials,
+    hipLaunchKernelGGL(hy::k_reduce_partials, dim3(hy::REDUCE_BLOCKS, c->nbox), dim3(256), 0, c->stream,
+                       c->tile_mass, c->ntiles, c->mass_partials);
+    hipLaunchKernelGGL(hy::k_reduce_sum, dim3(c->nbox), dim3(256), 0, c->stream, c->mass_partials,
                        (uint64_t)hy::REDUCE_BLOCKS, c->mass_total);
     launch_check("k_reduce_sum");
 }
@@ -355,6 +381,8 @@ hy::FftArgs<T> fft_args(hy_pm_ctx* c) {
     a.green_num = c->green_num;
     a.norm = 1.0 / (static_cast<double>(c->N) * static_cast<double>(c->N) * static_cast<double>(c->N));
     a.deconvolve = c->desc.deconvolve_cic ? 1 : 0;
+    a.box_real = (uint64_t)c->N * c->N * c->N;
+    a.box_cplx = (uint64_t)c->N * (c->N / 2 + 1) * c->N;
     a.err = c->err;
     return a;
 }
@@ -372,7 +400,7 @@ void launch_pass(hy_pm_ctx* c, const hy::FftKernels<T>& K, hy::FftArgs<T>& a, in
     else if (p == hy::FFT_X_GREEN) blocks = (uint64_t)g * N / P.rows;
     else blocks = (uint64_t)g * (N / P.rows);
     a.kz0 = kz0;
-    hipLaunchKernelGGL(P.fn, dim3((uint32_t)blocks), dim3(P.threads), P.lds, c->stream, a);
+    hipLaunchKernelGGL(P.fn, dim3((uint32_t)blocks, c->nbox), dim3(P.threads), P.lds, c->stream, a);
     launch_check(names[p]);
 }
 
@@ -430,7 +458,8 @@ hy::LatticeTiling lattice_tiling(uint64_t n, uint32_t block) {
 
 template <typename T, typename PT>
 void launch_gather(hy_pm_ctx* c, hy::GatherArgs<T, PT> a) {
-    a.lat = lattice_tiling(a.n, 256);
+    a.lat = c->nbox == 1 ? lattice_tiling(a.n, 256) : hy::LatticeTiling{0, 0, 0, 0};
+    a.dfac = c->dfac;
     hipLaunchKernelGGL((hy::k_gather_kdk<T, PT>), dim3(blocks_for(a.n, 256)), dim3(256), 0, c->stream, a);
     launch_check("k_gather_kdk");
 }
@@ -458,7 +487,7 @@ void run_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, con
     mark(c, 0);
     // the binary32 deposit reads records only: no particle order is written
     uint32_t* order = c->prec == HY_PREC_FP64 ? c->order : nullptr;
-    bucketize(c, n, x, y, z, 0, c->counts, c->offsets, c->nb, order, carried, masses);
+    bucketize(c, n, x, y, z, 0, c->counts, c->offsets, c->nball, order, carried, masses);
     mark(c, HY_STAGE_BUCKET + 1);
     auto plan = plan_deposit(c, n, x, y, z, masses, uniform, Min);
     launch_deposit(c, plan.P);
@@ -473,7 +502,7 @@ void run_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, con
             raise_errors(st.err & (hy::ERR_POSITION_NONFINITE | hy::ERR_MASS_INVALID));
             carried = false;
             reset_status(c);
-            bucketize(c, n, x, y, z, 0, c->counts, c->offsets, c->nb, order, false, masses);
+            bucketize(c, n, x, y, z, 0, c->counts, c->offsets, c->nball, order, false, masses);
             plan = plan_deposit(c, n, x, y, z, masses, uniform, Min);
             launch_deposit(c, plan.P);
             continue;
@@ -497,7 +526,14 @@ void run_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, con
         raise_errors(st.err);
         if (st.total != n) fail(HY_ELOGIC, "CIC bucket routing does not cover each particle exactly once");
         c->last_deposited_mass = st.mass;
-        check_mass(c, Min, st.mass, n);
+        if (c->nbox == 1) {
+            check_mass(c, Min, st.mass, n);
+        } else {
+            hip_check(hipMemcpyAsync(c->hmass, c->mass_total, c->nbox * sizeof(double), hipMemcpyDeviceToHost,
+                                     c->stream), "mass copy");
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+            for (uint32_t b = 0; b < c->nbox; ++b) check_mass(c, Min, c->hmass[b], n / c->nbox);
+        }
         return;
     }
 }
@@ -527,9 +563,9 @@ void fill_tuning(const hy_pm_ctx* c, const hy::FftKernels<T>& K, hy_fft_tuning* 
 template <typename T>
 void autotune(hy_pm_ctx* c, const hy::FftKernels<T>& K, uint32_t reps, hy_fft_tuning* t) {
     const uint64_t N = (uint64_t)c->N, NH = N / 2 + 1;
-    hip_check(hipMemsetAsync(c->R, 0, N * N * N * sizeof(T), c->stream), "memset R");
-    hip_check(hipMemsetAsync(c->A, 0, N * NH * N * 2 * sizeof(T), c->stream), "memset A");
-    hip_check(hipMemsetAsync(c->B, 0, N * NH * N * 2 * sizeof(T), c->stream), "memset B");
+    hip_check(hipMemsetAsync(c->R, 0, c->nbox * N * N * N * sizeof(T), c->stream), "memset R");
+    hip_check(hipMemsetAsync(c->A, 0, c->nbox * N * NH * N * 2 * sizeof(T), c->stream), "memset A");
+    hip_check(hipMemsetAsync(c->B, 0, c->nbox * N * NH * N * 2 * sizeof(T), c->stream), "memset B");
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "hipEventCreate");
     hip_check(hipEventCreate(&e1), "hipEventCreate");
@@ -619,6 +655,10 @@ int hy_pm_create(const hy_pm_desc* desc, hy_pm_ctx** out) {
             c->geom.inv_dx = 1.0 / dx;
         }
         c->geom.upper = std::nextafter((double)N, 0.0);
+        c->nbox = desc->boxes == 0 ? 1u : desc->boxes;
+        if (c->nbox > 65535) fail(HY_EINVAL, "box count exceeds 65535");
+        c->geom.nbox = c->nbox;
+        c->geom.nper = 0;
         c->geom.N = N;
         c->geom.T = tile_edge_for(N);
         c->geom.nT = N / c->geom.T;
@@ -644,9 +684,9 @@ int hy_pm_create(const hy_pm_desc* desc, hy_pm_ctx** out) {
         const uint64_t NH = N / 2 + 1;
         const uint64_t ccount = (uint64_t)N * NH * N;
         const size_t rb = c->real_bytes;
-        c->R = c->alloc(cells * rb);
-        c->A = c->alloc(ccount * 2 * rb);
-        c->B = c->alloc(ccount * 2 * rb);
+        c->R = c->alloc(c->nbox * cells * rb);
+        c->A = c->alloc(c->nbox * ccount * 2 * rb);
+        c->B = c->alloc(c->nbox * ccount * 2 * rb);
         c->tw = c->alloc((size_t)N * 2 * rb);
         c->lap = c->alloc((size_t)N * rb);
         c->win = c->alloc((size_t)N * rb);
@@ -693,16 +733,18 @@ int hy_pm_create(const hy_pm_desc* desc, hy_pm_ctx** out) {
         c->order = (uint32_t*)c->alloc(cap * 4);
         c->ntiles = (uint64_t)c->geom.nT * c->geom.nT * c->geom.nT;
         c->nb = c->ntiles * 8;
-        const uint64_t nb_max = std::max<uint64_t>(c->nb, (uint64_t)N);
+        c->nball = c->nb * c->nbox;
+        if (c->nball > 0xffffffffull) fail(HY_EINVAL, "bucket count over all boxes exceeds the 32-bit key range");
+        const uint64_t nb_max = std::max<uint64_t>(c->nball, (uint64_t)N);
         c->counts = (uint32_t*)c->alloc(nb_max * 4);
         c->offsets = (uint32_t*)c->alloc((nb_max + 1) * 4);
         c->block_sums = (uint32_t*)c->alloc((blocks_for(nb_max, hy::SCAN_TILE) + 1) * 4);
         c->grand_total = (uint32_t*)c->alloc(16);
         c->xcounts = (uint32_t*)c->alloc((size_t)N * 4);
         c->xoffsets = (uint32_t*)c->alloc(((size_t)N + 1) * 4);
-        c->tile_mass = (double*)c->alloc(c->ntiles * 8);
-        c->mass_total = (double*)c->alloc(16);
-        c->mass_partials = (double*)c->alloc(hy::REDUCE_BLOCKS * sizeof(double));
+        c->tile_mass = (double*)c->alloc(c->ntiles * c->nbox * 8);
+        c->mass_total = (double*)c->alloc(std::max<uint64_t>(2, c->nbox) * 8);
+        c->mass_partials = (double*)c->alloc((uint64_t)hy::REDUCE_BLOCKS * c->nbox * sizeof(double));
         c->scratch_cap = 2 * cap + (1u << 16);
         c->scratch = (uint64_t*)c->alloc(c->scratch_cap * 8);
         c->scratch_cursor = (unsigned long long*)c->alloc(16);
@@ -713,6 +755,7 @@ int hy_pm_create(const hy_pm_desc* desc, hy_pm_ctx** out) {
         c->gy = c->alloc(cap * rb);
         c->gz = c->alloc(cap * rb);
         hip_check(hipHostMalloc((void**)&c->hstatus, sizeof(hy_pm_ctx::Status), hipHostMallocDefault), "hipHostMalloc");
+        hip_check(hipHostMalloc((void**)&c->hmass, c->nbox * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
         hip_check(hipMemset(c->err, 0, 16), "memset");
         hip_check(hipMemset(c->mass_total, 0, 16), "memset");
         hip_check(hipMemset(c->grand_total, 0, 16), "memset");
@@ -740,6 +783,7 @@ int hy_pm_get_info(const hy_pm_ctx* c, hy_pm_info* info) {
         info->last_input_mass = c->last_input_mass;
         info->last_deposited_mass = c->last_deposited_mass;
         info->precision = c->prec;
+        info->boxes = c->nbox;
     });
 }
 
@@ -762,9 +806,11 @@ int hy_pm_density(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, co
                   const double* masses, double uniform, void* mesh_out) {
     return guarded([&] {
         if (!c || !x || !y || !z || !mesh_out) fail(HY_EINVAL, "hy_pm_density: null argument");
+        require_single_box(c, "hy_pm_density");
         set_device(c);
         c->keys_ready = false;
         require_particles(c, n);
+        set_boxes(c, n);
         const double M = input_mass(c, n, masses, uniform, false);
         reset_status(c);
         if (masses) {
@@ -780,6 +826,7 @@ int hy_pm_density(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, co
 int hy_pm_potential(hy_pm_ctx* c, const void* source_in, void* phi_out) {
     return guarded([&] {
         if (!c || !source_in || !phi_out) fail(HY_EINVAL, "hy_pm_potential: null argument");
+        require_single_box(c, "hy_pm_potential");
         set_device(c);
         const size_t bytes = (size_t)c->N * c->N * c->N * c->real_bytes;
         reset_status(c);
@@ -795,8 +842,10 @@ int hy_pm_gather(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, con
                  const void* phi, double* ax, double* ay, double* az) {
     return guarded([&] {
         if (!c || !x || !y || !z || !phi || !ax || !ay || !az) fail(HY_EINVAL, "hy_pm_gather: null argument");
+        require_single_box(c, "hy_pm_gather");
         set_device(c);
         require_particles(c, n);
+        set_boxes(c, n);
         reset_status(c);
         auto fill = [&](auto& a) {
             a.g = c->geom; a.n = n;
@@ -845,7 +894,9 @@ int hy_pm_force(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, cons
                     if (ins[j] && overlap(outs[i], ins[j], n)) fail(HY_EINVAL, "PMSolver force spans overlap");
             }
         }
-        const double M = input_mass(c, n, masses, uniform, host);
+        const uint64_t nper = set_boxes(c, n);
+        if (masses && c->nbox > 1) fail(HY_EINVAL, "explicit masses take a single-box context");
+        const double M = input_mass(c, nper, masses, uniform, host);
         const double *dx_ = x, *dy_ = y, *dz_ = z, *dm = masses;
         if (host) {
             if (!c->hpos) {
@@ -931,9 +982,11 @@ int hy_pm_kdk(hy_pm_ctx* c, uint64_t n, const double* x_in, const double* y_in, 
         set_device(c);
         require_particles(c, n);
         const bool sync = (flags & HY_ASYNC) == 0;
-        const double M = input_mass(c, n, masses, uniform, false);
+        const uint64_t nper = set_boxes(c, n);
+        if (masses && c->nbox > 1) fail(HY_EINVAL, "explicit masses take a single-box context");
+        const double M = input_mass(c, nper, masses, uniform, false);
         if (sync) reset_status(c);
-        else hip_check(hipMemsetAsync(c->scratch_cursor, 0, sizeof(unsigned long long), c->stream), "memset");
+        else fill_u32(c, c->scratch_cursor, 2);
         if (masses) hipLaunchKernelGGL(hy::k_check_mass, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, masses, c->err);
         // keys of the previous fused step are valid for exactly its output positions
         const bool carried = (flags & HY_REUSE_KEYS) && c->keys_ready && c->keys_n == n && c->keys_x == x_in &&
@@ -943,7 +996,7 @@ int hy_pm_kdk(hy_pm_ctx* c, uint64_t n, const double* x_in, const double* y_in, 
         // counts are consumed by the scan above; the gather below accumulates the
         // next step's counts into the zeroed buffer
         const bool make_keys = has_next;
-        if (make_keys) hip_check(hipMemsetAsync(c->counts, 0, c->nb * sizeof(uint32_t), c->stream), "memset counts");
+        if (make_keys) fill_u32(c, c->counts, c->nball);
         run_fft(c);
         auto fill = [&](auto& a) {
             a.g = c->geom; a.n = n;
@@ -996,9 +1049,11 @@ int hy_pm_bucket_order(hy_pm_ctx* c, uint64_t n, const double* x, const double* 
                        uint32_t* order, uint32_t* offsets) {
     return guarded([&] {
         if (!c || !x || !y || !z || !order || !offsets) fail(HY_EINVAL, "hy_pm_bucket_order: null argument");
+        require_single_box(c, "hy_pm_bucket_order");
         set_device(c);
         c->keys_ready = false;
         require_particles(c, n);
+        set_boxes(c, n);
         reset_status(c);
         bucketize(c, n, x, y, z, 0, c->counts, offsets, c->nb, order);
         hipLaunchKernelGGL(hy::k_canonicalize, dim3(std::min<uint64_t>(c->nb, 65535)), dim3(hy::CANON_THREADS), 0,
@@ -1013,9 +1068,11 @@ int hy_pm_bucket_order(hy_pm_ctx* c, uint64_t n, const double* x, const double* 
 int hy_pm_xplane_order(hy_pm_ctx* c, uint64_t n, const double* x, uint32_t* order, uint32_t* offsets) {
     return guarded([&] {
         if (!c || !x || !order || !offsets) fail(HY_EINVAL, "hy_pm_xplane_order: null argument");
+        require_single_box(c, "hy_pm_xplane_order");
         set_device(c);
         c->keys_ready = false;
         require_particles(c, n);
+        set_boxes(c, n);
         reset_status(c);
         bucketize(c, n, x, x, x, 1, c->xcounts, offsets, (uint64_t)c->N, order);
         hipLaunchKernelGGL(hy::k_canonicalize, dim3(c->N), dim3(hy::CANON_THREADS), 0, c->stream,
@@ -1105,6 +1162,121 @@ int hy_pm_autotune(hy_pm_ctx* c, uint32_t reps, hy_fft_tuning* out) {
         if (c->prec == HY_PREC_FP64) autotune(c, c->fft64, reps, &t);
         else autotune(c, c->fft32, reps, &t);
         if (out) *out = t;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble step graphs
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct hy_ensemble {
+    std::vector<hy_ensemble_member> members;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    double* dfac = nullptr;  // k_prev, k_next, d_next for the next replay
+    hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
+    ~hy_ensemble() {
+        for (hipGraphExec_t g : exec)
+            if (g) (void)hipGraphExecDestroy(g);
+        if (dfac) (void)hipFree(dfac);
+    }
+};
+
+extern "C" {
+
+int hy_ensemble_create(const hy_ensemble_member* members, uint32_t count, hy_ensemble** out) {
+    return guarded([&] {
+        if (!members || !out || count == 0) fail(HY_EINVAL, "hy_ensemble_create: null argument or no members");
+        *out = nullptr;
+        auto e = std::make_unique<hy_ensemble>();
+        for (uint32_t i = 0; i < count; ++i) {
+            const hy_ensemble_member& m = members[i];
+            if (!m.ctx || !m.x || !m.y || !m.z || !m.px || !m.py || !m.pz || m.n == 0)
+                fail(HY_EINVAL, "hy_ensemble_create: incomplete member");
+            if (i == 0) {
+                e->stream = m.ctx->stream;
+                e->device = m.ctx->desc.device;
+            } else if (m.ctx->stream != e->stream || m.ctx->desc.device != e->device) {
+                fail(HY_EINVAL, "ensemble members must share one stream and device");
+            }
+            e->members.push_back(m);
+        }
+        hip_check(hipSetDevice(e->device), "hipSetDevice");
+        hip_check(hipMalloc(&e->dfac, 4 * sizeof(double)), "hipMalloc");
+        hip_check(hipMemset(e->dfac, 0, 4 * sizeof(double)), "hipMemset");
+        *out = e.release();
+    });
+}
+
+int hy_ensemble_step(hy_ensemble* e, double k_prev, double k_next, double d_next, uint32_t flags) {
+    return guarded([&] {
+        if (!e) fail(HY_EINVAL, "hy_ensemble_step: null ensemble");
+        const bool has_prev = (flags & HY_HAS_PREV) != 0, has_next = (flags & HY_HAS_NEXT) != 0;
+        if (!has_prev && !has_next) fail(HY_EINVAL, "hy_ensemble_step needs HY_HAS_PREV and/or HY_HAS_NEXT");
+        if ((has_prev && !std::isfinite(k_prev)) || (has_next && (!std::isfinite(k_next) || !std::isfinite(d_next))))
+            fail(HY_EOVERFLOW, "Leapfrog drift/kick factors must be finite");
+        hip_check(hipSetDevice(e->device), "hipSetDevice");
+        hipLaunchKernelGGL(hy::k_set_factors, dim3(1), dim3(64), 0, e->stream, e->dfac, k_prev, k_next, d_next);
+        launch_check("k_set_factors");
+        const int idx = (has_prev ? 1 : 0) | (has_next ? 2 : 0);
+        if (!e->exec[idx]) {
+            const uint32_t kflags = (has_prev ? HY_HAS_PREV : 0u) | (has_next ? HY_HAS_NEXT : 0u) | HY_ASYNC |
+                                    HY_REUSE_KEYS;
+            std::vector<bool> prof(e->members.size());
+            for (size_t i = 0; i < e->members.size(); ++i) {
+                prof[i] = e->members[i].ctx->prof;
+                e->members[i].ctx->prof = false;
+                e->members[i].ctx->dfac = e->dfac;
+            }
+            hip_check(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            int rc = HY_OK;
+            std::string msg;
+            for (const hy_ensemble_member& m : e->members) {
+                rc = hy_pm_kdk(m.ctx, m.n, m.x, m.y, m.z, m.px, m.py, m.pz, m.x, m.y, m.z, m.px, m.py, m.pz,
+                               nullptr, m.uniform_mass, 0.0, 0.0, 0.0, nullptr, nullptr, nullptr, kflags);
+                if (rc != HY_OK) {
+                    msg = g_last_error;
+                    break;
+                }
+            }
+            hipGraph_t graph = nullptr;
+            const hipError_t ec = hipStreamEndCapture(e->stream, &graph);
+            for (size_t i = 0; i < e->members.size(); ++i) {
+                e->members[i].ctx->prof = prof[i];
+                e->members[i].ctx->dfac = nullptr;
+            }
+            if (rc != HY_OK) {
+                if (graph) (void)hipGraphDestroy(graph);
+                fail(rc, "ensemble capture: " + msg);
+            }
+            hip_check(ec, "hipStreamEndCapture");
+            const hipError_t ei = hipGraphInstantiate(&e->exec[idx], graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            hip_check(ei, "hipGraphInstantiate");
+        }
+        hip_check(hipGraphLaunch(e->exec[idx], e->stream), "hipGraphLaunch");
+    });
+}
+
+int hy_ensemble_check(hy_ensemble* e) {
+    if (!e) {
+        g_last_error = "hy_ensemble_check: null ensemble";
+        return HY_EINVAL;
+    }
+    for (const hy_ensemble_member& m : e->members) {
+        const int rc = hy_pm_check(m.ctx);
+        if (rc != HY_OK) return rc;
+    }
+    return HY_OK;
+}
+
+int hy_ensemble_destroy(hy_ensemble* e) {
+    return guarded([&] {
+        if (!e) return;
+        (void)hipSetDevice(e->device);
+        if (e->stream) (void)hipStreamSynchronize(e->stream);
+        delete e;
     });
 }
 

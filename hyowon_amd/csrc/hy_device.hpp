@@ -39,7 +39,19 @@ struct Geom {
     int N;          // mesh cells per axis
     int T;          // deposit tile edge (divides N, <= 16)
     int nT;         // N / T
+    uint32_t nbox;  // boxes of this geometry held by the context (>= 1)
+    uint32_t nper;  // particles per box in the current call (arrays are box-major)
 };
+
+// Box of global particle p (particle arrays concatenate the boxes).
+__device__ __forceinline__ uint32_t box_of(uint64_t p, const Geom& g) {
+    return g.nbox > 1 ? (uint32_t)p / g.nper : 0u;
+}
+
+// Deposit buckets per box: (N/T)^3 tiles x 8 boundary classes.
+__device__ __forceinline__ uint32_t buckets_per_box(const Geom& g) {
+    return (uint32_t)g.nT * (uint32_t)g.nT * (uint32_t)g.nT * 8u;
+}
 
 __device__ __forceinline__ double wrap_pos(double x, double L) {
     if (!isfinite(x)) return __builtin_nan("");

@@ -26,8 +26,19 @@ struct FftArgs {
     double norm;         // 1 / N^3
     int deconvolve;
     int kz0;             // first kz plane of this launch (y/x passes run over plane groups)
+    uint64_t box_real;   // elements per box of R (N^3) and of A, B (N^2 (N/2+1))
+    uint64_t box_cplx;
     uint32_t* err;
 };
+
+// Boxes of one geometry are laid out box-major; grid row blockIdx.y is the box.
+template <typename T>
+__device__ __forceinline__ void fft_box(FftArgs<T>& a) {
+    const uint64_t b = blockIdx.y;
+    a.R += b * a.box_real;
+    a.A += b * a.box_cplx;
+    a.B += b * a.box_cplx;
+}
 
 // Staged copies with compile-time trip counts: every thread issues all of its
 // loads before its first dependent store, so each wave keeps PER requests in
@@ -51,6 +62,7 @@ __device__ __forceinline__ void staged_copy(int tid, Load load, Store store) {
 // z pass: real rows -> half spectrum, written transposed into A.
 template <typename T, int N, int V>
 __global__ void __launch_bounds__((RowsCfg<T, N / 2, N, V>::THREADS)) k_fft_z_r2c(FftArgs<T> a) {
+    fft_box(a);
     using C = cplx<T>;
     constexpr int LEN = N / 2;
     constexpr int NH = N / 2 + 1;
@@ -87,6 +99,7 @@ __global__ void __launch_bounds__((RowsCfg<T, N / 2, N, V>::THREADS)) k_fft_z_r2
 // y pass forward: A rows over y -> B[kz][ky][x]
 template <typename T, int N, int V>
 __global__ void __launch_bounds__((RowsCfg<T, N, N, V>::THREADS)) k_fft_y_fwd(FftArgs<T> a) {
+    fft_box(a);
     using C = cplx<T>;
     constexpr int NH = N / 2 + 1;
     using RC = RowsCfg<T, N, N, V>;
@@ -111,6 +124,7 @@ __global__ void __launch_bounds__((RowsCfg<T, N, N, V>::THREADS)) k_fft_y_fwd(Ff
 // x pass: forward x FFT, Green multiply (with 1/N^3), backward x FFT, in place on B.
 template <typename T, int N, int V>
 __global__ void __launch_bounds__((RowsCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_fft_x_green(FftArgs<T> a) {
+    fft_box(a);
     using C = cplx<T>;
     constexpr int NH = N / 2 + 1;
     using RC = RowsCfg<T, N, NH * N, V>;
@@ -159,6 +173,7 @@ __global__ void __launch_bounds__((RowsCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) 
 // y pass backward: B[kz][ky][x] -> A rows over y
 template <typename T, int N, int V>
 __global__ void __launch_bounds__((RowsCfg<T, N, N, V>::THREADS)) k_fft_y_bwd(FftArgs<T> a) {
+    fft_box(a);
     using C = cplx<T>;
     constexpr int NH = N / 2 + 1;
     using RC = RowsCfg<T, N, N, V>;
@@ -183,6 +198,7 @@ __global__ void __launch_bounds__((RowsCfg<T, N, N, V>::THREADS)) k_fft_y_bwd(Ff
 // z pass backward (c2r): A[x][kz][y] -> R rows; flags a non-finite potential.
 template <typename T, int N, int V>
 __global__ void __launch_bounds__((RowsCfg<T, N / 2, N, V>::THREADS)) k_fft_z_c2r(FftArgs<T> a) {
+    fft_box(a);
     using C = cplx<T>;
     constexpr int LEN = N / 2;
     constexpr int NH = N / 2 + 1;
@@ -257,6 +273,7 @@ __device__ __forceinline__ T green_factor(const FftArgs<T>& a, int kx, int ky, i
 // z pass: real rows (as LEN = N/2 complex) -> half spectrum, transposed into A.
 template <typename T, int N, int V>
 __global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_r2c(FftArgs<T> a) {
+    fft_box(a);
     using C = cplx<T>;
     constexpr int LEN = N / 2, NH = N / 2 + 1;
     using Cfg = TLCfg<T, LEN, N, V>;
@@ -311,6 +328,7 @@ __global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_r2c(F
 // y pass forward: A rows over y -> B[kz][ky][x] through a transposing tile.
 template <typename T, int N, int V>
 __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_fwd(FftArgs<T> a) {
+    fft_box(a);
     using C = cplx<T>;
     constexpr int LEN = N, NH = N / 2 + 1;
     using Cfg = TLCfg<T, LEN, N, V>;
@@ -355,6 +373,7 @@ __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_fwd(FftAr
 // x pass: forward x FFT, Green multiply (with 1/N^3), backward x FFT, in place on B.
 template <typename T, int N, int V>
 __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_tl_x_green(FftArgs<T> a) {
+    fft_box(a);
     using C = cplx<T>;
     constexpr int LEN = N, NH = N / 2 + 1;
     using Cfg = TLCfg<T, LEN, NH * N, V>;
@@ -410,6 +429,7 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
 // y pass backward: B[kz][ky][x] -> A rows over y through a transposing tile.
 template <typename T, int N, int V>
 __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_bwd(FftArgs<T> a) {
+    fft_box(a);
     using C = cplx<T>;
     constexpr int LEN = N, NH = N / 2 + 1;
     using Cfg = TLCfg<T, LEN, N, V>;
@@ -454,6 +474,7 @@ __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_bwd(FftAr
 // z pass backward (c2r): A[x][kz][y] -> R rows; flags a non-finite potential.
 template <typename T, int N, int V>
 __global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_c2r(FftArgs<T> a) {
+    fft_box(a);
     using C = cplx<T>;
     constexpr int LEN = N / 2, NH = N / 2 + 1;
     using Cfg = TLCfg<T, LEN, N, V>;

@@ -75,7 +75,7 @@ __global__ void k_bucket_key(uint64_t n, const double* __restrict__ x, const dou
         double f;
         ok = cic_axis(x[p], g, bx, f);
         if (mode == 0) ok = ok && cic_axis(y[p], g, by, f) && cic_axis(z[p], g, bz, f);
-        if (ok) key = mode == 0 ? bucket_of(bx, by, bz, g) : (uint32_t)bx;
+        if (ok) key = mode == 0 ? box_of(p, g) * buckets_per_box(g) + bucket_of(bx, by, bz, g) : (uint32_t)bx;
         else atomicOr(err, ERR_POSITION_NONFINITE);
     }
     const uint32_t rank = aggregated_rank(counts, key, in && ok);
@@ -184,7 +184,7 @@ __global__ void k_scatter_checked(uint64_t n, const double* __restrict__ x, cons
         atomicOr(err, ERR_POSITION_NONFINITE);
         return;
     }
-    if (k == 0xffffffffu || bucket_of(bx, by, bz, g) != k) {
+    if (k == 0xffffffffu || box_of(p, g) * buckets_per_box(g) + bucket_of(bx, by, bz, g) != k) {
         atomicOr(err, ERR_STALE_KEYS);
         return;
     }
@@ -217,7 +217,7 @@ __global__ void k_scatter_records(uint64_t n, const double* __restrict__ x, cons
         atomicOr(err, ERR_POSITION_NONFINITE);
         return;
     }
-    if (k == 0xffffffffu || bucket_of(bx, by, bz, g) != k) {
+    if (k == 0xffffffffu || box_of(p, g) * buckets_per_box(g) + bucket_of(bx, by, bz, g) != k) {
         atomicOr(err, ERR_STALE_KEYS);
         return;
     }
@@ -533,14 +533,17 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restr
     const Geom& g = P.g;
     const int nT = g.nT;
     const uint32_t ntiles = (uint32_t)nT * nT * nT;
-    const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
-    if (tile >= ntiles) return;
+    const uint32_t gtile = xcd_block(blockIdx.x, gridDim.x);  // box-major tile index
+    if (gtile >= ntiles * g.nbox) return;
+    const uint32_t box = gtile / ntiles, tile = gtile % ntiles;
     const int tz = tile % nT, ty = (tile / nT) % nT, tx = tile / (nT * nT);
     const int Te = g.T + 1;
     const int ncell = Te * Te * Te;
+    mesh += (uint64_t)box * g.N * g.N * g.N;
 
     __shared__ uint8_t sh_rcls[DEP_NRANGES];
-    load_source_ranges(P.offsets, tx, ty, tz, nT, sh_rs, sh_rc, sh_rd, sh_rcls);
+    load_source_ranges(P.offsets + (uint64_t)box * buckets_per_box(g), tx, ty, tz, nT, sh_rs, sh_rc, sh_rd,
+                       sh_rcls);
     const uint32_t M = sh_rc[DEP_NRANGES];
     const bool use_lds = M <= (uint32_t)CAP;
 
@@ -609,7 +612,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restr
         if ((int)threadIdx.x < s) sh_red[threadIdx.x] += sh_red[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) P.tile_mass[tile] = sh_red[0];
+    if (threadIdx.x == 0) P.tile_mass[gtile] = sh_red[0];
 }
 
 template __global__ void k_deposit<float>(DepParams, float*);
@@ -649,13 +652,16 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
     const int nT = g.nT;
     const int T = g.T;
     const uint32_t ntiles = (uint32_t)nT * nT * nT;
-    const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
-    if (tile >= ntiles) return;
+    const uint32_t gtile = xcd_block(blockIdx.x, gridDim.x);  // box-major tile index
+    if (gtile >= ntiles * g.nbox) return;
+    const uint32_t box = gtile / ntiles, tile = gtile % ntiles;
     const int tz = tile % nT, ty = (tile / nT) % nT, tx = tile / (nT * nT);
     const int cells = T * T * T;
+    mesh += (uint64_t)box * g.N * g.N * g.N;
 
     for (int c = threadIdx.x; c < cells; c += blockDim.x) acc[c] = 0ull;
-    load_source_ranges(P.offsets, tx, ty, tz, nT, sh_rs, sh_rc, sh_rd, sh_rsrc);
+    load_source_ranges(P.offsets + (uint64_t)box * buckets_per_box(g), tx, ty, tz, nT, sh_rs, sh_rc, sh_rd,
+                       sh_rsrc);
     const uint32_t M = sh_rc[DEP_NRANGES];
     int r = 0;
     bool bad = false;
@@ -717,7 +723,23 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
     if (threadIdx.x == 0) {
         double s = 0.0;
         for (int w = 0; w < (int)(blockDim.x / 64); ++w) s += sh_red[w];
-        P.tile_mass[tile] = s * Q.fixed_to_mass;
+        P.tile_mass[gtile] = s * Q.fixed_to_mass;
+    }
+}
+
+// Zero fill (used instead of memset nodes in code that callers may capture
+// into a HIP graph).
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t value) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
+// Step factors written on the stream ahead of a graph replay.
+__global__ void k_set_factors(double* __restrict__ f, double k_prev, double k_next, double d_next) {
+    if (threadIdx.x == 0) {
+        f[0] = k_prev;
+        f[1] = k_next;
+        f[2] = d_next;
     }
 }
 
@@ -727,6 +749,8 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
 constexpr int REDUCE_BLOCKS = 256;
 __global__ void k_reduce_partials(const double* __restrict__ v, uint64_t n, double* __restrict__ partials) {
     __shared__ double sh[256];
+    v += (uint64_t)blockIdx.y * n;  // one box per grid row
+    partials += (uint64_t)blockIdx.y * REDUCE_BLOCKS;
     const uint64_t chunk = (n + REDUCE_BLOCKS - 1) / REDUCE_BLOCKS;
     const uint64_t lo = (uint64_t)blockIdx.x * chunk;
     const uint64_t hi = lo + chunk < n ? lo + chunk : n;
@@ -741,9 +765,11 @@ __global__ void k_reduce_partials(const double* __restrict__ v, uint64_t n, doub
     if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
 }
 
-// Fixed-order reduction (single workgroup).
+// Fixed-order reduction (one workgroup per box).
 __global__ void k_reduce_sum(const double* __restrict__ v, uint64_t n, double* __restrict__ out) {
     __shared__ double sh[256];
+    v += (uint64_t)blockIdx.x * n;
+    out += blockIdx.x;
     double s = 0.0;
     for (uint64_t i = threadIdx.x; i < n; i += 256) s += v[i];
     sh[threadIdx.x] = s;
@@ -807,6 +833,8 @@ struct GatherArgs {
     uint32_t* next_counts;      // optional: bucket counts/ranks of the drifted positions
     uint32_t* next_keys;
     uint32_t* next_ranks;
+    const double* dfac;         // optional: k_prev, k_next, d_next read from device memory
+                                // (graph replays take new factors without re-capture)
     double k_prev;              // K2 of the previous step
     double k_next;              // K1 of the next step
     double d_next;              // drift factor of the next step
@@ -850,8 +878,8 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
             // y / z stencil ends Y[j][a][c] at (ix+a, iy-1 / iy+2, iz+c) and
             // Z[j][a][b] at (ix+a, iy+b, iz-1 / iz+2).  All 32 loads are
             // issued before any arithmetic.
-            const T* phi = a.phi;
             const uint64_t NN = (uint64_t)N * N;
+            const T* phi = a.phi + (uint64_t)box_of(p, g) * NN * N;
             uint64_t xo[4], yo[4], zo[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
@@ -916,18 +944,18 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
                         fail = ERR_MOMENTUM_NONFINITE;
                     } else {
                         if (a.has_prev) {
-                            const PT k = (PT)a.k_prev;
+                            const PT k = (PT)(a.dfac ? a.dfac[0] : a.k_prev);
                             qx = qx + (PT)gx * k;
                             qy = qy + (PT)gy * k;
                             qz = qz + (PT)gz * k;
                         }
                         double nx = xp, ny = yp, nz = zp;
                         if (a.has_next) {
-                            const PT k = (PT)a.k_next;
+                            const PT k = (PT)(a.dfac ? a.dfac[1] : a.k_next);
                             qx = qx + (PT)gx * k;
                             qy = qy + (PT)gy * k;
                             qz = qz + (PT)gz * k;
-                            const double D = a.d_next;
+                            const double D = a.dfac ? a.dfac[2] : a.d_next;
                             const double L = g.L;
                             const double ddx = (double)qx * D, ddy = (double)qy * D, ddz = (double)qz * D;
                             nx = wrap_pos(wrap_pos(xp, L) + minimum_image(ddx, L), L);
@@ -950,7 +978,7 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
                                 double f;
                                 key_ok = cic_axis(nx, g, bx, f) && cic_axis(ny, g, by, f)
                                          && cic_axis(nz, g, bz, f);
-                                if (key_ok) next_key = bucket_of(bx, by, bz, g);
+                                if (key_ok) next_key = box_of(p, g) * buckets_per_box(g) + bucket_of(bx, by, bz, g);
                             }
                         }
                     }

@@ -148,7 +148,7 @@ class PMSolver:
 
     def __init__(self, box_size: float, mesh_size: int, deconvolve_cic: bool = False,
                  precision: str = "fp32", max_particles: int = 0, device: int = 0,
-                 stream=None):
+                 stream=None, boxes: int = 1):
         self.box_size = float(box_size)
         self.mesh_size = int(mesh_size)
         self.precision = PRECISIONS[precision]
@@ -164,11 +164,13 @@ class PMSolver:
                 pass
         d = hy_pm_desc(box_size=self.box_size, mesh_size=self.mesh_size,
                        deconvolve_cic=int(bool(deconvolve_cic)), precision=self.precision,
-                       device=device, stream=stream, max_particles=int(max_particles))
+                       device=device, stream=stream, max_particles=int(max_particles),
+                       boxes=int(boxes))
         h = C.c_void_p()
         check(lib().hy_pm_create(C.byref(d), C.byref(h)))
         self._h = h
         self.device = device
+        self.boxes = int(boxes)
 
     # -- lifetime
     def close(self) -> None:

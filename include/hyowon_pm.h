@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define HY_ABI_VERSION 1
+#define HY_ABI_VERSION 2
 
 typedef enum {
     HY_OK = 0,
@@ -67,7 +67,11 @@ typedef struct {
     int device;             /* HIP device ordinal */
     void* stream;           /* hipStream_t to use, or NULL for a private blocking stream
                                (ordered with the legacy default stream) */
-    uint64_t max_particles; /* capacity; particle counts above it are rejected */
+    uint64_t max_particles; /* capacity (all boxes); particle counts above it are rejected */
+    uint32_t boxes;         /* independent boxes of this geometry held by the context (0 or
+                               1 = one box).  Particle arrays of hy_pm_force / hy_pm_kdk then
+                               concatenate the boxes (n / boxes particles each, uniform mass);
+                               every box has its own mesh and its own mass identity. */
 } hy_pm_desc;
 
 typedef struct {
@@ -78,6 +82,7 @@ typedef struct {
     double last_input_mass;    /* exact input mass of the last deposit */
     double last_deposited_mass;/* deposited mass of the last deposit */
     int precision;
+    uint32_t boxes;            /* boxes held by the context */
 } hy_pm_info;
 
 typedef struct hy_pm_ctx hy_pm_ctx;
@@ -194,6 +199,31 @@ typedef struct {
 int hy_pm_get_tuning(hy_pm_ctx* ctx, hy_fft_tuning* out);
 int hy_pm_set_tuning(hy_pm_ctx* ctx, const hy_fft_tuning* in);
 int hy_pm_autotune(hy_pm_ctx* ctx, uint32_t reps, hy_fft_tuning* out);
+
+/* Ensemble step in one HIP graph launch.  No reference counterpart (the
+ * reference advances one box per process): several contexts, usually one per
+ * box geometry with many boxes each, that share one stream are advanced
+ * together.  Each member's state is updated in place (x, p).  The first
+ * hy_ensemble_step for a given (HY_HAS_PREV, HY_HAS_NEXT) combination records
+ * the members' fused steps into a HIP graph, with the kick/drift factors read
+ * from device memory, instantiates it and launches it; later steps only write
+ * the three factors and relaunch the graph.  Run
+ * one eager hy_pm_kdk per member with HY_HAS_NEXT before the first graph step
+ * so that the recorded step reuses the bucket keys the previous step made.
+ * Calls do not synchronise; hy_ensemble_check reports kernel-raised errors of
+ * any member. */
+typedef struct hy_ensemble hy_ensemble;
+typedef struct {
+    hy_pm_ctx* ctx;
+    uint64_t n;                 /* particles over all boxes of the context */
+    double *x, *y, *z;          /* device positions (updated in place) */
+    void *px, *py, *pz;         /* device momenta in the context precision (in place) */
+    double uniform_mass;
+} hy_ensemble_member;
+int hy_ensemble_create(const hy_ensemble_member* members, uint32_t count, hy_ensemble** out);
+int hy_ensemble_step(hy_ensemble* e, double k_prev, double k_next, double d_next, uint32_t flags);
+int hy_ensemble_check(hy_ensemble* e);
+int hy_ensemble_destroy(hy_ensemble* e);
 
 /* Host scalars of the step (binary64, long-double quadrature).
  * H(a): src/cosmology/cosmology_model.cpp:25-57,167-178.

@@ -276,6 +276,77 @@ def accuracy(args, pos, mom, m, K1, K2, D, nsteps):
             "run": f"{nsteps} KDK steps z={args.z_start:g}->0 in each precision"}
 
 
+def run_ensemble(args):
+    """Config-3 workload: 192 boxes (64 x 32^3 particles on 64^3, 64 x 64^3 on
+    128^3, 64 x 96^3 on 192^3; box side twice the mesh), seeds 1..192,
+    z = 49 -> 0 in 32 KDK steps, one graph launch per ensemble step.  Times the
+    steady graph steps; also times the same steps as eager per-context calls,
+    and reports the FP32-vs-FP64 final-position difference per class."""
+    import torch
+    import hyowon_amd as H
+    from hyowon_amd.ensemble import CONFIG3, EnsembleRun
+    steps = 32
+    a0 = 1.0 / (1.0 + args.z_start)
+    res = {}
+    finals = {}
+    for prec in ("fp32", "fp64"):
+        run = EnsembleRun(CONFIG3, a0, 1.0, steps, precision=prec)
+        run.first_step()
+        run.ensemble.check()
+        torch.cuda.synchronize()
+        timed = min(args.steps, steps - 2)
+        run.next_step()  # first replay records the graph
+        run.ensemble.check()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(timed):
+            run.next_step()
+        run.ensemble.check()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        graph_ms = (t1 - t0) * 1e3 / timed
+        # the same kind of step as eager per-context launches (no graph)
+        i = run.done
+        t0 = time.perf_counter()
+        for s, st, m in zip(run.solvers, run.states, run.masses):
+            s.kdk(st, st, m, run.K2[i - 1], run.K1[i], run.D[i], asynchronous=True, reuse_keys=True)
+        for s in run.solvers:
+            s.check()
+        torch.cuda.synchronize()
+        eager_ms = (time.perf_counter() - t0) * 1e3
+        run.done += 1
+        while run.done <= steps:
+            run.next_step()
+        run.ensemble.check()
+        finals[prec] = [torch.stack([st.x, st.y, st.z]) for st in run.states]
+        res[prec] = {"graph_step_ms": graph_ms, "eager_step_ms": eager_ms}
+        run.close()
+        torch.cuda.empty_cache()
+    n = sum(c.particles for c in CONFIG3)
+    alg = sum(c.boxes * (96.0 * c.n_side ** 3 + 24.0 * c.mesh ** 3) for c in CONFIG3)
+    pos_err = {}
+    for cls, a, b in zip(CONFIG3, finals["fp32"], finals["fp64"]):
+        d = a - b
+        d = d - cls.box_size * torch.round(d / cls.box_size)
+        dist = torch.sqrt((d ** 2).sum(0)) / (cls.box_size / cls.mesh)
+        pos_err[f"{cls.n_side}^3/{cls.mesh}^3"] = {"rms_cells": float(torch.sqrt((dist ** 2).mean())),
+                                                 "max_cells": float(dist.max())}
+    r = res["fp32"]
+    return {
+        "metric": "particle-updates/s per ensemble KDK step (192 boxes, config 3)",
+        "value": n / (r["graph_step_ms"] * 1e-3), "unit": "particle-updates/s", "n_gpus": 1,
+        "steps": min(args.steps, steps - 2), "ms_per_step": r["graph_step_ms"], "higher_is_better": True,
+        "dtype": "f32 mesh/FFT/forces/momenta, f64 positions", "data": "synthetic 1LPT ICs, seeds 1..192",
+        "config": {"workload": "ensemble KDK step: 64 x (32^3, 64^3 mesh, L=128), 64 x (64^3, 128^3, L=256), "
+                               "64 x (96^3, 192^3, L=384), 32 steps z=49->0, one HIP graph launch per step",
+                   "particles": n},
+        "step_hbm_frac": alg / (r["graph_step_ms"] * 1e-3) / HBM_PEAK,
+        "eager_ms_per_step": r["eager_step_ms"],
+        "fp64": {"ms_per_step": res["fp64"]["graph_step_ms"], "eager_ms_per_step": res["fp64"]["eager_step_ms"]},
+        "final_pos_err_fp32_vs_fp64": pos_err,
+    }
+
+
 def cpu_baseline(args):
     """The oracle (binary64 C, OpenMP over all host cores) on a bounded sample of
     the same workload: one force refresh + K1/drift + K2 of a 128^3-particle,
@@ -326,6 +397,9 @@ def main():
                     help="FP32-vs-FP64 force, final-position and P(k) errors (full run)")
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
                     help="measure the FFT launch plan before timing (untimed)")
+    ap.add_argument("--workload", default="config2", choices=["config2", "ensemble"],
+                    help="config2: the 512^3 / 1024^3 KDK step (default); ensemble: the 192-box "
+                         "graph-launched ensemble step")
     ap.add_argument("--fft-plan", default="",
                     help="pin the FFT launch plan 'v0,v1,v2,v3,v4[:plane_group]' instead of autotuning")
     ap.add_argument("--rocfft", action=argparse.BooleanOptionalAction, default=True,
@@ -341,6 +415,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "ensemble":
+        print(json.dumps(run_ensemble(args)), flush=True)
+        return
     n, N, res = run_gpu(args)
     main_prec = args.precisions[0]
     r = res[main_prec]

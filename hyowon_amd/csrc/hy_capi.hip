@@ -139,6 +139,9 @@ struct hy_pm_ctx {
     uint64_t nball = 0;                              // deposit buckets over all boxes
     const double* dfac = nullptr;                    // device step factors (graph capture)
     double* hmass = nullptr;                         // pinned per-box deposited masses
+    void* adj_phi = nullptr;                         // adjoint workspace (allocated on first use)
+    void* adj_a[3] = {nullptr, nullptr, nullptr};
+    double* adj_max = nullptr;
 
     // pinned host mirror of the status words
     struct Status {
@@ -1162,6 +1165,158 @@ int hy_pm_autotune(hy_pm_ctx* c, uint32_t reps, hy_fft_tuning* out) {
         if (c->prec == HY_PREC_FP64) autotune(c, c->fft64, reps, &t);
         else autotune(c, c->fft32, reps, &t);
         if (out) *out = t;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of the fused step and of the density
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+void ensure_adjoint_workspace(hy_pm_ctx* c) {
+    if (c->adj_phi) return;
+    const size_t bytes = (size_t)c->N * c->N * c->N * c->real_bytes;
+    c->adj_phi = c->alloc(bytes);
+    for (auto& a : c->adj_a) a = c->alloc(bytes);
+    c->adj_max = (double*)c->alloc(1024 * sizeof(double));
+}
+
+// max |q| over n values (synchronises).
+double abs_max(hy_pm_ctx* c, uint64_t n, const double* q) {
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(blocks_for(n, 256), 1), 1024);
+    hipLaunchKernelGGL(hy::k_absmax, dim3(blocks), dim3(256), 0, c->stream, n, q, c->adj_max);
+    launch_check("k_absmax");
+    std::vector<double> h(blocks);
+    hip_check(hipMemcpyAsync(h.data(), c->adj_max, blocks * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+              "d2h");
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    double m = 0.0;
+    for (double v : h) m = std::max(m, v);
+    return m;
+}
+
+// Raw CIC deposit of signed per-particle values q at positions whose buckets
+// (and, for binary64, particle order) the last run_deposit left in place.
+void deposit_values(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z,
+                    const double* q, double uniform, double Min) {
+    const double unit = abs_max(c, n, q);
+    if (unit == 0.0 || !std::isfinite(unit)) {
+        if (!std::isfinite(unit)) fail(HY_EOVERFLOW, "adjoint values are not finite");
+        fill_u32(c, c->R, (uint64_t)c->N * c->N * c->N * (c->real_bytes / 4));
+        return;
+    }
+    auto plan = plan_deposit(c, n, x, y, z, q, uniform, Min);
+    plan.P.raw = 1;
+    c->mass_unit = unit;
+    if (c->prec == HY_PREC_FP32_MIXED) {
+        hipLaunchKernelGGL(hy::k_scatter_values, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, q,
+                           1.0 / unit, c->keys, c->ranks, c->offsets, c->rec_mass);
+        launch_check("k_scatter_values");
+    }
+    launch_deposit(c, plan.P);
+}
+
+}  // namespace
+
+extern "C" {
+
+int hy_pm_kdk_vjp(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z,
+                  double uniform, double* lx, double* ly, double* lz, double* lpx, double* lpy, double* lpz,
+                  double k_prev, double k_next, double d_next, uint32_t flags) {
+    return guarded([&] {
+        if (!c || !x || !y || !z || !lx || !ly || !lz || !lpx || !lpy || !lpz)
+            fail(HY_EINVAL, "hy_pm_kdk_vjp: null argument");
+        require_single_box(c, "hy_pm_kdk_vjp");
+        if (n == 0) fail(HY_ERUNTIME, "PMSolver exact input mass identity must be finite and positive");
+        const bool has_prev = (flags & HY_HAS_PREV) != 0, has_next = (flags & HY_HAS_NEXT) != 0;
+        set_device(c);
+        require_particles(c, n);
+        set_boxes(c, n);
+        c->keys_ready = false;
+        const double M = input_mass(c, n, nullptr, uniform, false);
+        ensure_adjoint_workspace(c);
+        reset_status(c);
+        // momentum adjoint entering the kick: u = lp + d_next lx
+        hipLaunchKernelGGL(hy::k_adj_kick, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, lx, ly, lz, lpx,
+                           lpy, lpz, d_next, has_next ? 1 : 0);
+        launch_check("k_adj_kick");
+        const double ksum = (has_prev ? k_prev : 0.0) + (has_next ? k_next : 0.0);
+        if (ksum == 0.0) {
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+            return;
+        }
+        // forward potential at x
+        run_deposit(c, n, x, y, z, nullptr, uniform, M, true);
+        run_fft(c);
+        const size_t bytes = (size_t)c->N * c->N * c->N * c->real_bytes;
+        hip_check(hipMemcpyAsync(c->adj_phi, c->R, bytes, hipMemcpyDeviceToDevice, c->stream), "copy phi");
+        // A_a = deposit of u_a, then div A into R, then chi = G div A
+        const double* u[3] = {lpx, lpy, lpz};
+        for (int d = 0; d < 3; ++d) {
+            deposit_values(c, n, x, y, z, u[d], uniform, M);
+            hip_check(hipMemcpyAsync(c->adj_a[d], c->R, bytes, hipMemcpyDeviceToDevice, c->stream), "copy A");
+        }
+        const uint64_t cells = (uint64_t)c->N * c->N * c->N;
+        const double inv_2dx = 1.0 / (2.0 * c->geom.dx);
+        if (c->prec == HY_PREC_FP64)
+            hipLaunchKernelGGL(hy::k_div_stencil<double>, dim3(blocks_for(cells, 256)), dim3(256), 0, c->stream,
+                               (const double*)c->adj_a[0], (const double*)c->adj_a[1], (const double*)c->adj_a[2],
+                               (double*)c->R, c->N, inv_2dx);
+        else
+            hipLaunchKernelGGL(hy::k_div_stencil<float>, dim3(blocks_for(cells, 256)), dim3(256), 0, c->stream,
+                               (const float*)c->adj_a[0], (const float*)c->adj_a[1], (const float*)c->adj_a[2],
+                               (float*)c->R, c->N, inv_2dx);
+        launch_check("k_div_stencil");
+        run_fft(c);
+        const double dx3 = (c->geom.dx * c->geom.dx) * c->geom.dx;
+        auto gather = [&](auto tag) {
+            using T = decltype(tag);
+            hy::AdjGatherArgs<T> a{};
+            a.g = c->geom; a.n = n; a.x = x; a.y = y; a.z = z;
+            a.ux = lpx; a.uy = lpy; a.uz = lpz;
+            a.phi = static_cast<const T*>(c->adj_phi);
+            a.chi = static_cast<const T*>(c->R);
+            a.coef_a = ksum;
+            a.coef_b = ksum * uniform / dx3;
+            a.lx = lx; a.ly = ly; a.lz = lz;
+            a.err = c->err;
+            hipLaunchKernelGGL(hy::k_adj_gather<T>, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, a);
+            launch_check("k_adj_gather");
+        };
+        if (c->prec == HY_PREC_FP64) gather(double{});
+        else gather(float{});
+        const auto st = fetch_status(c);
+        raise_errors(st.err);
+    });
+}
+
+int hy_pm_density_vjp(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z,
+                      double uniform, const void* u_mesh, double coef, double* lx, double* ly, double* lz) {
+    return guarded([&] {
+        if (!c || !x || !y || !z || !u_mesh || !lx || !ly || !lz) fail(HY_EINVAL, "hy_pm_density_vjp: null argument");
+        require_single_box(c, "hy_pm_density_vjp");
+        set_device(c);
+        require_particles(c, n);
+        set_boxes(c, n);
+        if (!std::isfinite(uniform) || uniform <= 0.0) fail(HY_EINVAL, "PMSolver uniform mass must be finite and positive");
+        reset_status(c);
+        auto gather = [&](auto tag) {
+            using T = decltype(tag);
+            hy::AdjGatherArgs<T> a{};
+            a.g = c->geom; a.n = n; a.x = x; a.y = y; a.z = z;
+            a.chi = static_cast<const T*>(u_mesh);
+            a.coef_b = coef * uniform;
+            a.lx = lx; a.ly = ly; a.lz = lz;
+            a.err = c->err;
+            hipLaunchKernelGGL(hy::k_adj_gather<T>, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, a);
+            launch_check("k_adj_gather");
+        };
+        if (c->prec == HY_PREC_FP64) gather(double{});
+        else gather(float{});
+        const auto st = fetch_status(c);
+        raise_errors(st.err);
     });
 }
 

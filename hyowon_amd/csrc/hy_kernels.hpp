@@ -347,6 +347,7 @@ struct DepParams {
     uint64_t norm_mant;       // 1/(N_cells dx^3) as a 64-bit mantissa ...
     int norm_exp;             // ... times 2^norm_exp (the reference holds it in long double)
     int direct;               // 1: (m - mean) * inv_volume ; 0: fma-centred path
+    int raw;                  // 1: write the deposited sum itself (adjoint deposits of signed values)
     double* tile_mass;        // per-tile deposited mass (binary64)
     uint64_t* scratch;        // global fallback keys
     uint64_t scratch_cap;
@@ -389,6 +390,7 @@ __device__ __forceinline__ double scaled_product_ld(double c, uint64_t s_mant, i
 template <typename T>
 __device__ __forceinline__ T contrast_of(T sum, const DepParams& P) {
     const double m = (double)sum;
+    if (P.raw) return (T)m;
     double v;
     if (P.direct) {
         v = (m - P.mean_mass) * P.inv_volume;
@@ -698,7 +700,8 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
                     const int cz = lz + oz;
                     if (cz < 0 || cz >= T) continue;
                     const float w = mxy * wz[oz];
-                    const unsigned long long q = __float2ull_rn(w * 4294967296.0f);
+                    // signed 32.32 fixed point (two's complement sums wrap consistently)
+                    const unsigned long long q = (unsigned long long)__float2ll_rn(w * 4294967296.0f);
                     atomicAdd(&acc[(cx * T + cy) * T + cz], q);
                 }
             }
@@ -709,7 +712,7 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
 
     double mass_acc = 0.0;
     for (int c = threadIdx.x; c < cells; c += blockDim.x) {
-        const unsigned long long q = acc[c];
+        const long long q = (long long)acc[c];
         mass_acc += (double)q;
         const double mcell = (double)q * Q.fixed_to_mass;
         const int lz = c % T, ly = (c / T) % T, lx = c / (T * T);
@@ -1022,6 +1025,158 @@ __global__ void k_check_mass(uint64_t n, const double* __restrict__ m, uint32_t*
     bool bad = false;
     if (p < n) bad = !isfinite(m[p]) || m[p] < 0.0;
     if (__ballot(bad) && lane_id() == 0) atomicOr(err, ERR_MASS_INVALID);
+}
+
+// ===========================================================================
+// Adjoint (vector-Jacobian products) of the fused step.  The forward step is
+//   p' = p + (k_prev + k_next) g(x),   x' = wrap(x + d_next p'),
+//   g_j(x) = sum_c W(x_j - c) (-D phi)(c),  phi = G s,  s_c = (m_c - M/N_c)/dx^3,
+//   m_c = sum_k m W(x_k - c),   D_a f(c) = (f(c + e_a) - f(c - e_a)) / (2 dx).
+// For an adjoint vector u (one 3-vector per particle) the product
+// J_g(x)^T u has two parts per particle k:
+//   (a) through particle k's own gather weights:
+//       sum_c dW(x_k - c)/dx_k (u_k . (-D phi)(c)),
+//   (b) through the potential: with A_a = sum_j u_j,a W(x_j - c) deposited,
+//       chi = G (sum_a D_a A_a) (G symmetric, D_a antisymmetric),
+//       (m / dx^3) sum_c chi(c) dW(x_k - c)/dx_k.
+// ===========================================================================
+
+// u = lp + d_next lx (the momentum adjoint entering the kick), written over lp.
+__global__ void k_adj_kick(uint64_t n, const double* __restrict__ lx, const double* __restrict__ ly,
+                           const double* __restrict__ lz, double* __restrict__ lpx, double* __restrict__ lpy,
+                           double* __restrict__ lpz, double d_next, int has_next) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || !has_next) return;
+    lpx[p] = lpx[p] + d_next * lx[p];
+    lpy[p] = lpy[p] + d_next * ly[p];
+    lpz[p] = lpz[p] + d_next * lz[p];
+}
+
+// max |q| over particles (per block, then over blocks): the signed deposit unit.
+__global__ void k_absmax(uint64_t n, const double* __restrict__ q, double* __restrict__ out) {
+    __shared__ double sh[256];
+    double m = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        const double v = fabs(q[i]);
+        m = v > m ? v : m;
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + k]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+// Per-particle values q, scaled by inv_unit, into bucket order for the
+// binary32 record deposit (keys/ranks/offsets of the same positions).
+__global__ void k_scatter_values(uint64_t n, const double* __restrict__ q, double inv_unit,
+                                 const uint32_t* __restrict__ keys, const uint32_t* __restrict__ ranks,
+                                 const uint32_t* __restrict__ offsets, float* __restrict__ out) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t k = keys[p];
+    if (k == 0xffffffffu) return;
+    out[offsets[k] + ranks[p]] = (float)(q[p] * inv_unit);
+}
+
+// out = sum_a D_a A_a (periodic centred differences), one box.
+template <typename T>
+__global__ void k_div_stencil(const T* __restrict__ ax, const T* __restrict__ ay, const T* __restrict__ az,
+                              T* __restrict__ out, int N, double inv_2dx) {
+    const uint64_t NN = (uint64_t)N * N;
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= NN * N) return;
+    const int iz = (int)(c % N), iy = (int)((c / N) % N), ix = (int)(c / NN);
+    const uint64_t xp = (uint64_t)wrap_cell(ix + 1, N) * NN + (uint64_t)iy * N + iz;
+    const uint64_t xm = (uint64_t)wrap_cell(ix - 1, N) * NN + (uint64_t)iy * N + iz;
+    const uint64_t yp = (uint64_t)ix * NN + (uint64_t)wrap_cell(iy + 1, N) * N + iz;
+    const uint64_t ym = (uint64_t)ix * NN + (uint64_t)wrap_cell(iy - 1, N) * N + iz;
+    const uint64_t zp = (uint64_t)ix * NN + (uint64_t)iy * N + wrap_cell(iz + 1, N);
+    const uint64_t zm = (uint64_t)ix * NN + (uint64_t)iy * N + wrap_cell(iz - 1, N);
+    const double d = ((double)ax[xp] - (double)ax[xm]) + ((double)ay[yp] - (double)ay[ym]) +
+                     ((double)az[zp] - (double)az[zm]);
+    out[c] = (T)(d * inv_2dx);
+}
+
+template <typename T>
+struct AdjGatherArgs {
+    Geom g;
+    uint64_t n;
+    const double* x;
+    const double* y;
+    const double* z;
+    const double* ux;    // adjoint vector per particle (term (a)); null: no term (a)
+    const double* uy;
+    const double* uz;
+    const T* phi;        // forward potential (term (a))
+    const T* chi;        // scalar field of term (b)
+    double coef_a;       // lambda += coef_a * (a) + coef_b * (b)
+    double coef_b;
+    double* lx;
+    double* ly;
+    double* lz;
+    uint32_t* err;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_gather(AdjGatherArgs<T> a) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    const Geom& g = a.g;
+    const int N = g.N;
+    int ix, iy, iz;
+    double fx, fy, fz;
+    if (!(cic_axis(a.x[p], g, ix, fx) && cic_axis(a.y[p], g, iy, fy) && cic_axis(a.z[p], g, iz, fz))) {
+        atomicOr(a.err, ERR_POSITION_NONFINITE);
+        return;
+    }
+    const double inv_dx = 1.0 / g.dx;
+    const double w[3][2] = {{1.0 - fx, fx}, {1.0 - fy, fy}, {1.0 - fz, fz}};
+    const double dw[2] = {-inv_dx, inv_dx};
+    const uint64_t NN = (uint64_t)N * N;
+    const uint64_t boff = (uint64_t)box_of(p, g) * NN * N;
+    double ta[3] = {0.0, 0.0, 0.0}, tb[3] = {0.0, 0.0, 0.0};
+    const bool with_a = a.ux != nullptr;
+    const double u[3] = {with_a ? a.ux[p] : 0.0, with_a ? a.uy[p] : 0.0, with_a ? a.uz[p] : 0.0};
+    const double two_dx = 2.0 * g.dx;
+#pragma unroll
+    for (int ox = 0; ox < 2; ++ox) {
+        const int cx = wrap_cell(ix + ox, N);
+#pragma unroll
+        for (int oy = 0; oy < 2; ++oy) {
+            const int cy = wrap_cell(iy + oy, N);
+#pragma unroll
+            for (int oz = 0; oz < 2; ++oz) {
+                const int cz = wrap_cell(iz + oz, N);
+                const uint64_t c = boff + (uint64_t)cx * NN + (uint64_t)cy * N + cz;
+                // derivative of the corner weight along each axis
+                const double dwc[3] = {dw[ox] * w[1][oy] * w[2][oz], w[0][ox] * dw[oy] * w[2][oz],
+                                       w[0][ox] * w[1][oy] * dw[oz]};
+                const double xv = (double)a.chi[c];
+                double s = 0.0;
+                if (with_a) {
+                    const T* phi = a.phi + boff;
+                    const double gx = -((double)phi[(uint64_t)wrap_cell(cx + 1, N) * NN + (uint64_t)cy * N + cz] -
+                                        (double)phi[(uint64_t)wrap_cell(cx - 1, N) * NN + (uint64_t)cy * N + cz]) / two_dx;
+                    const double gy = -((double)phi[(uint64_t)cx * NN + (uint64_t)wrap_cell(cy + 1, N) * N + cz] -
+                                        (double)phi[(uint64_t)cx * NN + (uint64_t)wrap_cell(cy - 1, N) * N + cz]) / two_dx;
+                    const double gz = -((double)phi[(uint64_t)cx * NN + (uint64_t)cy * N + wrap_cell(cz + 1, N)] -
+                                        (double)phi[(uint64_t)cx * NN + (uint64_t)cy * N + wrap_cell(cz - 1, N)]) / two_dx;
+                    s = u[0] * gx + u[1] * gy + u[2] * gz;
+                }
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    ta[d] += dwc[d] * s;
+                    tb[d] += dwc[d] * xv;
+                }
+            }
+        }
+    }
+    a.lx[p] += a.coef_a * ta[0] + a.coef_b * tb[0];
+    a.ly[p] += a.coef_a * ta[1] + a.coef_b * tb[1];
+    a.lz[p] += a.coef_a * ta[2] + a.coef_b * tb[2];
 }
 
 }  // namespace hy

@@ -72,14 +72,24 @@ class Cosmology:
         return d, f
 
 
-def lpt_ics(n_side: int, box: float, a_start: float, seed: int = 1,
-            cosmo: Cosmology | None = None, device=None):
-    """Return (pos, mom) as lists of three tensors (float64) of n_side^3 particles."""
+def white_noise(n_side: int, seed: int, device=None):
+    """Seeded unit Gaussian white noise on the n_side^3 IC mesh (binary32)."""
     import torch  # data generation only
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    return torch.randn((n_side, n_side, n_side), generator=gen, device=device, dtype=torch.float32)
+
+
+def lpt_from_noise(noise, box: float, a_start: float, cosmo: Cosmology | None = None):
+    """1LPT lattice ICs from a white-noise field (torch ops only, so the map
+    noise -> (positions, momenta) is differentiable with torch.autograd; it is
+    linear in the noise apart from the periodic wrap).  Returns (pos, mom) as
+    lists of three float64 tensors of n^3 particles."""
+    import torch
 
     cosmo = cosmo or Cosmology()
-    device = device or ("cuda" if torch.cuda.is_available() else "cpu")
-    n = n_side
+    device = noise.device
+    n = noise.shape[0]
     D, f = cosmo.growth(a_start)
     H = 100.0 * float(cosmo.E(a_start))
     vel_fac = a_start * a_start * H * f
@@ -96,13 +106,7 @@ def lpt_ics(n_side: int, box: float, a_start: float, seed: int = 1,
     vcell = (box / n) ** 3
     amp = torch.as_tensor(np.sqrt(pk / vcell), device=device) * D
     amp[0, 0, 0] = 0.0
-
-    gen = torch.Generator(device=device)
-    gen.manual_seed(int(seed))
-    noise = torch.randn((n, n, n), generator=gen, device=device, dtype=torch.float32)
     delta_k = torch.fft.rfftn(noise).to(torch.complex128) * amp
-    del noise
-
     k2 = k2.clone()
     k2[0, 0, 0] = 1.0
     nyq = n // 2
@@ -126,6 +130,16 @@ def lpt_ics(n_side: int, box: float, a_start: float, seed: int = 1,
         mom.append((psi * vel_fac).reshape(-1).contiguous())
         del psi
     return pos, mom
+
+
+def lpt_ics(n_side: int, box: float, a_start: float, seed: int = 1,
+            cosmo: Cosmology | None = None, device=None):
+    """Return (pos, mom) as lists of three tensors (float64) of n_side^3 particles."""
+    import torch  # data generation only
+
+    device = device or ("cuda" if torch.cuda.is_available() else "cpu")
+    noise = white_noise(n_side, seed, device)
+    return lpt_from_noise(noise, box, a_start, cosmo)
 
 
 def particle_mass(n_side: int, box: float, omega_m: float = 0.3) -> float:

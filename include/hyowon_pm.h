@@ -200,6 +200,28 @@ int hy_pm_get_tuning(hy_pm_ctx* ctx, hy_fft_tuning* out);
 int hy_pm_set_tuning(hy_pm_ctx* ctx, const hy_fft_tuning* in);
 int hy_pm_autotune(hy_pm_ctx* ctx, uint32_t reps, hy_fft_tuning* out);
 
+/* Adjoint (vector-Jacobian products) for differentiable runs.  No reference
+ * counterpart: the reference is not differentiable.  Device pointers, single-box
+ * contexts, uniform mass; adjoint arrays are binary64.
+ *
+ * hy_pm_kdk_vjp is the exact reverse of one hy_pm_kdk call evaluated at the
+ * positions (x, y, z) that call read.  On entry lx.. / lpx.. hold dL/dx_out and
+ * dL/dp_out; on return dL/dx_in and dL/dp_in:
+ *   u   = dL/dp_out + d_next dL/dx_out         (HY_HAS_NEXT; else dL/dp_out)
+ *   dL/dp_in = u
+ *   dL/dx_in = dL/dx_out + (k_prev + k_next) J_g(x)^T u
+ * where J_g is the Jacobian of the PM acceleration (deposit -> Poisson ->
+ * centred-gradient gather).  Periodic wraps have unit derivative.
+ *
+ * hy_pm_density_vjp adds coef * d/dx_k sum_c u_c m_c(x) to lx.. for a mesh
+ * u (context precision), m_c the CIC-deposited mass of cell c. */
+int hy_pm_kdk_vjp(hy_pm_ctx* ctx, uint64_t n, const double* x, const double* y, const double* z,
+                  double uniform_mass, double* lx, double* ly, double* lz, double* lpx, double* lpy,
+                  double* lpz, double k_prev, double k_next, double d_next, uint32_t flags);
+int hy_pm_density_vjp(hy_pm_ctx* ctx, uint64_t n, const double* x, const double* y, const double* z,
+                      double uniform_mass, const void* u_mesh, double coef, double* lx, double* ly,
+                      double* lz);
+
 /* Ensemble step in one HIP graph launch.  No reference counterpart (the
  * reference advances one box per process): several contexts, usually one per
  * box geometry with many boxes each, that share one stream are advanced

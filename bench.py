@@ -347,6 +347,89 @@ def run_ensemble(args):
     }
 
 
+def run_adjoint(args):
+    """Config-4 workload: 128^3 particles from a 128^3 white-noise field (seed
+    7), 256^3 mesh, L = 256 Mpc/h, 16 KDK steps z = 19 -> 0; loss = mean
+    squared difference between the final CIC density contrast and that of the
+    seed-8 field run through the same forward; hand-written adjoint with
+    checkpoints every 4 refreshes.  Reports forward / backward times and the
+    gradient checks: central differences (eps = 1e-3 as specified, and 1e-4)
+    on 64 random noise components in binary64, and FP32 vs binary64 gradients.
+    The loss is piecewise smooth (CIC weights kink at cell faces): the
+    central-difference deviation from the adjoint shrinks with eps as fewer
+    particles cross a face inside the differencing interval."""
+    import numpy as np
+    import torch
+    import hyowon_amd as H
+    from hyowon_amd import ic
+    from hyowon_amd.adjoint import cic_density, density_loss, pm_run
+    from hyowon_amd.ensemble import schedule
+    n_side, N, L, steps = 128, 256, 256.0, 16
+    a0 = 1.0 / 20.0
+    K1, D, K2 = schedule(a0, 1.0, steps)
+    dev = torch.device("cuda:0")
+    noise = ic.white_noise(n_side, 7, dev).to(torch.float64)
+    out = {}
+    grads = {}
+    for prec in ("fp32", "fp64"):
+        s = H.PMSolver(L, N, precision=prec, max_particles=n_side ** 3)
+        pdt = torch.float64 if prec == "fp64" else torch.float32
+        m = ic.particle_mass(n_side, L)
+        with torch.no_grad():  # target: the seed-8 field through the same forward
+            pos, mom = ic.lpt_from_noise(ic.white_noise(n_side, 8, dev).to(torch.float64), L, a0)
+            st = pm_run((pos[0], pos[1], pos[2], mom[0].to(pdt), mom[1].to(pdt), mom[2].to(pdt)), s, m,
+                        K1, D, K2, 4)
+            target = cic_density(st[0], st[1], st[2], s, m).to(torch.float64)
+        warm = noise.clone().requires_grad_(True)  # first launches load code objects
+        density_loss(warm, target, s, L, a0, K1, D, K2, checkpoint_every=4, precision=prec).backward()
+        nz = noise.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss = density_loss(nz, target, s, L, a0, K1, D, K2, checkpoint_every=4, precision=prec)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        grads[prec] = nz.grad.detach().reshape(-1).clone()
+        out[prec] = {"loss": float(loss), "forward_ms": (t1 - t0) * 1e3, "backward_ms": (t2 - t1) * 1e3}
+        if prec == "fp64":
+            rng = np.random.default_rng(2024)
+            idx = rng.choice(noise.numel(), size=64, replace=False)
+            flat = noise.reshape(-1)
+            for eps in (1e-3, 1e-4, 1e-5):
+                errs, fds = [], []
+                for i in idx:
+                    lp = flat.clone()
+                    lp[i] += eps
+                    lm = flat.clone()
+                    lm[i] -= eps
+                    with torch.no_grad():
+                        Lp = float(density_loss(lp.view_as(noise), target, s, L, a0, K1, D, K2,
+                                                precision="fp64"))
+                        Lm = float(density_loss(lm.view_as(noise), target, s, L, a0, K1, D, K2,
+                                                precision="fp64"))
+                    fds.append((Lp - Lm) / (2 * eps))
+                    if len(fds) % 16 == 0:
+                        print(f"finite differences eps={eps:g}: {len(fds)}/64", file=sys.stderr, flush=True)
+                g = grads["fp64"][torch.as_tensor(idx, device=dev)].cpu().numpy()
+                fds = np.array(fds)
+                rel = np.abs(g - fds) / np.abs(fds)
+                out[f"fd_eps_{eps:g}"] = {"median_rel_err": float(np.median(rel)), "max_rel_err": float(rel.max()),
+                                          "l2_rel_err": float(np.linalg.norm(g - fds) / np.linalg.norm(fds)),
+                                          "components": 64}
+        s.close()
+        torch.cuda.empty_cache()
+    g32, g64 = grads["fp32"], grads["fp64"]
+    out["fp32_vs_fp64_grad_l2_rel"] = float(torch.linalg.norm(g32 - g64) / torch.linalg.norm(g64))
+    return {"metric": "differentiable PM run: forward + hand-written adjoint (config 4)",
+            "value": out["fp32"]["forward_ms"] + out["fp32"]["backward_ms"], "unit": "ms",
+            "higher_is_better": False, "n_gpus": 1,
+            "config": {"workload": "128^3 particles from white noise (seed 7), 256^3 mesh, L=256, 16 KDK steps "
+                                   "z=19->0, density-MSE loss vs the seed-8 forward, checkpoints every 4"},
+            "results": out}
+
+
 def cpu_baseline(args):
     """The oracle (binary64 C, OpenMP over all host cores) on a bounded sample of
     the same workload: one force refresh + K1/drift + K2 of a 128^3-particle,
@@ -397,9 +480,9 @@ def main():
                     help="FP32-vs-FP64 force, final-position and P(k) errors (full run)")
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
                     help="measure the FFT launch plan before timing (untimed)")
-    ap.add_argument("--workload", default="config2", choices=["config2", "ensemble"],
+    ap.add_argument("--workload", default="config2", choices=["config2", "ensemble", "adjoint"],
                     help="config2: the 512^3 / 1024^3 KDK step (default); ensemble: the 192-box "
-                         "graph-launched ensemble step")
+                         "graph-launched ensemble step; adjoint: the differentiable 128^3 / 256^3 run")
     ap.add_argument("--fft-plan", default="",
                     help="pin the FFT launch plan 'v0,v1,v2,v3,v4[:plane_group]' instead of autotuning")
     ap.add_argument("--rocfft", action=argparse.BooleanOptionalAction, default=True,
@@ -417,6 +500,9 @@ def main():
 
     if args.workload == "ensemble":
         print(json.dumps(run_ensemble(args)), flush=True)
+        return
+    if args.workload == "adjoint":
+        print(json.dumps(run_adjoint(args)), flush=True)
         return
     n, N, res = run_gpu(args)
     main_prec = args.precisions[0]

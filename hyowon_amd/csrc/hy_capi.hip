@@ -356,12 +356,14 @@ DepositPlan plan_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double
 
 void launch_deposit(hy_pm_ctx* c, const hy::DepParams& P) {
     if (c->prec == HY_PREC_FP64)
-        hipLaunchKernelGGL(hy::k_deposit<double>, dim3(c->ntiles * c->nbox), dim3(hy::DEP_THREADS), 0, c->stream, P,
+        hipLaunchKernelGGL((c->geom.T == 16 ? hy::k_deposit<double, 16> : hy::k_deposit<double, 0>),
+                           dim3(c->ntiles * c->nbox), dim3(hy::DEP_THREADS), 0, c->stream, P,
                            static_cast<double*>(c->R));
     else {
         hy::DepFxParams Q{P, c->mass_unit, std::ldexp(c->mass_unit, -32), c->rec,
                           P.masses ? c->rec_mass : nullptr};
-        hipLaunchKernelGGL(hy::k_deposit_fixed, dim3(c->ntiles * c->nbox), dim3(hy::DEPFX_THREADS), 0, c->stream, Q,
+        hipLaunchKernelGGL(c->geom.T == 16 ? hy::k_deposit_fixed<16> : hy::k_deposit_fixed<0>,
+                           dim3(c->ntiles * c->nbox), dim3(hy::DEPFX_THREADS), 0, c->stream, Q,
                            static_cast<float*>(c->R));
     }
     launch_check("k_deposit");
@@ -440,7 +442,7 @@ void run_fft(hy_pm_ctx* c) {
 // tiles z-rows exactly (S a multiple of 256); about 64K particles (the
 // blocks resident on one XCD) per column tile.  Identity otherwise.
 hy::LatticeTiling lattice_tiling(uint64_t n, uint32_t block) {
-    hy::LatticeTiling t{0, 0, 0, 0};
+    hy::LatticeTiling t{0, 0, 0, 0, -1, -1, -1, -1};
     if (std::getenv("HY_NO_LATTICE_SCHEDULE")) return t;
     const uint64_t S = (uint64_t)std::llround(std::cbrt((double)n));
     if (S == 0 || S * S * S != n || S % block != 0 || S > 2048) return t;
@@ -456,12 +458,17 @@ hy::LatticeTiling lattice_tiling(uint64_t n, uint32_t block) {
     t.TX = TX;
     t.TY = TY;
     t.ZC = ZC;
+    auto lg = [](uint64_t v) { int l = 0; while ((1ull << l) < v) ++l; return (1ull << l) == v ? l : -1; };
+    const int a = lg(S), b = lg(TX), cc = lg(TY), d = lg(ZC);
+    if (a >= 0 && b >= 0 && cc >= 0 && d >= 0) {
+        t.lS = a; t.lTX = b; t.lTY = cc; t.lZC = d;
+    }
     return t;
 }
 
 template <typename T, typename PT>
 void launch_gather(hy_pm_ctx* c, hy::GatherArgs<T, PT> a) {
-    a.lat = c->nbox == 1 ? lattice_tiling(a.n, 256) : hy::LatticeTiling{0, 0, 0, 0};
+    a.lat = c->nbox == 1 ? lattice_tiling(a.n, 256) : hy::LatticeTiling{0, 0, 0, 0, -1, -1, -1, -1};
     a.dfac = c->dfac;
     hipLaunchKernelGGL((hy::k_gather_kdk<T, PT>), dim3(blocks_for(a.n, 256)), dim3(256), 0, c->stream, a);
     launch_check("k_gather_kdk");

@@ -29,7 +29,10 @@ namespace hy {
 // Deposit bucket of a base cell: tile (edge T) and boundary class
 // (bit2/1/0: last cell of the tile along x/y/z).
 __device__ __forceinline__ uint32_t bucket_of(int bx, int by, int bz, const Geom& g) {
-    const int tx = bx / g.T, ty = by / g.T, tz = bz / g.T;
+    // T = 16 (every mesh divisible by 16) takes shifts instead of divisions
+    const int tx = g.T == 16 ? bx >> 4 : bx / g.T;
+    const int ty = g.T == 16 ? by >> 4 : by / g.T;
+    const int tz = g.T == 16 ? bz >> 4 : bz / g.T;
     const uint32_t cls = ((uint32_t)(bx - tx * g.T == g.T - 1) << 2)
                        | ((uint32_t)(by - ty * g.T == g.T - 1) << 1)
                        | (uint32_t)(bz - tz * g.T == g.T - 1);
@@ -223,7 +226,8 @@ __global__ void k_scatter_records(uint64_t n, const double* __restrict__ x, cons
     }
     const uint32_t dst = offsets[k] + ranks[p];
     if (order) order[dst] = (uint32_t)p;
-    const int lx = bx % g.T, ly = by % g.T, lz = bz % g.T;
+    const int lx = g.T == 16 ? bx & 15 : bx % g.T, ly = g.T == 16 ? by & 15 : by % g.T,
+              lz = g.T == 16 ? bz & 15 : bz % g.T;
     DepRecord r;
     r.cell = (uint32_t)((lx * g.T + ly) * g.T + lz);
     r.fx = (float)fx;
@@ -401,16 +405,17 @@ __device__ __forceinline__ T contrast_of(T sum, const DepParams& P) {
     return (T)v;
 }
 
-template <typename T, bool LDS_PAYLOAD, typename KP, typename FP>
+template <typename T, bool LDS_PAYLOAD, int TE, typename KP, typename FP>
 __device__ __forceinline__ void deposit_cells(const DepParams& P, T* __restrict__ mesh, int tx,
                                               int ty, int tz, KP keys, uint32_t M,
                                               const uint32_t* start, FP fx, FP fy, FP fz, FP fm,
                                               double& mass_acc) {
     const Geom& g = P.g;
-    const int Te = g.T + 1;
-    const int cells = g.T * g.T * g.T;
+    const int TT = TE > 0 ? TE : g.T;
+    const int Te = TT + 1;
+    const int cells = TT * TT * TT;
     for (int c = threadIdx.x; c < cells; c += blockDim.x) {
-        const int lz = c % g.T, ly = (c / g.T) % g.T, lx = c / (g.T * g.T);
+        const int lz = c % TT, ly = (c / TT) % TT, lx = c / (TT * TT);
         T sum = (T)0;
         for (int phase = 0; phase < 2; ++phase) {
             const int ex = lx + 1 - phase;  // phase 0: base x == cell x; phase 1: base x == cell x - 1
@@ -520,7 +525,7 @@ __device__ __forceinline__ void load_source_ranges(const uint32_t* __restrict__ 
     __syncthreads();
 }
 
-template <typename T>
+template <typename T, int TE = 0>
 __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restrict__ mesh) {
     using Cfg = DepCfg<T>;
     constexpr int CAP = Cfg::CAP;
@@ -539,7 +544,8 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restr
     if (gtile >= ntiles * g.nbox) return;
     const uint32_t box = gtile / ntiles, tile = gtile % ntiles;
     const int tz = tile % nT, ty = (tile / nT) % nT, tx = tile / (nT * nT);
-    const int Te = g.T + 1;
+    const int TT = TE > 0 ? TE : g.T;
+    const int Te = TT + 1;
     const int ncell = Te * Te * Te;
     mesh += (uint64_t)box * g.N * g.N * g.N;
 
@@ -598,12 +604,12 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restr
     if (use_lds) {
         wg_sort_u64(sh_keys, M);
         build_cell_starts(sh_keys, M, sh_start, ncell);
-        deposit_cells<T, true>(P, mesh, tx, ty, tz, sh_keys, M, sh_start, sh_fx, sh_fy, sh_fz,
+        deposit_cells<T, true, TE>(P, mesh, tx, ty, tz, sh_keys, M, sh_start, sh_fx, sh_fy, sh_fz,
                                sh_m, mass_acc);
     } else {
         wg_sort_u64(gkeys, M);
         build_cell_starts(gkeys, M, sh_start, ncell);
-        deposit_cells<T, false>(P, mesh, tx, ty, tz, gkeys, M, sh_start, sh_fx, sh_fy, sh_fz,
+        deposit_cells<T, false, TE>(P, mesh, tx, ty, tz, gkeys, M, sh_start, sh_fx, sh_fy, sh_fz,
                                 sh_m, mass_acc);
     }
 
@@ -617,8 +623,8 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restr
     if (threadIdx.x == 0) P.tile_mass[gtile] = sh_red[0];
 }
 
-template __global__ void k_deposit<float>(DepParams, float*);
-template __global__ void k_deposit<double>(DepParams, double*);
+template __global__ void k_deposit<double, 0>(DepParams, double*);
+template __global__ void k_deposit<double, 16>(DepParams, double*);
 
 // ---------------------------------------------------------------------------
 // Binary32 deposit: fixed-point accumulation in LDS.
@@ -642,6 +648,9 @@ struct DepFxParams {
     const float* rec_mass;          // explicit masses / mass_unit in bucket order (or null)
 };
 
+// TE: compile-time tile edge (16, the edge of every mesh divisible by 16:
+// cell index arithmetic becomes shifts), or 0 to read it from the geometry.
+template <int TE>
 __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, float* __restrict__ mesh) {
     const DepParams& P = Q.base;
     __shared__ unsigned long long acc[DEPFX_MAX_CELLS];
@@ -652,7 +661,7 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
 
     const Geom& g = P.g;
     const int nT = g.nT;
-    const int T = g.T;
+    const int T = TE > 0 ? TE : g.T;
     const uint32_t ntiles = (uint32_t)nT * nT * nT;
     const uint32_t gtile = xcd_block(blockIdx.x, gridDim.x);  // box-major tile index
     if (gtile >= ntiles * g.nbox) return;
@@ -795,10 +804,21 @@ __global__ void k_reduce_sum(const double* __restrict__ v, uint64_t n, double* _
 // compact region of the potential that stays in its L2.  S == 0: identity.
 struct LatticeTiling {
     uint32_t S, TX, TY, ZC;  // lattice side, column tile, 256-chunks per z-row
+    int lS, lTX, lTY, lZC;   // log2 of each when all four are powers of two, else lS < 0
 };
 
 __device__ __forceinline__ uint32_t lattice_block(uint32_t L, const LatticeTiling& t) {
     if (t.S == 0) return L;
+    if (t.lS >= 0) {  // powers of two: shifts and masks
+        const int lper = t.lTX + t.lTY + t.lZC;
+        const uint32_t tile = L >> lper, w = L & ((1u << lper) - 1u);
+        const int ltiles_y = t.lS - t.lTY;
+        const uint32_t tx = tile >> ltiles_y, ty = tile & ((1u << ltiles_y) - 1u);
+        const uint32_t zc = w & ((1u << t.lZC) - 1u), dy = (w >> t.lZC) & ((1u << t.lTY) - 1u);
+        const uint32_t dx = w >> (t.lZC + t.lTY);
+        const uint32_t ix = (tx << t.lTX) + dx, iy = (ty << t.lTY) + dy;
+        return (((ix << t.lS) + iy) << t.lZC) + zc;
+    }
     // every quantity is below S^2 * ZC = n / 256 < 2^32
     const uint32_t per_tile = t.TX * t.TY * t.ZC;
     const uint32_t tile = L / per_tile, w = L % per_tile;
@@ -906,7 +926,12 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
                         Y[j][u][v] = phi[xo[u + 1] + yo[j * 3] + zo[v + 1]];
                         Z[j][u][v] = phi[xo[u + 1] + yo[v + 1] + zo[j * 3]];
                     }
+            // -(f+ - f-) / (2 dx); when dx is a power of two the quotient equals
+            // the product with the exact reciprocal bit for bit
             const T two_dx = (T)(2.0 * g.dx);
+            const T inv_two_dx = (T)(0.5 * g.inv_dx);
+            const bool exact = g.inv_exact != 0;
+            auto cdiff = [&](T d) -> T { return exact ? (-d) * inv_two_dx : -d / two_dx; };
             T gx = 0, gy = 0, gz = 0;
 #pragma unroll
             for (int ox = 0; ox < 2; ++ox) {
@@ -922,9 +947,9 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
                         const T dxv = X[ox + 2][oy][oz] - X[ox][oy][oz];
                         const T dyv = ypl - ymi;
                         const T dzv = zpl - zmi;
-                        gx += w * (-dxv / two_dx);
-                        gy += w * (-dyv / two_dx);
-                        gz += w * (-dzv / two_dx);
+                        gx += w * cdiff(dxv);
+                        gy += w * cdiff(dyv);
+                        gz += w * cdiff(dzv);
                     }
                 }
             }

@@ -55,7 +55,7 @@ def step_alg_bytes(n_p: int, N: int, real: int) -> float:
     return (96.0 * n_p + 24.0 * M) if real == 4 else (120.0 * n_p + 48.0 * M)
 
 
-STAGE_KERNEL = {"bucket": "hy::k_scatter_records", "deposit": "hy::k_deposit_fixed",
+STAGE_KERNEL = {"bucket": "hy::k_scatter_records", "deposit": "void hy::k_deposit_fixed<16>",
                 "gather_kdk": "hy::k_gather_kdk<float, float>"}
 FFT_KERNEL = {"fft_z_r2c": "k_fft_z_r2c", "fft_y_fwd": "k_fft_y_fwd", "fft_x_green": "k_fft_x_green",
               "fft_y_bwd": "k_fft_y_bwd", "fft_z_c2r": "k_fft_z_c2r"}
@@ -71,7 +71,7 @@ def measured_traffic(stage: str, n_side: int, N: int, prec: str, plan: dict):
         prof = json.load(f)
     if stage in FFT_KERNEL:
         v = plan["variant"][stage[4:]]
-        name = f"void hy::{FFT_KERNEL[stage]}<float, {N}, {v}>"
+        name = f"void hy::{FFT_KERNEL[stage].replace('k_fft_', 'k_tl_')}<float, {N}, {v}>"
     else:
         name = STAGE_KERNEL.get(stage)
     k = prof.get("kernels", {}).get(name)

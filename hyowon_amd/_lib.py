@@ -24,6 +24,7 @@ EXPORTS = [
     "hy_gravitational_constant", "hy_rho_crit0", "hy_pm_profile", "hy_pm_profile_read",
     "hy_pm_get_tuning", "hy_pm_set_tuning", "hy_pm_autotune", "hy_ensemble_create", "hy_ensemble_step",
     "hy_ensemble_check", "hy_ensemble_destroy", "hy_pm_kdk_vjp", "hy_pm_density_vjp",
+    "hy_ic_gaussian_modes", "hy_ic_last_error",
 ]
 
 HY_FFT_PASSES = 5
@@ -150,6 +151,8 @@ def lib():
     fn("hy_ensemble_destroy", i, vp)
     fn("hy_pm_kdk_vjp", i, vp, u64, vp, vp, vp, d, vp, vp, vp, vp, vp, vp, d, d, d, u32)
     fn("hy_pm_density_vjp", i, vp, u64, vp, vp, vp, d, vp, d, vp, vp, vp)
+    fn("hy_ic_gaussian_modes", i, u64, u64, u64, vp, vp, u32, i, vp)
+    fn("hy_ic_last_error", C.c_char_p)
     _lib = L
     return L
 

@@ -80,6 +80,39 @@ def white_noise(n_side: int, seed: int, device=None):
     return torch.randn((n_side, n_side, n_side), generator=gen, device=device, dtype=torch.float32)
 
 
+def variance_by_m2(n_side: int, box: float, a_start: float, cosmo: Cosmology | None = None,
+                   max_mode: int | None = None):
+    """Per-mode variance N^6/L^3 P(k) D1(a)^2 of the r2c modes of an n_side^3
+    IC mesh, indexed by m^2 = mx^2 + my^2 + mz^2 (float64, length 3 K^2 + 1
+    with K = n_side // 2); entry 0 is unused."""
+    cosmo = cosmo or Cosmology()
+    D, _ = cosmo.growth(a_start)
+    K = n_side // 2
+    m2 = np.arange(3 * K * K + 1, dtype=np.float64)
+    k = np.sqrt(m2) * (2.0 * math.pi / box)
+    pk = np.zeros_like(k)
+    pk[1:] = cosmo.power(k[1:])
+    return float(n_side) ** 6 / box ** 3 * pk * D * D
+
+
+def philox_modes(n_side: int, box: float, a_start: float, seed: int, cosmo: Cosmology | None = None,
+                 max_mode: int | None = None, device=0):
+    """Mode-keyed Gaussian initial density modes delta_k (complex128 r2c half
+    spectrum [n][n][n/2+1] on the GPU) drawn by libhyowon_pm's
+    hy_ic_gaussian_modes: the reference's Philox4x32-10 mode-counter
+    convention, so a mode's draw depends only on (seed, integer mode)."""
+    import torch
+    from ._lib import check, lib
+    var = torch.as_tensor(variance_by_m2(n_side, box, a_start, cosmo), device=f"cuda:{device}")
+    out = torch.empty((n_side, n_side, n_side // 2 + 1), dtype=torch.complex128, device=f"cuda:{device}")
+    K = n_side // 2 if max_mode is None else int(max_mode)
+    torch.cuda.synchronize(device)
+    rc = lib().hy_ic_gaussian_modes(n_side, int(seed), K, var.data_ptr(), out.data_ptr(), 0, device, None)
+    if rc != 0:
+        raise RuntimeError(lib().hy_ic_last_error().decode())
+    return out
+
+
 def lpt_from_noise(noise, box: float, a_start: float, cosmo: Cosmology | None = None):
     """1LPT lattice ICs from a white-noise field (torch ops only, so the map
     noise -> (positions, momenta) is differentiable with torch.autograd; it is
@@ -107,6 +140,28 @@ def lpt_from_noise(noise, box: float, a_start: float, cosmo: Cosmology | None = 
     amp = torch.as_tensor(np.sqrt(pk / vcell), device=device) * D
     amp[0, 0, 0] = 0.0
     delta_k = torch.fft.rfftn(noise).to(torch.complex128) * amp
+    return lpt_from_delta_k(delta_k, box, a_start, cosmo)
+
+
+def lpt_from_delta_k(delta_k, box: float, a_start: float, cosmo: Cosmology | None = None):
+    """1LPT lattice ICs from the initial density modes delta_k (r2c half
+    spectrum of the n^3 lattice mesh, at a_start): psi = IFFT(i k / k^2
+    delta_k), x = q + psi, p = a^2 H f psi.  Torch ops (differentiable)."""
+    import torch
+
+    cosmo = cosmo or Cosmology()
+    device = delta_k.device
+    n = delta_k.shape[0]
+    _, f = cosmo.growth(a_start)
+    H = 100.0 * float(cosmo.E(a_start))
+    vel_fac = a_start * a_start * H * f
+    kf = 2.0 * math.pi / box
+    m = torch.fft.fftfreq(n, d=1.0 / n, device=device, dtype=torch.float64)
+    mz = torch.fft.rfftfreq(n, d=1.0 / n, device=device, dtype=torch.float64)
+    kx = (m * kf).view(n, 1, 1)
+    ky = (m * kf).view(1, n, 1)
+    kz = (mz * kf).view(1, 1, n // 2 + 1)
+    k2 = kx * kx + ky * ky + kz * kz
     k2 = k2.clone()
     k2[0, 0, 0] = 1.0
     nyq = n // 2
@@ -133,11 +188,19 @@ def lpt_from_noise(noise, box: float, a_start: float, cosmo: Cosmology | None = 
 
 
 def lpt_ics(n_side: int, box: float, a_start: float, seed: int = 1,
-            cosmo: Cosmology | None = None, device=None):
-    """Return (pos, mom) as lists of three tensors (float64) of n_side^3 particles."""
+            cosmo: Cosmology | None = None, device=None, rng: str = "torch"):
+    """Return (pos, mom) as lists of three tensors (float64) of n_side^3 particles.
+
+    rng="torch": seeded torch white noise (fast synthetic ICs);
+    rng="philox": the reference's mode-keyed Philox Gaussian field, drawn on
+    the GPU by hy_ic_gaussian_modes."""
     import torch  # data generation only
 
     device = device or ("cuda" if torch.cuda.is_available() else "cpu")
+    if rng == "philox":
+        dev = torch.device(device)
+        dk = philox_modes(n_side, box, a_start, seed, cosmo, device=dev.index or 0)
+        return lpt_from_delta_k(dk, box, a_start, cosmo)
     noise = white_noise(n_side, seed, device)
     return lpt_from_noise(noise, box, a_start, cosmo)
 

@@ -247,6 +247,24 @@ int hy_ensemble_step(hy_ensemble* e, double k_prev, double k_next, double d_next
 int hy_ensemble_check(hy_ensemble* e);
 int hy_ensemble_destroy(hy_ensemble* e);
 
+/* Mode-keyed Gaussian initial field (src/ic/random_field.cpp:70-330 with
+ * src/ic/philox_rng.cpp:1-90), on the GPU.  Writes the r2c half spectrum
+ * out[(i*N + j)*(N/2+1) + k] (binary64 complex, device memory) of an N^3 IC
+ * mesh.  Each mode draws Philox4x32-10 with key (seed lo, seed hi) and counter
+ * (m_x, m_y, m_z, stream) of its canonical mode: on the k = 0 and Nyquist
+ * planes the mode whose conjugate (i, j) is lexicographically smaller takes the
+ * conjugate draw; self-conjugate modes are real with standard deviation
+ * sqrt(var), others have sqrt(var/2) per part; the zero mode and modes with
+ * any |m_axis| > max_mode are zero.  var = variance_by_m2[m_x^2+m_y^2+m_z^2]
+ * (device, 3 (N/2)^2 + 1 entries) is the caller's N^6/L^3 P(|m| 2 pi/L).
+ * flags: HY_IC_FIXED_AMPLITUDE (|X|^2 = var, phase from stream 1),
+ * HY_IC_REVERSED_PAIR (negated field).  Synchronises the stream. */
+#define HY_IC_FIXED_AMPLITUDE 0x1u
+#define HY_IC_REVERSED_PAIR   0x2u
+int hy_ic_gaussian_modes(uint64_t N, uint64_t seed, uint64_t max_mode, const double* variance_by_m2,
+                         void* out, uint32_t flags, int device, void* stream);
+const char* hy_ic_last_error(void);
+
 /* Host scalars of the step (binary64, long-double quadrature).
  * H(a): src/cosmology/cosmology_model.cpp:25-57,167-178.
  * D = int da/(a^3 H), K = int da/(a^2 H):

@@ -105,6 +105,10 @@ def lib():
         L.fn("or_time_boundaries", C.c_int, _d, _d, _d, _pd, _u64, _pd, _u64, _pu64)
         L.fn("or_run", C.c_int, _u64, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _d,
              _d, _u64, C.c_int, _d, _d, _pd, _u64)
+        L.fn("or_philox4x32_10", None, _pu32, _pu32, _pu32)
+        L.fn("or_uniform_pair", None, _u64, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, _pd)
+        L.fn("or_normal_pair", None, _u64, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, _pd)
+        L.fn("or_gaussian_modes", C.c_int, _u64, _u64, _u64, _pd, C.c_uint32, _pd)
         L.fn("or_set_threads", None, C.c_int)
         L.fn("or_get_threads", C.c_int)
         L.fn("or_gravitational_constant", _d)
@@ -146,6 +150,9 @@ def ref():
              _d, _d, _d, _d, _d, _d, _d, FORCE_CB, C.c_void_p, _pu64)
         L.fn("ref_gravitational_constant", _d)
         L.fn("ref_rho_crit0", _d)
+        L.fn("ref_philox4x32_10", None, _pu32, _pu32, _pu32)
+        L.fn("ref_uniform_pair", None, _u64, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, _pd)
+        L.fn("ref_normal_pair", None, _u64, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, _pd)
         _ref = L.lib
     return _ref
 
@@ -310,3 +317,20 @@ def set_threads(n: int) -> None:
 
 def get_threads() -> int:
     return int(lib().or_get_threads())
+
+
+def philox(counter, key):
+    c = np.ascontiguousarray(counter, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().or_philox4x32_10(_ptr(c, _pu32), _ptr(k, _pu32), _ptr(out, _pu32))
+    return out
+
+
+def gaussian_modes(N, seed, max_mode, variance_by_m2, flags=0):
+    """Mode-keyed Gaussian r2c half spectrum, complex128 [N][N][N/2+1]."""
+    var = _f64(variance_by_m2)
+    out = np.zeros((N, N, N // 2 + 1), dtype=np.complex128)
+    _check(lib().or_gaussian_modes(N, seed, max_mode, _ptr(var), flags, out.ctypes.data_as(_pd)),
+           "or_gaussian_modes")
+    return out

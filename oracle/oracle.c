@@ -1191,3 +1191,125 @@ int or_run(uint64_t n, double* x, double* y, double* z,
     }
     return err;
 }
+
+
+/* ------------------------------------------------------------------------ */
+/* Mode-keyed Gaussian initial field (src/ic/philox_rng.cpp:1-90,
+ * src/ic/random_field.cpp:70-330). */
+void or_philox4x32_10(const uint32_t* counter, const uint32_t* key, uint32_t* out) {
+    uint32_t c0 = counter[0], c1 = counter[1], c2 = counter[2], c3 = counter[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int round = 0; round < 10; ++round) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        if (round != 9) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+static double open53(uint32_t hi, uint32_t lo) {
+    const uint64_t bits = ((uint64_t)hi << 21) | ((uint64_t)lo >> 11);
+    if (bits == 0) return 0x1.0p-54;
+    return (double)bits * 0x1.0p-53;
+}
+
+void or_uniform_pair(uint64_t seed, int32_t kx, int32_t ky, int32_t kz, uint32_t stream, double* out) {
+    const uint32_t ctr[4] = {(uint32_t)kx, (uint32_t)ky, (uint32_t)kz, stream};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t w[4];
+    or_philox4x32_10(ctr, key, w);
+    out[0] = open53(w[0], w[1]);
+    out[1] = open53(w[2], w[3]);
+}
+
+void or_normal_pair(uint64_t seed, int32_t kx, int32_t ky, int32_t kz, uint32_t stream, double* out) {
+    double u[2];
+    or_uniform_pair(seed, kx, ky, kz, stream, u);
+    const double r = sqrt(-2.0 * log(u[0]));
+    const double th = 2.0 * 3.14159265358979323846 * u[1];
+    out[0] = r * cos(th);
+    out[1] = r * sin(th);
+}
+
+static long long or_mode(long long i, long long n) { return i <= n / 2 ? i : i - n; }
+
+int or_gaussian_modes(uint64_t N, uint64_t seed, uint64_t max_mode, const double* var_m2, uint32_t flags,
+                      double* out) {
+    const long long n = (long long)N, nh = n / 2 + 1;
+    const int fixed = (flags & 1u) != 0, reverse = (flags & 2u) != 0;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < n; ++i) {
+        for (long long j = 0; j < n; ++j) {
+            for (long long k = 0; k < nh; ++k) {
+                double* o = out + 2 * ((i * n + j) * nh + k);
+                const long long mx = or_mode(i, n), my = or_mode(j, n), mz = k;
+                o[0] = 0.0;
+                o[1] = 0.0;
+                if (mx == 0 && my == 0 && mz == 0) continue;
+                if (llabs(mx) > (long long)max_mode || llabs(my) > (long long)max_mode || mz > (long long)max_mode)
+                    continue;
+                const int plane = (k == 0) || (n % 2 == 0 && k == n / 2);
+                long long ci = i, cj = j;
+                int conj = 0, self = 0;
+                if (plane) {
+                    const long long ti = (n - i) % n, tj = (n - j) % n;
+                    self = ti == i && tj == j;
+                    if (!self && (ti < i || (ti == i && tj < j))) {
+                        ci = ti;
+                        cj = tj;
+                        conj = 1;
+                    }
+                }
+                const int32_t cx = (int32_t)or_mode(ci, n), cy = (int32_t)or_mode(cj, n), cz = (int32_t)mz;
+                const double var = var_m2[mx * mx + my * my + mz * mz];
+                double re, im;
+                if (fixed) {
+                    double u[2];
+                    or_uniform_pair(seed, cx, cy, cz, 1u, u);
+                    const double amp = sqrt(var);
+                    if (self) {
+                        re = (u[0] < 0.5 ? -1.0 : 1.0) * amp;
+                        im = 0.0;
+                    } else {
+                        const double ph = 2.0 * 3.14159265358979323846 * u[0];
+                        re = amp * cos(ph);
+                        im = amp * sin(ph);
+                        if (conj) im = -im;
+                    }
+                } else {
+                    double g[2];
+                    or_normal_pair(seed, cx, cy, cz, 0u, g);
+                    if (self) {
+                        re = sqrt(var) * g[0];
+                        im = 0.0;
+                    } else {
+                        const double sd = sqrt(var / 2.0);
+                        re = sd * g[0];
+                        im = sd * g[1];
+                        if (conj) im = -im;
+                    }
+                }
+                if (reverse) {
+                    re = -re;
+                    im = -im;
+                }
+                o[0] = re;
+                o[1] = im;
+            }
+        }
+    }
+    return 0;
+}

@@ -168,6 +168,28 @@ int or_run(uint64_t n, double* x, double* y, double* z,
            int deconvolve_cic, double omega_m, double omega_lambda,
            const double* boundaries, uint64_t n_boundaries);
 
+/* Mode-keyed Gaussian initial field (src/ic/philox_rng.cpp:1-90,
+ * src/ic/random_field.cpp:70-330).
+ * or_philox4x32_10: Random123 Philox4x32 with ten rounds (key bumped by the
+ * Weyl constants between rounds).
+ * or_uniform_pair: the block of counter (kx, ky, kz, stream), key (seed lo,
+ * seed hi), mapped to two uniforms in (0,1) on the 53-bit grid (0 -> 2^-54).
+ * or_normal_pair: Box-Muller of that pair.
+ * or_gaussian_modes: the r2c half spectrum [i][j][k] (k < N/2+1) of an N^3 IC
+ * mesh: zero mode and modes with any |m_axis| > max_mode are zero; on the
+ * k = 0 and Nyquist planes the mode with the lexicographically smaller
+ * conjugate (i, j) is drawn as the conjugate of that mode's draw; a
+ * self-conjugate mode is real with standard deviation sqrt(var); otherwise
+ * both parts have standard deviation sqrt(var / 2).  var comes from
+ * variance_by_m2[mx^2 + my^2 + mz^2].  flags bit 0: fixed amplitude (phase
+ * from stream 1), bit 1: reversed pair (negated field).  out holds
+ * N * N * (N/2+1) complex values (re, im). */
+void or_philox4x32_10(const uint32_t* counter, const uint32_t* key, uint32_t* out);
+void or_uniform_pair(uint64_t seed, int32_t kx, int32_t ky, int32_t kz, uint32_t stream, double* out);
+void or_normal_pair(uint64_t seed, int32_t kx, int32_t ky, int32_t kz, uint32_t stream, double* out);
+int or_gaussian_modes(uint64_t N, uint64_t seed, uint64_t max_mode, const double* variance_by_m2,
+                      uint32_t flags, double* out);
+
 /* Thread count used by OpenMP regions (0 = runtime default). */
 void or_set_threads(int threads);
 int or_get_threads(void);

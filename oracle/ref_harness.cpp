@@ -15,6 +15,7 @@
 #include "cosmo_nbody/cosmology/cosmology_model.hpp"
 #include "cosmo_nbody/cosmology/drift_kick_factors.hpp"
 #include "cosmo_nbody/cosmology/units.hpp"
+#include "cosmo_nbody/ic/philox_rng.hpp"
 #include "cosmo_nbody/math/cic_mass_conservation.hpp"
 #include "cosmo_nbody/math/density_normalization.hpp"
 #include "cosmo_nbody/math/exact_geometric_mean.hpp"
@@ -76,6 +77,23 @@ cn::config::CosmologyParams cosmology(double om, double ol) {
 extern "C" {
 
 double ref_gravitational_constant() { return cn::cosmology::units::G; }
+
+// Philox4x32-10 block and its open-interval uniform / Box-Muller maps
+// (src/ic/philox_rng.cpp).
+void ref_philox4x32_10(const uint32_t* counter, const uint32_t* key, uint32_t* out) {
+    const auto r = cn::ic::rng::philox4x32_10({counter[0], counter[1], counter[2], counter[3]}, {key[0], key[1]});
+    for (int i = 0; i < 4; ++i) out[i] = r[i];
+}
+void ref_uniform_pair(uint64_t key, int32_t kx, int32_t ky, int32_t kz, uint32_t stream, double* out) {
+    const auto [a, b] = cn::ic::rng::uniform_pair_open01(key, kx, ky, kz, stream);
+    out[0] = a;
+    out[1] = b;
+}
+void ref_normal_pair(uint64_t key, int32_t kx, int32_t ky, int32_t kz, uint32_t stream, double* out) {
+    const auto [a, b] = cn::ic::rng::normal_pair(key, kx, ky, kz, stream);
+    out[0] = a;
+    out[1] = b;
+}
 double ref_rho_crit0() { return cn::cosmology::units::rho_crit0; }
 
 double ref_wrap(double x, double L) { return cn::math::wrap(x, L); }

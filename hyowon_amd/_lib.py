@@ -24,7 +24,7 @@ EXPORTS = [
     "hy_gravitational_constant", "hy_rho_crit0", "hy_pm_profile", "hy_pm_profile_read",
     "hy_pm_get_tuning", "hy_pm_set_tuning", "hy_pm_autotune", "hy_ensemble_create", "hy_ensemble_step",
     "hy_ensemble_check", "hy_ensemble_destroy", "hy_pm_kdk_vjp", "hy_pm_density_vjp",
-    "hy_ic_gaussian_modes", "hy_ic_last_error",
+    "hy_ic_gaussian_modes", "hy_ic_last_error", "hy_pm_power_spectrum",
 ]
 
 HY_FFT_PASSES = 5
@@ -99,6 +99,21 @@ class hy_ensemble_member(C.Structure):
                 ("uniform_mass", C.c_double)]
 
 
+HY_PK_MAX_BINS = 1024
+
+
+class hy_pk_options(C.Structure):
+    _fields_ = [("num_bins", C.c_int), ("subtract_shot_noise", C.c_int), ("interlaced", C.c_int),
+                ("max_k_fraction_nyquist", C.c_double)]
+
+
+class hy_pk_bin(C.Structure):
+    _fields_ = [("k_low", C.c_double), ("k_mean", C.c_double), ("k_high", C.c_double),
+                ("p_raw", C.c_double), ("p_corrected", C.c_double),
+                ("mean_cic_power_deconvolution_factor", C.c_double),
+                ("shot_noise_reference", C.c_double), ("mode_count", C.c_uint64)]
+
+
 _lib = None
 
 
@@ -153,6 +168,8 @@ def lib():
     fn("hy_pm_density_vjp", i, vp, u64, vp, vp, vp, d, vp, d, vp, vp, vp)
     fn("hy_ic_gaussian_modes", i, u64, u64, u64, vp, vp, u32, i, vp)
     fn("hy_ic_last_error", C.c_char_p)
+    fn("hy_pm_power_spectrum", i, vp, u64, vp, vp, vp, vp, d, C.POINTER(hy_pk_options),
+       C.POINTER(hy_pk_bin), u32)
     _lib = L
     return L
 

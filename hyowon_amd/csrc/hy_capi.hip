@@ -7,6 +7,7 @@
 #include "hy_host_scalars.hpp"
 #include "hy_fft_table.hpp"
 #include "hy_kernels.hpp"
+#include "hy_pk.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -142,6 +143,11 @@ struct hy_pm_ctx {
     void* adj_phi = nullptr;                         // adjoint workspace (allocated on first use)
     void* adj_a[3] = {nullptr, nullptr, nullptr};
     double* adj_max = nullptr;
+    void* pk_aux = nullptr;      // interlaced partner spectrum (allocated on first use)
+    double* pk_pos = nullptr;    // shifted positions, 3 * cap (allocated on first use)
+    double* pk_tab = nullptr;    // k components, windows, alias factors (3N), shell edges
+    double* pk_part = nullptr;   // workgroup shell partials
+    uint64_t pk_part_cap = 0;    // doubles held by pk_part
 
     // pinned host mirror of the status words
     struct Status {
@@ -217,10 +223,12 @@ void end_call(hy_pm_ctx* c) {
     if (c->prof && c->prof_calls < c->prof_cap) ++c->prof_calls;
 }
 
+const char* const kMsgPositions = "CIC positions must be finite";
+
 // Map the kernel error word onto the reference's exceptions.
 void raise_errors(uint32_t e) {
     using namespace hy;
-    if (e & ERR_POSITION_NONFINITE) fail(HY_EINVAL, "CIC positions must be finite");
+    if (e & ERR_POSITION_NONFINITE) fail(HY_EINVAL, kMsgPositions);
     if (e & ERR_MASS_INVALID) fail(HY_EINVAL, "CIC masses must be finite and non-negative");
     if (e & ERR_STALE_KEYS) fail(HY_ELOGIC, "reused bucket keys do not match the input positions");
     if (e & ERR_COORDINATE) fail(HY_ELOGIC, "CIC bounded coordinate escaped the mesh domain");
@@ -229,6 +237,7 @@ void raise_errors(uint32_t e) {
     if (e & ERR_FORCE_NONFINITE) fail(HY_EOVERFLOW, "CIC centered-gradient gathered acceleration is non-finite");
     if (e & ERR_MOMENTUM_NONFINITE) fail(HY_EOVERFLOW, "Leapfrog kick input is not finite");
     if (e & ERR_UPDATE_NONFINITE) fail(HY_EOVERFLOW, "Leapfrog K1/drift update is not finite and representable");
+    if (e & ERR_SPECTRUM_NONFINITE) fail(HY_ERUNTIME, "FourierDensityBuilder Fourier mode is non-finite");
 }
 
 // Input mass identity (src/gravity/pm_solver.cpp:120-130,665-684).
@@ -545,6 +554,14 @@ void run_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, con
             for (uint32_t b = 0; b < c->nbox; ++b) check_mass(c, Min, c->hmass[b], n / c->nbox);
         }
         return;
+    }
+}
+
+// Device staging of host-pointer calls: positions and masses (4 * cap), accelerations (3 * cap).
+void ensure_host_staging(hy_pm_ctx* c) {
+    if (!c->hpos) {
+        c->hpos = (double*)c->alloc(c->cap * 4 * sizeof(double));
+        c->hacc = (double*)c->alloc(c->cap * 3 * sizeof(double));
     }
 }
 
@@ -909,10 +926,7 @@ int hy_pm_force(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, cons
         const double M = input_mass(c, nper, masses, uniform, host);
         const double *dx_ = x, *dy_ = y, *dz_ = z, *dm = masses;
         if (host) {
-            if (!c->hpos) {
-                c->hpos = (double*)c->alloc(c->cap * 4 * sizeof(double));
-                c->hacc = (double*)c->alloc(c->cap * 3 * sizeof(double));
-            }
+            ensure_host_staging(c);
             double* s = c->hpos;
             hip_check(hipMemcpyAsync(s, x, n * 8, hipMemcpyHostToDevice, c->stream), "h2d");
             hip_check(hipMemcpyAsync(s + c->cap, y, n * 8, hipMemcpyHostToDevice, c->stream), "h2d");
@@ -1324,6 +1338,237 @@ int hy_pm_density_vjp(hy_pm_ctx* c, uint64_t n, const double* x, const double* y
         else gather(float{});
         const auto st = fetch_status(c);
         raise_errors(st.err);
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Matter power spectrum
+// ---------------------------------------------------------------------------
+namespace {
+
+// r2c transform of the mesh in R into dst ([kz][ky][kx] after the forward x pass).
+template <typename T>
+void launch_forward(hy_pm_ctx* c, const hy::FftKernels<T>& K, void* dst) {
+    hy::FftArgs<T> a = fft_args<T>(c);
+    a.B = static_cast<hy::cplx<T>*>(dst);
+    const int NH = c->N / 2 + 1;
+    launch_pass(c, K, a, hy::FFT_Z_R2C, c->fft_var[hy::FFT_Z_R2C], 0, NH);
+    launch_pass(c, K, a, hy::FFT_Y_FWD, c->fft_var[hy::FFT_Y_FWD], 0, NH);
+    const hy::FftPass<T>& P = K.x_fwd;
+    a.kz0 = 0;
+    const uint64_t blocks = (uint64_t)NH * (uint64_t)c->N / (uint64_t)P.rows;
+    hipLaunchKernelGGL(P.fn, dim3((uint32_t)blocks, 1), dim3(P.threads), P.lds, c->stream, a);
+    launch_check("k_fft_x_fwd");
+}
+
+void run_forward(hy_pm_ctx* c, void* dst) {
+    if (c->prec == HY_PREC_FP64) launch_forward<double>(c, c->fft64, dst);
+    else launch_forward<float>(c, c->fft32, dst);
+}
+
+// Workgroups of the shell-binning pass: a function of the mesh and the shell
+// count only, so that the fixed-order sums do not depend on the device.
+struct PkLaunch {
+    int waves, groups;
+    size_t lds;
+};
+
+PkLaunch pk_launch(int N, int nb) {
+    PkLaunch l{};
+    const size_t per_wave = (size_t)nb * hy::PK_Q * sizeof(double);
+    const size_t budget = 64 * 1024 - ((size_t)nb + 1) * sizeof(double);
+    l.waves = (int)std::max<size_t>(1, std::min<size_t>(hy::PK_BLOCK / 64, budget / per_wave));
+    const uint64_t rows = (uint64_t)(N / 2 + 1) * (uint64_t)N;
+    uint64_t g = (rows + l.waves - 1) / l.waves;
+    g = std::min<uint64_t>(g, 2048);
+    g = std::min<uint64_t>(g, std::max<uint64_t>(1, (1u << 20) / (uint64_t)nb));
+    l.groups = (int)g;
+    l.lds = (size_t)l.waves * per_wave + ((size_t)nb + 1) * sizeof(double);
+    return l;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hy_pm_power_spectrum(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z,
+                         const double* masses, double uniform, const hy_pk_options* o, hy_pk_bin* bins,
+                         uint32_t flags) {
+    return guarded([&] {
+        if (!c || !o || !bins) fail(HY_EINVAL, "hy_pm_power_spectrum: null argument");
+        // option checks in the order of src/analysis/matter_power_spectrum.cpp:60-70
+        if (o->num_bins < 1) fail(HY_EINVAL, "MatterPowerSpectrum num_bins must be positive");
+        if (o->num_bins > HY_PK_MAX_BINS) fail(HY_EINVAL, "MatterPowerSpectrum num_bins exceeds HY_PK_MAX_BINS");
+        if ((o->subtract_shot_noise != 0 && o->subtract_shot_noise != 1) || (o->interlaced != 0 && o->interlaced != 1))
+            fail(HY_ELOGIC, "Required scalar coordinate was not specified");
+        const double frac = o->max_k_fraction_nyquist;
+        if (!std::isfinite(frac) || frac <= 0.0 || frac > 1.0)
+            fail(HY_EINVAL, "MatterPowerSpectrum max_k_fraction_nyquist must be in (0,1]");
+        const bool interlaced = o->interlaced == 1, subtract = o->subtract_shot_noise == 1;
+        if (interlaced && subtract)
+            fail(HY_EINVAL,
+                 "Interlaced shot-noise subtraction is disabled until the estimator uses an interlacing-matched "
+                 "alias model");
+        if (n == 0) fail(HY_EINVAL, "FourierDensityBuilder requires owned particles");
+        if (!x || !y || !z) fail(HY_EINVAL, "hy_pm_power_spectrum: null argument");
+        require_single_box(c, "hy_pm_power_spectrum");
+        set_device(c);
+        c->keys_ready = false;
+        require_particles(c, n);
+        set_boxes(c, n);
+        const bool host = (flags & HY_HOST_POINTERS) != 0;
+        const int N = c->N, nb = o->num_bins;
+        const double L = c->geom.L, dx = c->geom.dx;
+
+        // mass moments (src/analysis/fourier_density_field.cpp:160-225)
+        double msf;
+        if (!masses) {
+            if (!std::isfinite(uniform) || uniform <= 0.0)
+                fail(HY_EINVAL, "FourierDensityBuilder uniform mass must be finite and positive");
+            msf = hy::host::mass_square_fraction_uniform(n);
+        } else {
+            std::vector<double> h(n);
+            if (host) std::memcpy(h.data(), masses, n * sizeof(double));
+            else hip_check(hipMemcpyAsync(h.data(), masses, n * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+                           "mass copy");
+            hip_check(hipStreamSynchronize(c->stream), "sync");
+            msf = hy::host::mass_square_fraction(h.data(), n, nullptr);
+        }
+        const double M = input_mass(c, n, masses, uniform, host);
+
+        const double *px = x, *py = y, *pz = z, *pm = masses;
+        if (host) {
+            ensure_host_staging(c);
+            double* s = c->hpos;
+            hip_check(hipMemcpyAsync(s, x, n * 8, hipMemcpyHostToDevice, c->stream), "h2d");
+            hip_check(hipMemcpyAsync(s + c->cap, y, n * 8, hipMemcpyHostToDevice, c->stream), "h2d");
+            hip_check(hipMemcpyAsync(s + 2 * c->cap, z, n * 8, hipMemcpyHostToDevice, c->stream), "h2d");
+            px = s; py = s + c->cap; pz = s + 2 * c->cap;
+            if (masses) {
+                hip_check(hipMemcpyAsync(s + 3 * c->cap, masses, n * 8, hipMemcpyHostToDevice, c->stream), "h2d");
+                pm = s + 3 * c->cap;
+            }
+        }
+        const uint64_t ccount = (uint64_t)N * (uint64_t)(N / 2 + 1) * (uint64_t)N;
+        if (interlaced && !c->pk_aux) {
+            c->pk_aux = c->alloc(ccount * 2 * c->real_bytes);
+            c->pk_pos = static_cast<double*>(c->alloc(c->cap * 3 * sizeof(double)));
+        }
+        auto deposit = [&](const double* dx_, const double* dy_, const double* dz_) {
+            reset_status(c);
+            try {
+                run_deposit(c, n, dx_, dy_, dz_, pm, uniform, M, true);
+            } catch (const HyError& e) {
+                if (e.code == HY_EINVAL && e.what == kMsgPositions)
+                    fail(HY_EINVAL, "FourierDensityBuilder particle positions must be finite");
+                throw;
+            }
+        };
+        // contrast transforms (src/analysis/fourier_density_field.cpp:230-336)
+        deposit(px, py, pz);
+        run_forward(c, c->B);
+        const double shift = 0.5 * dx;
+        if (interlaced) {
+            double* s = c->pk_pos;
+            hipLaunchKernelGGL(hy::k_pk_shift, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, px, py, pz,
+                               shift, L, s, s + c->cap, s + 2 * c->cap);
+            launch_check("k_pk_shift");
+            deposit(s, s + c->cap, s + 2 * c->cap);
+            run_forward(c, c->pk_aux);
+        }
+
+        // shells (src/analysis/matter_power_spectrum.cpp:85-125)
+        const double knyq = std::numbers::pi * static_cast<double>(N) / L;
+        const double kmax = frac * knyq;
+        const double kf = 2.0 * std::numbers::pi / L;
+        if (!(kmax > kf)) fail(HY_EINVAL, "MatterPowerSpectrum requested range contains no nonzero shell");
+        if (!(kf >= 0x1p-450) || !(kmax <= 0x1p450))
+            fail(HY_EINVAL, "MatterPowerSpectrum wavenumbers outside [2^-450, 2^450] are not supported");
+        const std::vector<double> edges = hy::host::log_shell_edges(kf, kmax, nb);
+        const double shot = hy::host::volume_times(L, msf);
+        std::vector<double> tab(3 * (size_t)N + (size_t)nb + 1);
+        for (int i = 0; i < N; ++i) {
+            const int m = i <= N / 2 ? i : i - N;
+            const double k = 2.0 * std::numbers::pi * static_cast<double>(m) / L;
+            const double arg = 0.5 * k * dx;
+            const double s2 = std::sin(arg) * std::sin(arg);
+            tab[i] = k;
+            tab[N + i] = static_cast<double>(hy::host::cic_window_axis(k, dx));
+            tab[2 * N + i] = 1.0 - 2.0 / 3.0 * s2;
+        }
+        std::copy(edges.begin(), edges.end(), tab.begin() + 3 * N);
+        if (!c->pk_tab)
+            c->pk_tab = static_cast<double*>(c->alloc((3 * (size_t)N + HY_PK_MAX_BINS + 1) * sizeof(double)));
+        hip_check(hipMemcpyAsync(c->pk_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream),
+                  "h2d tables");
+        const PkLaunch l = pk_launch(N, nb);
+        const uint64_t slab = (uint64_t)nb * hy::PK_Q;
+        const uint64_t need = ((uint64_t)l.groups + 1) * slab;
+        if (need > c->pk_part_cap) {
+            void* p = nullptr;
+            hip_check(hipMalloc(&p, need * sizeof(double)), "hipMalloc shell partials");
+            c->allocations.push_back(p);
+            c->device_bytes += need * sizeof(double);
+            c->pk_part = static_cast<double*>(p);
+            c->pk_part_cap = need;
+        }
+        hy::PkArgs a{};
+        a.m1 = c->B;
+        a.m2 = interlaced ? c->pk_aux : nullptr;
+        a.kc = c->pk_tab;
+        a.win = c->pk_tab + N;
+        a.ali = c->pk_tab + 2 * N;
+        a.edges = c->pk_tab + 3 * N;
+        a.partials = c->pk_part;
+        a.err = c->err;
+        a.N = N;
+        a.nb = nb;
+        a.waves = l.waves;
+        a.subtract = subtract ? 1 : 0;
+        a.kmax = kmax;
+        a.perp_skip = kmax * (1.0 + 0x1p-40);
+        {
+            // delta_k = FFT(source) L^3 / (M N^3): the density source (m - M/N^3)/dx^3
+            // is the contrast times M / L^3, and the reference scales modes by 1/N^3
+            const long double cells = (long double)N * (long double)N * (long double)N;
+            const long double L3 = (long double)L * (long double)L * (long double)L;
+            a.amp = static_cast<double>(L3 / ((long double)M * cells));
+            a.volume = static_cast<double>(L3);
+        }
+        a.shot = shot;
+        a.shift = shift;
+        double* sums = c->pk_part + (uint64_t)l.groups * slab;
+        if (c->prec == HY_PREC_FP64)
+            hipLaunchKernelGGL(hy::k_pk_bin<double>, dim3(l.groups), dim3(hy::PK_BLOCK), l.lds, c->stream, a);
+        else
+            hipLaunchKernelGGL(hy::k_pk_bin<float>, dim3(l.groups), dim3(hy::PK_BLOCK), l.lds, c->stream, a);
+        launch_check("k_pk_bin");
+        hipLaunchKernelGGL(hy::k_pk_reduce, dim3((uint32_t)blocks_for(slab, 256)), dim3(256), 0, c->stream,
+                           (const double*)c->pk_part, l.groups, (int)slab, sums);
+        launch_check("k_pk_reduce");
+        std::vector<double> h(slab);
+        hip_check(hipMemcpyAsync(h.data(), sums, slab * sizeof(double), hipMemcpyDeviceToHost, c->stream), "d2h");
+        const auto st = fetch_status(c);
+        raise_errors(st.err);
+        // shell means (src/analysis/matter_power_spectrum.cpp:219-240)
+        for (int b = 0; b < nb; ++b) {
+            hy_pk_bin& r = bins[b];
+            r = hy_pk_bin{};
+            r.k_low = edges[b];
+            r.k_high = edges[b + 1];
+            r.shot_noise_reference = shot;
+            const double* q = h.data() + (size_t)b * hy::PK_Q;
+            const uint64_t count = (uint64_t)q[4];
+            if (count == 0) continue;
+            const double cnt = static_cast<double>(count);
+            r.k_mean = q[0] / cnt;
+            r.p_raw = q[1] / cnt;
+            r.p_corrected = q[2] / cnt;
+            r.mean_cic_power_deconvolution_factor = q[3] / cnt;
+            r.mode_count = count;
+        }
     });
 }
 

@@ -170,6 +170,28 @@ __global__ void __launch_bounds__((RowsCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) 
         tid, [&](int l) { return lds[(l / N) * RS + lds_slot(l % N)]; }, [&](int l, C v) { rows[l] = v; });
 }
 
+// x pass, forward only (spectral analysis): in place on B, unnormalised.
+template <typename T, int N, int V>
+__global__ void __launch_bounds__((RowsCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_fft_x_fwd(FftArgs<T> a) {
+    fft_box(a);
+    using C = cplx<T>;
+    constexpr int NH = N / 2 + 1;
+    using RC = RowsCfg<T, N, NH * N, V>;
+    constexpr int ROWS = RC::ROWS, RS = RC::RS, NT = RC::THREADS;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const uint64_t q0 = (uint64_t)a.kz0 * N + (uint64_t)b * ROWS;
+    const int tid = threadIdx.x;
+    C* rows = a.B + q0 * N;
+    staged_copy<ROWS * N, NT, C>(
+        tid, [&](int l) { return rows[l]; }, [&](int l, C v) { lds[(l / N) * RS + lds_slot(l % N)] = v; });
+    __syncthreads();
+    fft_rows<C, N, ROWS, RS, -1>(lds, a.tw, 1, tid);
+    staged_copy<ROWS * N, NT, C>(
+        tid, [&](int l) { return lds[(l / N) * RS + lds_slot(l % N)]; }, [&](int l, C v) { rows[l] = v; });
+}
+
 // y pass backward: B[kz][ky][x] -> A rows over y
 template <typename T, int N, int V>
 __global__ void __launch_bounds__((RowsCfg<T, N, N, V>::THREADS)) k_fft_y_bwd(FftArgs<T> a) {
@@ -423,6 +445,41 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
         dft_reg<A, +1>(v);
 #pragma unroll
         for (int k2 = 0; k2 < A; ++k2) row[t + B * k2] = v[k2];
+    }
+}
+
+// x pass, forward only (spectral analysis): in place on B, unnormalised.
+template <typename T, int N, int V>
+__global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_tl_x_fwd(FftArgs<T> a) {
+    fft_box(a);
+    using C = cplx<T>;
+    constexpr int LEN = N, NH = N / 2 + 1;
+    using Cfg = TLCfg<T, LEN, NH * N, V>;
+    using TL = TwoLevel<LEN>;
+    constexpr int A = TL::A, B = TL::B, XS = TL::XS, ROWS = Cfg::ROWS, TPR = Cfg::TPR;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    const int tid = threadIdx.x, r = tid / TPR, t = tid % TPR;
+    C* xr = lds + r * XS;
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const uint64_t q = (uint64_t)a.kz0 * N + (uint64_t)b * ROWS + r;
+    C* row = a.B + q * N;
+    C v[TL::EMAX];
+    if (t < B) {
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) v[n1] = row[t + B * n1];
+        dft_reg<A, -1>(v);
+        tl_twiddle<A, LEN, -1>(v, t, a.tw, 1);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) xr[k1 * (B + 1) + t] = v[k1];
+    }
+    __syncthreads();
+    if (t < A) {
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) v[n2] = xr[t * (B + 1) + n2];
+        dft_reg<B, -1>(v);  // v[k2] = X[t + A k2]
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) row[t + A * k2] = v[k2];
     }
 }
 

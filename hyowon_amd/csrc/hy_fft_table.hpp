@@ -21,6 +21,7 @@ struct FftPass {
 template <typename T>
 struct FftKernels {
     FftPass<T> pass[FFT_PASSES][kFftVariants];
+    FftPass<T> x_fwd;  // forward-only x pass of the spectral estimators (in place on B)
     int variants = 0;  // launch variants compiled for this size (variant 0 is the default)
 };
 
@@ -48,6 +49,7 @@ void fill_variant(FftKernels<T>& k) {
         k.pass[FFT_X_GREEN][V] = {k_tl_x_green<T, N, V>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
         k.pass[FFT_Y_BWD][V] = {k_tl_y_bwd<T, N, V>, RY::ROWS, RY::THREADS, RY::LDS_BYTES};
         k.pass[FFT_Z_C2R][V] = {k_tl_z_c2r<T, N, V>, RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES};
+        if constexpr (V == 0) k.x_fwd = {k_tl_x_fwd<T, N, V>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
     } else {
         using RZ = RowsCfg<T, N / 2, N, V>;
         using RY = RowsCfg<T, N, N, V>;
@@ -57,6 +59,7 @@ void fill_variant(FftKernels<T>& k) {
         k.pass[FFT_X_GREEN][V] = {k_fft_x_green<T, N, V>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
         k.pass[FFT_Y_BWD][V] = {k_fft_y_bwd<T, N, V>, RY::ROWS, RY::THREADS, RY::LDS_BYTES};
         k.pass[FFT_Z_C2R][V] = {k_fft_z_c2r<T, N, V>, RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES};
+        if constexpr (V == 0) k.x_fwd = {k_fft_x_fwd<T, N, V>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
     }
 }
 

@@ -16,4 +16,12 @@ std::vector<double> time_boundaries(double a_start, double a_final, double delta
                                     std::vector<double> stops);
 double exact_sum(const double* values, uint64_t count);
 
+// Matter power-spectrum estimator scalars (hy_pk_host.cpp).
+std::vector<double> log_shell_edges(double minimum, double maximum, int bins);
+long double cic_window_axis(double k, double dx);
+double mass_square_fraction_uniform(uint64_t n);
+// Validates masses (finite, positive); weight_sum receives sum(m / max m).
+double mass_square_fraction(const double* masses, uint64_t n, double* weight_sum);
+double volume_times(double L, double value);  // L^3 * value in a shared exponent domain
+
 }  // namespace hy::host

@@ -15,7 +15,7 @@ import numpy as np
 
 from ._lib import (FFT_PASS_NAMES, HY_ASYNC, HY_FFT_PASSES, HY_HAS_NEXT, HY_HAS_PREV,
                    HY_HOST_POINTERS, HY_PREC_FP32_MIXED, HY_PREC_FP64, HY_REUSE_KEYS, check,
-                   hy_fft_tuning, hy_pm_desc, hy_pm_info, lib)
+                   hy_fft_tuning, hy_pk_bin, hy_pk_options, hy_pm_desc, hy_pm_info, lib)
 
 PRECISIONS = {"fp64": HY_PREC_FP64, "fp32": HY_PREC_FP32_MIXED, "fp32_mixed": HY_PREC_FP32_MIXED}
 
@@ -302,6 +302,24 @@ class PMSolver:
 
     def check(self) -> None:
         check(lib().hy_pm_check(self._h))
+
+    def power_spectrum(self, x, y, z, num_bins: int, subtract_shot_noise, interlaced,
+                       max_k_fraction_nyquist: float, masses=None, uniform_mass: float = 1.0):
+        """Shell-averaged P(k) of the CIC density contrast on this context's
+        mesh (MatterPowerSpectrum::compute_pk).  subtract_shot_noise and
+        interlaced must be given explicitly (True/False); None is "not
+        specified".  Returns a list of hy_pk_bin."""
+        def explicit(v):
+            return -1 if v is None else int(bool(v))
+        o = hy_pk_options(num_bins=int(num_bins), subtract_shot_noise=explicit(subtract_shot_noise),
+                          interlaced=explicit(interlaced),
+                          max_k_fraction_nyquist=float(max_k_fraction_nyquist))
+        nb = max(int(num_bins), 0)
+        out = (hy_pk_bin * max(nb, 1))()
+        flags = HY_HOST_POINTERS if _is_host(x) else 0
+        check(lib().hy_pm_power_spectrum(self._h, len(x), _ptr(x), _ptr(y), _ptr(z), _ptr(masses),
+                                         float(uniform_mass), C.byref(o), out, flags))
+        return list(out)[:nb]
 
 
 class ParticleState:

@@ -265,6 +265,44 @@ int hy_ic_gaussian_modes(uint64_t N, uint64_t seed, uint64_t max_mode, const dou
                          void* out, uint32_t flags, int device, void* stream);
 const char* hy_ic_last_error(void);
 
+/* Matter power spectrum on the context's mesh (N = the estimator's mesh_size,
+ * L = the periodic domain's box size). */
+#define HY_PK_MAX_BINS 1024
+
+/* include/cosmo_nbody/analysis/matter_power_spectrum.hpp:30-38 */
+typedef struct {
+    int num_bins;                  /* 1..HY_PK_MAX_BINS logarithmic shells in [2 pi/L, k_max] */
+    int subtract_shot_noise;       /* 0 or 1 (core::ExplicitBoolean: any other value is "not
+                                      specified", a logic_error) */
+    int interlaced;                /* 0 or 1, likewise */
+    double max_k_fraction_nyquist; /* (0, 1]: k_max = fraction * pi N / L */
+} hy_pk_options;
+
+/* include/cosmo_nbody/analysis/matter_power_spectrum.hpp:15-28 */
+typedef struct {
+    double k_low;
+    double k_mean;
+    double k_high;
+    double p_raw;
+    double p_corrected;
+    double mean_cic_power_deconvolution_factor;
+    double shot_noise_reference;
+    uint64_t mode_count;
+} hy_pk_bin;
+
+/* Replaces analysis::MatterPowerSpectrum::compute_pk
+ * (include/cosmo_nbody/analysis/matter_power_spectrum.hpp:39-46;
+ * src/analysis/matter_power_spectrum.cpp:57-241) over
+ * FourierDensityBuilder::build (src/analysis/fourier_density_field.cpp:112-343):
+ * CIC deposit and r2c transform on the context's mesh and workspace, then the
+ * shell sums on the device.  masses == NULL selects the uniform mass.  bins
+ * (host memory) receives opts->num_bins shells.  Single-box contexts only.
+ * An interlaced estimate allocates a partner spectrum and staging for the
+ * shifted positions on first use.  Synchronous. */
+int hy_pm_power_spectrum(hy_pm_ctx* ctx, uint64_t n, const double* x, const double* y, const double* z,
+                         const double* masses, double uniform_mass, const hy_pk_options* opts,
+                         hy_pk_bin* bins, uint32_t flags);
+
 /* Host scalars of the step (binary64, long-double quadrature).
  * H(a): src/cosmology/cosmology_model.cpp:25-57,167-178.
  * D = int da/(a^3 H), K = int da/(a^2 H):

@@ -109,6 +109,13 @@ def lib():
         L.fn("or_uniform_pair", None, _u64, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, _pd)
         L.fn("or_normal_pair", None, _u64, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, _pd)
         L.fn("or_gaussian_modes", C.c_int, _u64, _u64, _u64, _pd, C.c_uint32, _pd)
+        L.fn("or_power_spectrum", C.c_int, _u64, _pd, _pd, _pd, _pd, _d, _d, _u64, C.c_int,
+             C.c_int, C.c_int, _d, _pd)
+        L.fn("or_pk_edges", C.c_int, _d, _d, C.c_int, _pd)
+        L.fn("or_pk_bin_index", C.c_int, _pd, C.c_int, _d)
+        L.fn("or_pk_window_axis", C.c_longdouble, _d, _d)
+        L.fn("or_pk_raw_power", _d, _d, _d, _d)
+        L.fn("or_pk_raw_power_minus", _d, _d, _d, _d, _d)
         L.fn("or_set_threads", None, C.c_int)
         L.fn("or_get_threads", C.c_int)
         L.fn("or_gravitational_constant", _d)
@@ -135,6 +142,13 @@ def ref():
         L.fn("ref_cic_coordinate", C.c_int, _d, _d, _u64, _pi64, _pd)
         L.fn("ref_deposit", C.c_int, _u64, _pd, _pd, _pd, _pd, _d, _d, _u64, _pd)
         L.fn("ref_density_contrast", C.c_int, _pd, _u64, _d, _d)
+        L.fn("ref_deposit_unit", C.c_int, _u64, _pd, _pd, _pd, _pd, _d, _d, _d, _u64, _pd)
+        L.fn("ref_pk_edges", C.c_int, _d, _d, C.c_int, _pd)
+        L.fn("ref_pk_bin_index", C.c_int, _d, _d, C.c_int, _d, C.POINTER(C.c_int))
+        L.fn("ref_pk_window_axis", C.c_int, _d, _d, C.POINTER(C.c_longdouble))
+        L.fn("ref_pk_raw_power", C.c_int, _d, _d, _d, _pd)
+        L.fn("ref_pk_raw_power_minus", C.c_int, _d, _d, _d, _d, _pd)
+        L.fn("ref_pk_volume_times", C.c_int, _d, _d, _pd)
         L.fn("ref_density_direct_available", C.c_int, _u64, _d, _d, C.POINTER(C.c_int))
         L.fn("ref_green_apply", C.c_int, _u64, _d, C.c_int, _pd)
         L.fn("ref_gather", C.c_int, _u64, _pd, _pd, _pd, _pd, _d, _u64, _pd, _pd, _pd)
@@ -333,4 +347,26 @@ def gaussian_modes(N, seed, max_mode, variance_by_m2, flags=0):
     out = np.zeros((N, N, N // 2 + 1), dtype=np.complex128)
     _check(lib().or_gaussian_modes(N, seed, max_mode, _ptr(var), flags, out.ctypes.data_as(_pd)),
            "or_gaussian_modes")
+    return out
+
+
+PK_FIELDS = ("k_low", "k_mean", "k_high", "p_raw", "p_corrected",
+             "mean_cic_power_deconvolution_factor", "shot_noise_reference", "mode_count")
+
+
+def power_spectrum(x, y, z, L, N, num_bins, subtract_shot_noise, interlaced, max_fraction,
+                   masses=None, uniform_mass=1.0):
+    """MatterPowerSpectrum::compute_pk restated: (num_bins, 8) float64 rows in PK_FIELDS order."""
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    m = _f64(masses) if masses is not None else None
+    out = np.zeros((num_bins, 8), np.float64)
+    _check(lib().or_power_spectrum(len(x), _ptr(x), _ptr(y), _ptr(z), _ptr(m), uniform_mass, L, N,
+                                   int(num_bins), int(subtract_shot_noise), int(interlaced),
+                                   max_fraction, _ptr(out)), "or_power_spectrum")
+    return out
+
+
+def pk_edges(minimum, maximum, bins):
+    out = np.zeros(bins + 1, np.float64)
+    _check(lib().or_pk_edges(minimum, maximum, bins, _ptr(out)), "or_pk_edges")
     return out

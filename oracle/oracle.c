@@ -1313,3 +1313,430 @@ int or_gaussian_modes(uint64_t N, uint64_t seed, uint64_t max_mode, const double
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Matter power spectrum: MatterPowerSpectrum::compute_pk                     */
+/* (src/analysis/matter_power_spectrum.cpp:57-241) over                       */
+/* FourierDensityBuilder::build (src/analysis/fourier_density_field.cpp:      */
+/* 112-343), with the helpers of analysis/spectral_numeric.hpp and            */
+/* analysis/cic_spectral_window.hpp.                                          */
+/* ------------------------------------------------------------------------- */
+
+/* log(upper/lower) through log1p of the relative gap (spectral_numeric.hpp:28-47) */
+static int pk_log_ratio(double upper, double lower, double* out) {
+    if (!isfinite(upper) || !isfinite(lower) || lower <= 0.0 || upper < lower) return OR_EINVAL;
+    if (upper == lower) { *out = 0.0; return OR_OK; }
+    const double rel = (upper - lower) / lower;
+    const double v = isfinite(rel) ? log1p(rel) : log(upper) - log(lower);
+    if (!isfinite(v) || v <= 0.0) return OR_EOVERFLOW;
+    *out = v;
+    return OR_OK;
+}
+
+/* lower * (upper/lower)^f (spectral_numeric.hpp:49-87) */
+static int pk_log_interp(double lower, double upper, double f, double* out) {
+    if (!isfinite(lower) || !isfinite(upper) || lower <= 0.0 || upper < lower || !isfinite(f) || f < 0.0 ||
+        f > 1.0)
+        return OR_EINVAL;
+    if (f == 0.0 || lower == upper) { *out = lower; return OR_OK; }
+    if (f == 1.0) { *out = upper; return OR_OK; }
+    double span = 0.0;
+    const int e = pk_log_ratio(upper, lower, &span);
+    if (e != OR_OK) return e;
+    double r;
+    if (span <= 1.0) {
+        r = lower * exp(f * span);
+    } else {
+        const long double lv = logl((long double)lower) + (long double)f * (long double)span;
+        const long double w = expl(lv);
+        if (!isfinite(w) || w <= 0.0L || w > (long double)DBL_MAX) return OR_EOVERFLOW;
+        r = (double)w;
+    }
+    if (!isfinite(r) || r <= 0.0 || r < lower || r > upper) return OR_EOVERFLOW;
+    *out = r;
+    return OR_OK;
+}
+
+int or_pk_edges(double minimum, double maximum, int bins, double* edges) {
+    if (!isfinite(minimum) || !isfinite(maximum) || minimum <= 0.0 || maximum <= minimum || bins <= 0)
+        return OR_EINVAL;
+    for (int i = 0; i <= bins; ++i) {
+        const int e = pk_log_interp(minimum, maximum, (double)i / (double)bins, &edges[i]);
+        if (e != OR_OK) return e;
+        if (i != 0 && !(edges[i] > edges[i - 1])) return OR_EOVERFLOW;
+    }
+    return OR_OK;
+}
+
+/* [edge_i, edge_i+1), last bin closed; -1 outside (spectral_numeric.hpp:124-141) */
+int or_pk_bin_index(const double* edges, int bins, double v) {
+    if (v < edges[0] || v > edges[bins]) return -1;
+    if (v == edges[bins]) return bins - 1;
+    int lo = 0, hi = bins + 1; /* first edge > v */
+    while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (edges[mid] > v) hi = mid; else lo = mid + 1;
+    }
+    return lo - 1;
+}
+
+/* sinc^2(k dx / 2) in long double, |sinc| clamped to 1 (cic_spectral_window.hpp:13-37) */
+long double or_pk_window_axis(double k, double dx) {
+    const long double arg = 0.5L * (long double)k * (long double)dx;
+    if (arg == 0.0L) return 1.0L;
+    const long double s = sinl(arg) / arg;
+    long double m = fabsl(s);
+    if (m > 1.0L) m = 1.0L;
+    return m * m;
+}
+
+/* value * L^3 in a shared exponent domain (spectral_numeric.hpp:170-205) */
+static double pk_volume_times(double L, long double value) {
+    if (value == 0.0L) return 0.0;
+    int be = 0, ve = 0, ce = 0;
+    const long double bf = frexpl((long double)L, &be);
+    const long double vf = frexpl(value, &ve);
+    long double c = vf;
+    for (int i = 0; i < 3; ++i) c *= bf;
+    c = frexpl(c, &ce);
+    return (double)scalbnl(c, 3 * be + ve + ce);
+}
+
+/* |v|^2 L^3 (spectral_numeric.hpp:304-321 with :240-271) */
+double or_pk_raw_power(double L, double re, double im) {
+    const double scale = fmax(fabs(re), fabs(im));
+    if (scale == 0.0) return 0.0;
+    const long double rs = (long double)re / (long double)scale;
+    const long double is = (long double)im / (long double)scale;
+    const long double shape = rs * rs + is * is;
+    int be = 0, se = 0, ce = 0;
+    const long double bf = frexpl((long double)L, &be);
+    const long double sf = frexpl((long double)scale, &se);
+    long double c = shape * sf * sf * bf * bf * bf;
+    c = frexpl(c, &ce);
+    return (double)scalbnl(c, se + se + 3 * be + ce);
+}
+
+/* Exact sum of products of binary64 factors, rounded once to nearest-even:
+ * the contract of math/exact_binary64_product_sum.hpp.  Products of up to five
+ * factors are formed exactly as 320-bit integers times a power of two and
+ * added into a two's-complement fixed-point accumulator spanning
+ * [2^-5376, 2^5376). */
+#define PK_ACC_LIMBS 170
+#define PK_ACC_BASE (-5376)
+typedef struct { uint64_t d[PK_ACC_LIMBS]; } pk_bigacc;
+
+static void pk_acc_add_shifted(pk_bigacc* a, const uint64_t* v, int nv, int64_t bit, int negate) {
+    /* a += (negate ? -1 : 1) * v * 2^bit, bit >= 0 relative to the accumulator */
+    uint64_t tmp[PK_ACC_LIMBS];
+    memset(tmp, 0, sizeof(tmp));
+    const int64_t limb = bit / 64, sh = bit % 64;
+    for (int i = 0; i < nv; ++i) {
+        const int64_t j = limb + i;
+        if (j < PK_ACC_LIMBS) tmp[j] |= v[i] << sh;
+        if (sh && j + 1 < PK_ACC_LIMBS) tmp[j + 1] |= v[i] >> (64 - sh);
+    }
+    if (negate) {
+        unsigned __int128 c = 1;
+        for (int i = 0; i < PK_ACC_LIMBS; ++i) {
+            c += (unsigned __int128)(~tmp[i]);
+            tmp[i] = (uint64_t)c;
+            c >>= 64;
+        }
+    }
+    unsigned __int128 c = 0;
+    for (int i = 0; i < PK_ACC_LIMBS; ++i) {
+        c += (unsigned __int128)a->d[i] + tmp[i];
+        a->d[i] = (uint64_t)c;
+        c >>= 64;
+    }
+}
+
+static double pk_exact_product_sum(const double* const* terms, const int* counts, int nterms) {
+    pk_bigacc acc;
+    memset(&acc, 0, sizeof(acc));
+    for (int t = 0; t < nterms; ++t) {
+        uint64_t v[6] = {1, 0, 0, 0, 0, 0};
+        int64_t e = 0;
+        int neg = 0, zero = 0;
+        for (int f = 0; f < counts[t]; ++f) {
+            const double x = terms[t][f];
+            if (x == 0.0) { zero = 1; break; }
+            if (x < 0.0) neg ^= 1;
+            int ex = 0;
+            const double fr = frexp(fabs(x), &ex);
+            const uint64_t m = (uint64_t)ldexp(fr, 53);
+            e += ex - 53;
+            unsigned __int128 c = 0;
+            for (int i = 0; i < 6; ++i) {
+                c += (unsigned __int128)v[i] * m;
+                v[i] = (uint64_t)c;
+                c >>= 64;
+            }
+        }
+        if (zero) continue;
+        pk_acc_add_shifted(&acc, v, 6, e - PK_ACC_BASE, neg);
+    }
+    /* sign and magnitude */
+    const int negative = (acc.d[PK_ACC_LIMBS - 1] >> 63) != 0;
+    if (negative) {
+        unsigned __int128 c = 1;
+        for (int i = 0; i < PK_ACC_LIMBS; ++i) {
+            c += (unsigned __int128)(~acc.d[i]);
+            acc.d[i] = (uint64_t)c;
+            c >>= 64;
+        }
+    }
+    int top = -1;
+    for (int i = PK_ACC_LIMBS - 1; i >= 0 && top < 0; --i)
+        if (acc.d[i]) top = i * 64 + 63 - __builtin_clzll(acc.d[i]);
+    if (top < 0) return 0.0;
+    /* keep 53 significant bits (fewer below the normal range) */
+    const int64_t vexp = (int64_t)top + PK_ACC_BASE; /* value in [2^vexp, 2^(vexp+1)) */
+    int64_t lsb = top - 52;
+    if (vexp - 52 < -1074) lsb = -1074 - PK_ACC_BASE;
+    uint64_t mant = 0;
+    for (int64_t b = top; b >= lsb; --b)
+        mant = (mant << 1) | ((acc.d[b / 64] >> (b % 64)) & 1u);
+    int round = 0, sticky = 0;
+    if (lsb - 1 >= 0) {
+        round = (int)((acc.d[(lsb - 1) / 64] >> ((lsb - 1) % 64)) & 1u);
+        for (int64_t b = lsb - 2; b >= 0 && !sticky; --b)
+            if ((acc.d[b / 64] >> (b % 64)) & 1u) sticky = 1;
+    }
+    if (round && (sticky || (mant & 1u))) mant += 1;
+    const double r = ldexp((double)mant, (int)(lsb + PK_ACC_BASE));
+    return negative ? -r : r;
+}
+
+/* |v|^2 L^3 - s (spectral_numeric.hpp:326-391).  Severe cancellation is
+ * resolved by an exact sum of the binary64 products, rounded once. */
+double or_pk_raw_power_minus(double L, double re, double im, double s) {
+    const double scale = fmax(fabs(re), fabs(im));
+    if (scale == 0.0) return s == 0.0 ? 0.0 : -s;
+    const long double rs = (long double)re / (long double)scale;
+    const long double is = (long double)im / (long double)scale;
+    const long double shape = rs * rs + is * is;
+    int be = 0, se = 0, re_ = 0;
+    const long double bf = frexpl((long double)L, &be);
+    const long double sf = frexpl((long double)scale, &se);
+    long double rc = shape * sf * sf * bf * bf * bf;
+    rc = frexpl(rc, &re_);
+    const int rexp = 2 * se + 3 * be + re_;
+    if (s == 0.0) return (double)scalbnl(rc, rexp);
+    int sexp = 0;
+    const long double sc = frexpl((long double)s, &sexp);
+    const int common = rexp > sexp ? rexp : sexp;
+    const long double ar = scalbnl(rc, rexp - common);
+    const long double as = scalbnl(sc, sexp - common);
+    const long double diff = ar - as;
+    const long double env = 32.0L * LDBL_EPSILON * (fabsl(ar) + fabsl(as));
+    if (fabsl(ar) + fabsl(as) > 0.0L && fabsl(diff) <= env) {
+        const double t1[5] = {re, re, L, L, L}, t2[5] = {im, im, L, L, L}, t3[1] = {-s};
+        const double* terms[3] = {t1, t2, t3};
+        const int counts[3] = {5, 5, 1};
+        return pk_exact_product_sum(terms, counts, 3);
+    }
+    return (double)scalbnl(diff, common);
+}
+
+typedef struct { long double k, raw, corr, dec; uint64_t count; } pk_acc;
+
+/* Neumaier-compensated non-negative sum (fourier_density_field.cpp:27-59) */
+static void pk_comp_add(long double v, long double* sum, long double* comp) {
+    if (v == 0.0L) return;
+    const long double u = *sum + v;
+    *comp += fabsl(*sum) >= fabsl(v) ? (*sum - u) + v : (v - u) + *sum;
+    *sum = u;
+}
+
+/* One contrast transform: deposit, density / mean - 1, r2c, times 1/N^3
+ * (fourier_density_field.cpp:230-295). */
+static int pk_build_one(uint64_t n, const double* x, const double* y, const double* z, const double* w,
+                        double L, uint64_t N, double mean, double* mesh, double* modes) {
+    int e = or_deposit(n, x, y, z, w, 1.0, L, N, mesh);
+    if (e != OR_OK) return e;
+    const uint64_t cells = N * N * N;
+    for (uint64_t i = 0; i < cells; ++i) {
+        const double c = mesh[i] / mean - 1.0;
+        if (!isfinite(c)) return OR_ERUNTIME;
+        mesh[i] = c;
+    }
+    e = or_fft_r2c_3d(N, mesh, modes);
+    if (e != OR_OK) return e;
+    const double norm = 1.0 / (double)cells;
+    const uint64_t m = N * N * (N / 2 + 1) * 2;
+    for (uint64_t i = 0; i < m; ++i) modes[i] *= norm;
+    return OR_OK;
+}
+
+int or_power_spectrum(uint64_t n, const double* x, const double* y, const double* z,
+                      const double* masses, double uniform_mass, double L, uint64_t N, int num_bins,
+                      int subtract_shot_noise, int interlaced, double max_fraction, double* out) {
+    if (num_bins < 1) return OR_EINVAL;
+    if (!isfinite(max_fraction) || max_fraction <= 0.0 || max_fraction > 1.0) return OR_EINVAL;
+    if (interlaced && subtract_shot_noise) return OR_EINVAL;
+    if (N < 2 || n == 0) return OR_EINVAL;
+    for (uint64_t p = 0; p < n; ++p)
+        if (!isfinite(x[p]) || !isfinite(y[p]) || !isfinite(z[p])) return OR_EINVAL;
+
+    /* mass moments in units of one positive mass (fourier_density_field.cpp:160-225) */
+    double wsum, msf;
+    double* w = NULL;
+    if (!masses) {
+        if (!isfinite(uniform_mass) || uniform_mass <= 0.0) return OR_EINVAL;
+        wsum = (double)n;
+        msf = (double)(1.0L / (long double)n);
+    } else {
+        double unit = 0.0;
+        for (uint64_t p = 0; p < n; ++p) {
+            if (!isfinite(masses[p]) || masses[p] <= 0.0) return OR_EINVAL;
+            unit = fmax(unit, masses[p]);
+        }
+        w = (double*)malloc(n * sizeof(double));
+        if (!w) return OR_ERUNTIME;
+        long double s1 = 0.0L, c1 = 0.0L, s2 = 0.0L, c2 = 0.0L;
+        for (uint64_t p = 0; p < n; ++p) {
+            w[p] = masses[p] / unit;
+            const long double v = (long double)w[p];
+            pk_comp_add(v, &s1, &c1);
+            pk_comp_add(v * v, &s2, &c2);
+        }
+        const long double sw = s1 + c1, sw2 = s2 + c2;
+        wsum = (double)sw;
+        const double r = (double)(sw2 / (sw * sw));
+        if (r > 1.0 + 64.0 * DBL_EPSILON) { free(w); return OR_ELOGIC; }
+        msf = fmin(1.0, r);
+    }
+    const uint64_t NH = N / 2 + 1, cells = N * N * N, cm = N * N * NH;
+    const double mean = wsum / (double)cells;
+    double* mesh = (double*)malloc(cells * sizeof(double));
+    double* modes = (double*)malloc(cm * 2 * sizeof(double));
+    double* m2 = interlaced ? (double*)malloc(cm * 2 * sizeof(double)) : NULL;
+    double* sx = interlaced ? (double*)malloc(3 * n * sizeof(double)) : NULL;
+    double* edges = (double*)malloc((size_t)(num_bins + 1) * sizeof(double));
+    pk_acc* slabs = (pk_acc*)calloc(N * (uint64_t)num_bins, sizeof(pk_acc));
+    int err = OR_OK;
+    if (!mesh || !modes || !edges || !slabs || (interlaced && (!m2 || !sx))) { err = OR_ERUNTIME; goto done; }
+    err = pk_build_one(n, x, y, z, w, L, N, mean, mesh, modes);
+    if (err != OR_OK) goto done;
+    const double dx = L / (double)N;
+    if (interlaced) {
+        /* a second deposit shifted by -dx/2 and a phase-corrected average
+         * (fourier_density_field.cpp:300-336) */
+        const double shift = 0.5 * dx;
+        for (uint64_t p = 0; p < n; ++p) {
+            sx[p] = or_wrap(x[p] - shift, L);
+            sx[n + p] = or_wrap(y[p] - shift, L);
+            sx[2 * n + p] = or_wrap(z[p] - shift, L);
+        }
+        err = pk_build_one(n, sx, sx + n, sx + 2 * n, w, L, N, mean, mesh, m2);
+        if (err != OR_OK) goto done;
+        for (uint64_t ix = 0; ix < N; ++ix) {
+            const double kx = 2.0 * OR_PI * (double)mode_number(ix, N) / L;
+            for (uint64_t iy = 0; iy < N; ++iy) {
+                const double ky = 2.0 * OR_PI * (double)mode_number(iy, N) / L;
+                for (uint64_t iz = 0; iz < NH; ++iz) {
+                    const double kz = 2.0 * OR_PI * (double)mode_number(iz, N) / L;
+                    const uint64_t i = 2 * (iz + NH * (iy + N * ix));
+                    const double ph = -(kx + ky + kz) * shift;
+                    const double c = cos(ph), s = sin(ph);
+                    const double pr = m2[i] * c - m2[i + 1] * s;
+                    const double pi = m2[i] * s + m2[i + 1] * c;
+                    modes[i] = 0.5 * (modes[i] + pr);
+                    modes[i + 1] = 0.5 * (modes[i + 1] + pi);
+                }
+            }
+        }
+    }
+    modes[0] = 0.0;
+    modes[1] = 0.0;
+
+    {
+        const double knyq = OR_PI * (double)N / L;
+        const double kmax = max_fraction * knyq;
+        const double kf = 2.0 * OR_PI / L;
+        if (!(kmax > kf)) { err = OR_EINVAL; goto done; }
+        err = or_pk_edges(kf, kmax, num_bins, edges);
+        if (err != OR_OK) goto done;
+        const double shot = pk_volume_times(L, (long double)msf);
+        const int even = (N % 2) == 0;
+        int failed = 0;
+#pragma omp parallel for schedule(static) num_threads(OR_NT) reduction(|: failed)
+        for (int64_t ix = 0; ix < (int64_t)N; ++ix) {
+            pk_acc* local = slabs + (uint64_t)ix * (uint64_t)num_bins;
+            const double kx = 2.0 * OR_PI * (double)mode_number((uint64_t)ix, N) / L;
+            const double sxa = 0.5 * kx * dx;
+            const long double cx = or_pk_window_axis(kx, dx);
+            for (uint64_t iy = 0; iy < N; ++iy) {
+                const double ky = 2.0 * OR_PI * (double)mode_number(iy, N) / L;
+                const double sya = 0.5 * ky * dx;
+                const long double cy = or_pk_window_axis(ky, dx);
+                for (uint64_t iz = 0; iz < NH; ++iz) {
+                    const double kz = 2.0 * OR_PI * (double)mode_number(iz, N) / L;
+                    const double sza = 0.5 * kz * dx;
+                    const long double cz = or_pk_window_axis(kz, dx);
+                    const double kmag = hypot(hypot(kx, ky), kz);
+                    if (!(kmag > 0.0) || kmag > kmax) continue;
+                    const uint64_t i = 2 * (iz + NH * (iy + N * (uint64_t)ix));
+                    const double praw = or_pk_raw_power(L, modes[i], modes[i + 1]);
+                    const long double win = cx * cy * cz;
+                    if (!(win > 0.0L) || win > 1.0L) { failed = 1; continue; }
+                    const long double decw = 1.0L / (win * win);
+                    const double dec = (double)decw;
+                    const double s2x = sin(sxa) * sin(sxa);
+                    const double s2y = sin(sya) * sin(sya);
+                    const double s2z = sin(sza) * sin(sza);
+                    const double alias = shot * (1.0 - 2.0 / 3.0 * s2x) * (1.0 - 2.0 / 3.0 * s2y) *
+                                         (1.0 - 2.0 / 3.0 * s2z);
+                    double pc = praw;
+                    if (subtract_shot_noise) pc = or_pk_raw_power_minus(L, modes[i], modes[i + 1], alias);
+                    pc *= dec;
+                    if (!isfinite(pc)) { failed = 1; continue; }
+                    const uint64_t mult = (iz == 0 || (even && iz == N / 2)) ? 1u : 2u;
+                    const int b = or_pk_bin_index(edges, num_bins, kmag);
+                    if (b < 0) continue;
+                    const long double lm = (long double)mult;
+                    local[b].k += (long double)kmag * lm;
+                    local[b].raw += (long double)praw * lm;
+                    local[b].corr += (long double)pc * lm;
+                    local[b].dec += (long double)dec * lm;
+                    local[b].count += mult;
+                }
+            }
+        }
+        if (failed) { err = OR_EOVERFLOW; goto done; }
+        for (int b = 0; b < num_bins; ++b) {
+            pk_acc t = {0.0L, 0.0L, 0.0L, 0.0L, 0};
+            for (uint64_t ix = 0; ix < N; ++ix) {
+                const pk_acc* s = slabs + ix * (uint64_t)num_bins + b;
+                t.k += s->k;
+                t.raw += s->raw;
+                t.corr += s->corr;
+                t.dec += s->dec;
+                t.count += s->count;
+            }
+            double* o = out + 8 * b;
+            o[0] = edges[b];
+            o[2] = edges[b + 1];
+            o[6] = shot;
+            o[1] = o[3] = o[4] = o[5] = o[7] = 0.0;
+            if (t.count == 0) continue;
+            const long double cnt = (long double)t.count;
+            o[1] = (double)(t.k / cnt);
+            o[3] = (double)(t.raw / cnt);
+            o[4] = (double)(t.corr / cnt);
+            o[5] = (double)(t.dec / cnt);
+            o[7] = (double)t.count;
+        }
+    }
+done:
+    free(w);
+    free(mesh);
+    free(modes);
+    free(m2);
+    free(sx);
+    free(edges);
+    free(slabs);
+    return err;
+}

@@ -190,6 +190,23 @@ void or_normal_pair(uint64_t seed, int32_t kx, int32_t ky, int32_t kz, uint32_t 
 int or_gaussian_modes(uint64_t N, uint64_t seed, uint64_t max_mode, const double* variance_by_m2,
                       uint32_t flags, double* out);
 
+/* Matter power spectrum (src/analysis/matter_power_spectrum.cpp:57-241 over
+ * src/analysis/fourier_density_field.cpp:112-343).  masses == NULL selects the
+ * uniform mass.  out: num_bins rows of 8 doubles (k_low, k_mean, k_high, p_raw,
+ * p_corrected, mean CIC power deconvolution factor, shot-noise reference,
+ * mode count).  Helpers: the logarithmic bin edges and their inclusive lookup
+ * (analysis/spectral_numeric.hpp:93-141), the per-axis CIC window
+ * (analysis/cic_spectral_window.hpp:13-37), |v|^2 L^3 and |v|^2 L^3 - s
+ * (spectral_numeric.hpp:304-391). */
+int or_power_spectrum(uint64_t n, const double* x, const double* y, const double* z,
+                      const double* masses, double uniform_mass, double L, uint64_t N, int num_bins,
+                      int subtract_shot_noise, int interlaced, double max_fraction, double* out);
+int or_pk_edges(double minimum, double maximum, int bins, double* edges);
+int or_pk_bin_index(const double* edges, int bins, double v);
+long double or_pk_window_axis(double k, double dx);
+double or_pk_raw_power(double L, double re, double im);
+double or_pk_raw_power_minus(double L, double re, double im, double s);
+
 /* Thread count used by OpenMP regions (0 = runtime default). */
 void or_set_threads(int threads);
 int or_get_threads(void);

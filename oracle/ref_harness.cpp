@@ -11,6 +11,8 @@
 // 4 underflow_error, 6 runtime_error, 7 other.
 #include "cic_coordinate.hpp"
 
+#include "cosmo_nbody/analysis/cic_spectral_window.hpp"
+#include "cosmo_nbody/analysis/spectral_numeric.hpp"
 #include "cosmo_nbody/core/particle_store.hpp"
 #include "cosmo_nbody/cosmology/cosmology_model.hpp"
 #include "cosmo_nbody/cosmology/drift_kick_factors.hpp"
@@ -122,6 +124,65 @@ int ref_deposit(uint64_t n, const double* x, const double* y, const double* z,
         cic.deposit(std::span<const double>(x, n), std::span<const double>(y, n),
                     std::span<const double>(z, n), m, uniform, field);
         std::memcpy(mesh, field.data(), geom.real_size() * sizeof(double));
+    });
+}
+
+// Deposit in units of a positive mass (src/mesh/mass_assignment_deposit.inc:1-60),
+// as FourierDensityBuilder::build uses it (src/analysis/fourier_density_field.cpp:227-232).
+int ref_deposit_unit(uint64_t n, const double* x, const double* y, const double* z,
+                     const double* masses, double uniform_mass, double unit, double L, uint64_t N,
+                     double* mesh) {
+    return guarded([&] {
+        const cn::mesh::MeshGeometry geom(L, N);
+        cn::mesh::CICMassAssignment cic(geom, {}, unit);
+        cn::mesh::RealField field(geom.padded_real_size());
+        std::optional<double> uniform;
+        std::span<const double> m;
+        if (masses) m = std::span<const double>(masses, n);
+        else uniform = uniform_mass;
+        cic.deposit(std::span<const double>(x, n), std::span<const double>(y, n),
+                    std::span<const double>(z, n), m, uniform, field);
+        std::memcpy(mesh, field.data(), geom.real_size() * sizeof(double));
+    });
+}
+
+// Header-only estimator helpers (analysis/spectral_numeric.hpp,
+// analysis/cic_spectral_window.hpp).
+int ref_pk_edges(double minimum, double maximum, int bins, double* edges) {
+    return guarded([&] {
+        const cn::analysis::detail::PositiveLogBinGrid grid(minimum, maximum, bins);
+        for (std::size_t i = 0; i <= grid.bin_count(); ++i) edges[i] = grid.edge(i);
+    });
+}
+
+int ref_pk_bin_index(double minimum, double maximum, int bins, double value, int* out) {
+    return guarded([&] {
+        const cn::analysis::detail::PositiveLogBinGrid grid(minimum, maximum, bins);
+        *out = grid.inclusive_bin_index(value);
+    });
+}
+
+int ref_pk_window_axis(double k, double dx, long double* out) {
+    return guarded([&] { *out = cn::analysis::detail::cic_window_axis_wide(k, dx); });
+}
+
+int ref_pk_raw_power(double L, double re, double im, double* out) {
+    return guarded([&] {
+        *out = cn::analysis::detail::scaled_box_volume_times_complex_norm_squared(
+            L, std::complex<double>(re, im), "raw power");
+    });
+}
+
+int ref_pk_raw_power_minus(double L, double re, double im, double s, double* out) {
+    return guarded([&] {
+        *out = cn::analysis::detail::scaled_box_volume_times_complex_norm_squared_minus(
+            L, std::complex<double>(re, im), s, "raw power minus");
+    });
+}
+
+int ref_pk_volume_times(double L, double value, double* out) {
+    return guarded([&] {
+        *out = cn::analysis::detail::scaled_box_volume_times(L, static_cast<long double>(value), "volume");
     });
 }
 

@@ -109,7 +109,7 @@ def run_gpu(args):
         torch.mul(src, 1.0, out=dst)  # elementwise kernel: reads and writes calibrate_copy bytes
         torch.cuda.synchronize()
         del src, dst
-    pos, mom = ic.lpt_ics(n_side, L, a0, seed=1, device=dev)
+    pos, mom = ic.lpt_ics(n_side, L, a0, seed=1, device=dev, rng="philox", max_mode=n_side // 2 - 1)
     results = {}
     for prec in args.precisions:
         mdt = torch.float32 if prec == "fp32" else torch.float64
@@ -347,6 +347,66 @@ def run_ensemble(args):
     }
 
 
+def run_pk(args):
+    """Power-spectrum workload: the config-2 particle load (512^3 1LPT
+    particles at z = 49, seed 1, L = 512 Mpc/h) on a 1024^3 CIC mesh, 32
+    logarithmic shells up to 0.5 k_Nyq, interlaced.  Times hy_pm_power_spectrum
+    (deposits, r2c transforms, shell sums) per precision and reports the shells,
+    the FP32-vs-FP64 shell differences and the ratio to the linear input
+    spectrum D(a)^2 P_lin(k)."""
+    import numpy as np
+    import torch
+    import hyowon_amd as H
+    from hyowon_amd import ic
+    n_side, N, L = args.n_side, args.mesh, args.box
+    n = n_side ** 3
+    a0 = 1.0 / (1.0 + args.z_start)
+    dev = torch.device("cuda:0")
+    pos, mom = ic.lpt_ics(n_side, L, a0, seed=1, device=dev, rng="philox", max_mode=n_side // 2 - 1)
+    del mom
+    m = ic.particle_mass(n_side, L)
+    nb, frac = 32, 0.5
+    res = {}
+    for prec in args.precisions:
+        s = H.PMSolver(L, N, precision=prec, max_particles=n)
+        reps = args.steps if prec == args.precisions[0] else max(2, args.fp64_steps)
+        for _ in range(max(1, args.warmup)):
+            s.power_spectrum(*pos, nb, False, True, frac, uniform_mass=m)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            bins = s.power_spectrum(*pos, nb, False, True, frac, uniform_mass=m)
+        torch.cuda.synchronize()
+        res[prec] = {"ms": (time.perf_counter() - t0) * 1e3 / reps, "reps": reps,
+                     "bins": [[b.k_mean, b.p_corrected, b.p_raw, int(b.mode_count)] for b in bins]}
+        print(f"pk {prec}: {res[prec]['ms']:.2f} ms per estimate", file=sys.stderr, flush=True)
+        s.close()
+        torch.cuda.empty_cache()
+    cosmo = ic.Cosmology()
+    d1, _ = cosmo.growth(a0)
+    main_prec = args.precisions[0]
+    rows = res[main_prec]["bins"]
+    ratio = [b[1] / (d1 * d1 * float(cosmo.power(np.array([b[0]]))[0])) if b[3] else None for b in rows]
+    out = {
+        "metric": "matter power-spectrum estimates/s (512^3 particles, 1024^3 mesh, interlaced)",
+        "value": 1e3 / res[main_prec]["ms"], "unit": "estimates/s", "n_gpus": 1,
+        "steps": res[main_prec]["reps"], "warmup": args.warmup, "ms_per_step": res[main_prec]["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 mesh/FFT, f64 shell sums" if main_prec == "fp32" else "f64",
+        "data": "synthetic 1LPT ICs (Philox mode-keyed field, BBKS, sigma8=0.8, z=49, seed 1, K=n/2-1)",
+        "config": {"workload": "MatterPowerSpectrum::compute_pk on the config-2 particle load: "
+                               "512^3 particles, 1024^3 mesh, L=512, 32 shells to 0.5 k_Nyq, interlaced",
+                   "particles": n, "mesh": N},
+        "shells": {"k_mean": [b[0] for b in rows], "p_corrected": [b[1] for b in rows],
+                   "mode_count": [b[3] for b in rows], "ratio_to_linear": ratio},
+    }
+    if "fp64" in res and main_prec != "fp64":
+        r64 = res["fp64"]["bins"]
+        rel = [abs(a[1] / b[1] - 1.0) for a, b in zip(rows, r64) if b[3] and b[1] != 0.0]
+        out["fp64"] = {"ms_per_estimate": res["fp64"]["ms"], "max_rel_shell_diff_fp32_vs_fp64": max(rel)}
+    return out
+
+
 def run_adjoint(args):
     """Config-4 workload: 128^3 particles from a 128^3 white-noise field (seed
     7), 256^3 mesh, L = 256 Mpc/h, 16 KDK steps z = 19 -> 0; loss = mean
@@ -480,7 +540,7 @@ def main():
                     help="FP32-vs-FP64 force, final-position and P(k) errors (full run)")
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
                     help="measure the FFT launch plan before timing (untimed)")
-    ap.add_argument("--workload", default="config2", choices=["config2", "ensemble", "adjoint"],
+    ap.add_argument("--workload", default="config2", choices=["config2", "ensemble", "adjoint", "pk"],
                     help="config2: the 512^3 / 1024^3 KDK step (default); ensemble: the 192-box "
                          "graph-launched ensemble step; adjoint: the differentiable 128^3 / 256^3 run")
     ap.add_argument("--fft-plan", default="",
@@ -503,6 +563,9 @@ def main():
         return
     if args.workload == "adjoint":
         print(json.dumps(run_adjoint(args)), flush=True)
+        return
+    if args.workload == "pk":
+        print(json.dumps(run_pk(args)), flush=True)
         return
     n, N, res = run_gpu(args)
     main_prec = args.precisions[0]
@@ -528,7 +591,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 mesh/FFT/forces/momenta, f64 positions" if main_prec == "fp32" else "f64",
-        "data": "synthetic 1LPT ICs (BBKS, sigma8=0.8, z=49, seed 1)",
+        "data": "synthetic 1LPT ICs (Philox mode-keyed field, BBKS, sigma8=0.8, z=49, seed 1, K=n/2-1)",
         "config": {"workload": "PM KDK step, config 2: 512^3 particles, 1024^3 CIC mesh, "
                                "L=512 Mpc/h, 64 log-a steps z=49->0",
                    "particles": n, "mesh": N, "box": args.box, "precision": main_prec},

@@ -1528,6 +1528,7 @@ int hy_pm_power_spectrum(hy_pm_ctx* c, uint64_t n, const double* x, const double
         a.waves = l.waves;
         a.subtract = subtract ? 1 : 0;
         a.kmax = kmax;
+        a.kf = kf;
         a.perp_skip = kmax * (1.0 + 0x1p-40);
         {
             // delta_k = FFT(source) L^3 / (M N^3): the density source (m - M/N^3)/dx^3
@@ -1545,7 +1546,7 @@ int hy_pm_power_spectrum(hy_pm_ctx* c, uint64_t n, const double* x, const double
         else
             hipLaunchKernelGGL(hy::k_pk_bin<float>, dim3(l.groups), dim3(hy::PK_BLOCK), l.lds, c->stream, a);
         launch_check("k_pk_bin");
-        hipLaunchKernelGGL(hy::k_pk_reduce, dim3((uint32_t)blocks_for(slab, 256)), dim3(256), 0, c->stream,
+        hipLaunchKernelGGL(hy::k_pk_reduce, dim3((uint32_t)slab), dim3(256), 0, c->stream,
                            (const double*)c->pk_part, l.groups, (int)slab, sums);
         launch_check("k_pk_reduce");
         std::vector<double> h(slab);

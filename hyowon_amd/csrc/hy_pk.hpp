@@ -72,6 +72,7 @@ struct PkArgs {
     int waves;            // waves per workgroup owning an LDS accumulator each
     int subtract;         // subtract the aliased shot noise
     double kmax;          // largest binned |k|
+    double kf;            // fundamental 2 pi / L
     double perp_skip;     // rows with hypot(ky, kz) above this hold no binned mode
     double amp;           // transform value -> delta_k (L^3 / (M N^3) for a density-source mesh)
     double volume;        // L^3
@@ -103,11 +104,18 @@ __global__ void __launch_bounds__(PK_BLOCK) k_pk_bin(PkArgs a) {
             const double ky = a.kc[iy], kz = a.kc[iz];
             if (hypot_corrected(ky, kz) > a.perp_skip) continue;  // uniform over the wave
             const double mult = (iz == 0 || (even && iz == N / 2)) ? 1.0 : 2.0;
-            for (int x0 = 0; x0 < N; x0 += 64) {
-                const int ix = x0 + lane;
+            // |m_x| <= sqrt(kmax^2 - k_perp^2) / k_f (+1 for rounding): the
+            // x indices [0, mxm] and [N - mxm, N) hold every binned mode of the row
+            const double rem = a.kmax * a.kmax - (ky * ky + kz * kz);
+            const int mxm = min(N / 2, (int)(sqrt(fmax(rem, 0.0)) / a.kf) + 1);
+            const int lo2 = max(N - mxm, N / 2 + 1);
+            const int span1 = mxm + 1, span = span1 + (N - lo2);
+            for (int c0 = 0; c0 < span; c0 += 64) {
+                const int c = c0 + lane;
+                const int ix = c < span1 ? c : lo2 + (c - span1);
                 int shell = -1;
                 double v[PK_Q];
-                if (ix < N) {
+                if (c < span) {
                     const double kx = a.kc[ix];
                     const double kmag = hypot_corrected(hypot_corrected(kx, ky), kz);
                     if (kmag > 0.0 && !(kmag > a.kmax)) shell = shell_index(edges, a.nb, kmag);
@@ -175,14 +183,21 @@ __global__ void __launch_bounds__(PK_BLOCK) k_pk_bin(PkArgs a) {
     }
 }
 
-// Fixed-order sum of the workgroup partials: out[i] = sum_g partials[g][i].
+// Fixed-order sum of the workgroup partials, one workgroup per output:
+// out[i] = sum_g partials[g][i] (strided per-thread sums, then a fixed tree).
 __global__ void __launch_bounds__(256) k_pk_reduce(const double* __restrict__ partials, int groups, int slab,
                                                    double* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= slab) return;
+    __shared__ double red[256];
+    const int i = blockIdx.x;
     double s = 0.0;
-    for (int g = 0; g < groups; ++g) s += partials[(size_t)g * slab + i];
-    out[i] = s;
+    for (int g = threadIdx.x; g < groups; g += blockDim.x) s += partials[(size_t)g * slab + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[i] = red[0];
 }
 
 // Interlacing partner positions: wrap(x - shift, L)

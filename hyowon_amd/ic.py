@@ -188,18 +188,20 @@ def lpt_from_delta_k(delta_k, box: float, a_start: float, cosmo: Cosmology | Non
 
 
 def lpt_ics(n_side: int, box: float, a_start: float, seed: int = 1,
-            cosmo: Cosmology | None = None, device=None, rng: str = "torch"):
+            cosmo: Cosmology | None = None, device=None, rng: str = "torch",
+            max_mode: int | None = None):
     """Return (pos, mom) as lists of three tensors (float64) of n_side^3 particles.
 
     rng="torch": seeded torch white noise (fast synthetic ICs);
     rng="philox": the reference's mode-keyed Philox Gaussian field, drawn on
-    the GPU by hy_ic_gaussian_modes."""
+    the GPU by hy_ic_gaussian_modes, with the Cartesian band limit max_mode
+    (default n_side // 2)."""
     import torch  # data generation only
 
     device = device or ("cuda" if torch.cuda.is_available() else "cpu")
     if rng == "philox":
         dev = torch.device(device)
-        dk = philox_modes(n_side, box, a_start, seed, cosmo, device=dev.index or 0)
+        dk = philox_modes(n_side, box, a_start, seed, cosmo, max_mode=max_mode, device=dev.index or 0)
         return lpt_from_delta_k(dk, box, a_start, cosmo)
     noise = white_noise(n_side, seed, device)
     return lpt_from_noise(noise, box, a_start, cosmo)

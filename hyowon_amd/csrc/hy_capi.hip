@@ -479,7 +479,11 @@ template <typename T, typename PT>
 void launch_gather(hy_pm_ctx* c, hy::GatherArgs<T, PT> a) {
     a.lat = c->nbox == 1 ? lattice_tiling(a.n, 256) : hy::LatticeTiling{0, 0, 0, 0, -1, -1, -1, -1};
     a.dfac = c->dfac;
-    hipLaunchKernelGGL((hy::k_gather_kdk<T, PT>), dim3(blocks_for(a.n, 256)), dim3(256), 0, c->stream, a);
+    const bool o32 = c->nbox == 1 && (uint64_t)c->N * c->N * c->N * sizeof(T) <= (1ull << 32);
+    if (o32)
+        hipLaunchKernelGGL((hy::k_gather_kdk<T, PT, true>), dim3(blocks_for(a.n, 256)), dim3(256), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL((hy::k_gather_kdk<T, PT, false>), dim3(blocks_for(a.n, 256)), dim3(256), 0, c->stream, a);
     launch_check("k_gather_kdk");
 }
 

@@ -866,7 +866,10 @@ struct GatherArgs {
     uint32_t* err;
 };
 
-template <typename T, typename PT>
+// O32: one box whose mesh spans at most 2^32 bytes; the stencil then uses
+// 32-bit byte offsets from the mesh base (one scalar base, no per-lane 64-bit
+// address arithmetic).
+template <typename T, typename PT, bool O32 = false>
 __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
     // each XCD walks a contiguous particle range (spatially compact for
     // lattice-ordered particles), so potential lines are reused in its L2
@@ -901,31 +904,63 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
             // y / z stencil ends Y[j][a][c] at (ix+a, iy-1 / iy+2, iz+c) and
             // Z[j][a][b] at (ix+a, iy+b, iz-1 / iz+2).  All 32 loads are
             // issued before any arithmetic.
-            const uint64_t NN = (uint64_t)N * N;
-            const T* phi = a.phi + (uint64_t)box_of(p, g) * NN * N;
-            uint64_t xo[4], yo[4], zo[4];
+            using Off = typename std::conditional<O32, uint32_t, uint64_t>::type;
+            const Off NN = (Off)N * (Off)N;
+            const Off es = O32 ? (Off)sizeof(T) : (Off)1;  // offsets in bytes (O32) or elements
+            const T* phi = O32 ? a.phi : a.phi + (uint64_t)box_of(p, g) * (uint64_t)NN * (uint64_t)N;
+            auto ld = [&](Off o) -> T {
+                if constexpr (O32) return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(phi) + o);
+                else return phi[o];
+            };
+            Off xo[4], yo[4], zo[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                xo[k] = (uint64_t)wrap_cell(ix - 1 + k, N) * NN;
-                yo[k] = (uint64_t)wrap_cell(iy - 1 + k, N) * (uint64_t)N;
-                zo[k] = (uint64_t)wrap_cell(iz - 1 + k, N);
+                xo[k] = (Off)wrap_cell(ix - 1 + k, N) * NN * es;
+                yo[k] = (Off)wrap_cell(iy - 1 + k, N) * (Off)N * es;
+                zo[k] = (Off)wrap_cell(iz - 1 + k, N) * es;
             }
             T X[4][2][2], Y[2][2][2], Z[2][2][2];
+            // The stencil as 12 z-contiguous runs: the (ix+a, iy+b) lines hold
+            // iz-1..iz+2 (x and z stencils), the x-end and y-end lines
+            // iz..iz+1.  A run that does not wrap in z is one vector load.
+            typedef T V4 __attribute__((ext_vector_type(4), aligned(sizeof(T))));
+            typedef T V2 __attribute__((ext_vector_type(2), aligned(sizeof(T))));
+            auto run = [&](Off xy, int z0, T* dst, int cnt) {
+                if (z0 >= 0 && z0 + cnt <= N) {
+                    const Off o = xy + (Off)z0 * es;
+                    const char* src = O32 ? reinterpret_cast<const char*>(phi) + o
+                                          : reinterpret_cast<const char*>(phi + o);
+                    if (cnt == 4) {
+                        const V4 v = *reinterpret_cast<const V4*>(src);
+                        dst[0] = v.x; dst[1] = v.y; dst[2] = v.z; dst[3] = v.w;
+                    } else {
+                        const V2 v = *reinterpret_cast<const V2*>(src);
+                        dst[0] = v.x; dst[1] = v.y;
+                    }
+                } else {
+                    for (int c = 0; c < cnt; ++c) dst[c] = ld(xy + (Off)wrap_cell(z0 + c, N) * es);
+                }
+            };
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
+            for (int u = 0; u < 2; ++u)
 #pragma unroll
-                for (int b = 0; b < 2; ++b)
+                for (int b = 0; b < 2; ++b) {
+                    T q[4];
+                    run(xo[u + 1] + yo[b + 1], iz - 1, q, 4);
+                    Z[0][u][b] = q[0];
+                    X[u + 1][b][0] = q[1];
+                    X[u + 1][b][1] = q[2];
+                    Z[1][u][b] = q[3];
+                }
 #pragma unroll
-                    for (int c = 0; c < 2; ++c) X[k][b][c] = phi[xo[k] + yo[b + 1] + zo[c + 1]];
+            for (int b = 0; b < 2; ++b) {
+                run(xo[0] + yo[b + 1], iz, X[0][b], 2);
+                run(xo[3] + yo[b + 1], iz, X[3][b], 2);
+            }
 #pragma unroll
             for (int j = 0; j < 2; ++j)
 #pragma unroll
-                for (int u = 0; u < 2; ++u)
-#pragma unroll
-                    for (int v = 0; v < 2; ++v) {
-                        Y[j][u][v] = phi[xo[u + 1] + yo[j * 3] + zo[v + 1]];
-                        Z[j][u][v] = phi[xo[u + 1] + yo[v + 1] + zo[j * 3]];
-                    }
+                for (int u = 0; u < 2; ++u) run(xo[u + 1] + yo[j * 3], iz, Y[j][u], 2);
             // -(f+ - f-) / (2 dx); when dx is a power of two the quotient equals
             // the product with the exact reciprocal bit for bit
             const T two_dx = (T)(2.0 * g.dx);

@@ -499,28 +499,30 @@ __device__ __forceinline__ void load_source_ranges(const uint32_t* __restrict__ 
                                                    int tz, int nT, uint32_t* rs, uint32_t* rc,
                                                    uint8_t* rd, uint8_t* rcls) {
     const int t = threadIdx.x;
-    if (t < 20) {
-        const RangeSlot sl = kRangeSlots[t];
-        const int d = sl.d;
-        const int sx = (tx - ((d >> 2) & 1) + nT) % nT, sy = (ty - ((d >> 1) & 1) + nT) % nT,
-                  sz = (tz - (d & 1) + nT) % nT;
-        const uint32_t t8 = (((uint32_t)sx * nT + sy) * nT + sz) * 8u;
-        const uint32_t lo = d == 0 ? t8 : t8 + sl.c;
-        const uint32_t hi = d == 0 ? t8 + 8 : t8 + sl.c + 1;
-        const uint32_t a = offsets[lo], b = offsets[hi];
-        rs[t] = a;
-        rc[t + 1] = b - a;  // count, prefix-summed below
-        rd[t] = (uint8_t)d;
-        rcls[t] = sl.c;
-    }
-    __syncthreads();
-    if (t == 0) {
-        uint32_t run = 0;
-        rc[0] = 0;
-        for (int r = 0; r < 20; ++r) {
-            run += rc[r + 1];
-            rc[r + 1] = run;
+    if (t < 64) {  // the first wave: one lane per range, then a shuffle scan of the counts
+        uint32_t cnt = 0;
+        if (t < 20) {
+            const RangeSlot sl = kRangeSlots[t];
+            const int d = sl.d;
+            const int sx = (tx - ((d >> 2) & 1) + nT) % nT, sy = (ty - ((d >> 1) & 1) + nT) % nT,
+                      sz = (tz - (d & 1) + nT) % nT;
+            const uint32_t t8 = (((uint32_t)sx * nT + sy) * nT + sz) * 8u;
+            const uint32_t lo = d == 0 ? t8 : t8 + sl.c;
+            const uint32_t hi = d == 0 ? t8 + 8 : t8 + sl.c + 1;
+            const uint32_t a = offsets[lo], b = offsets[hi];
+            rs[t] = a;
+            cnt = b - a;
+            rd[t] = (uint8_t)d;
+            rcls[t] = sl.c;
         }
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o);
+            if (t >= o) incl += y;
+        }
+        if (t < 20) rc[t + 1] = incl;
+        if (t == 0) rc[0] = 0;
     }
     __syncthreads();
 }
@@ -676,42 +678,62 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
     const uint32_t M = sh_rc[DEP_NRANGES];
     int r = 0;
     bool bad = false;
-    for (uint32_t v = threadIdx.x; v < M; v += blockDim.x) {
-        while (sh_rc[r + 1] <= v) ++r;
-        const uint32_t slot = sh_rs[r] + (v - sh_rc[r]);
-        const DepRecord rec = Q.rec[slot];
-        const int d = sh_rd[r];
-        const int c = (int)rec.cell;
-        const int cz_ = c % T, cy_ = (c / T) % T, cx_ = c / (T * T);
-        // a neighbour record sits just below this tile along its shifted axes
-        const int lx = (d & 4) ? -1 : cx_;
-        const int ly = (d & 2) ? -1 : cy_;
-        const int lz = (d & 1) ? -1 : cz_;
-        if (((d & 4) && cx_ != T - 1) || ((d & 2) && cy_ != T - 1) || ((d & 1) && cz_ != T - 1)) {
-            bad = true;
-            continue;
+    // records in pairs: both loads are issued before the first atomic, so a
+    // thread with two records waits for one memory round trip, not two
+    for (uint32_t v0 = threadIdx.x; v0 < M; v0 += 2 * blockDim.x) {
+        DepRecord recs[2];
+        float ms[2];
+        int ds[2];
+        bool have[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t v = v0 + (uint32_t)h * blockDim.x;
+            have[h] = v < M;
+            if (have[h]) {
+                while (sh_rc[r + 1] <= v) ++r;
+                const uint32_t slot = sh_rs[r] + (v - sh_rc[r]);
+                recs[h] = Q.rec[slot];
+                ms[h] = Q.rec_mass ? Q.rec_mass[slot] : 1.0f;
+                ds[h] = sh_rd[r];
+            }
         }
-        const float m = Q.rec_mass ? Q.rec_mass[slot] : 1.0f;
-        const float wx[2] = {1.0f - rec.fx, rec.fx};
-        const float wy[2] = {1.0f - rec.fy, rec.fy};
-        const float wz[2] = {1.0f - rec.fz, rec.fz};
 #pragma unroll
-        for (int ox = 0; ox < 2; ++ox) {
-            const int cx = lx + ox;
-            if (cx < 0 || cx >= T) continue;
+        for (int h = 0; h < 2; ++h) {
+            if (!have[h]) continue;
+            const DepRecord rec = recs[h];
+            const int d = ds[h];
+            const int c = (int)rec.cell;
+            const int cz_ = c % T, cy_ = (c / T) % T, cx_ = c / (T * T);
+            // a neighbour record sits just below this tile along its shifted axes
+            const int lx = (d & 4) ? -1 : cx_;
+            const int ly = (d & 2) ? -1 : cy_;
+            const int lz = (d & 1) ? -1 : cz_;
+            if (((d & 4) && cx_ != T - 1) || ((d & 2) && cy_ != T - 1) || ((d & 1) && cz_ != T - 1)) {
+                bad = true;
+                continue;
+            }
+            const float m = ms[h];
+            const float wx[2] = {1.0f - rec.fx, rec.fx};
+            const float wy[2] = {1.0f - rec.fy, rec.fy};
+            const float wz[2] = {1.0f - rec.fz, rec.fz};
 #pragma unroll
-            for (int oy = 0; oy < 2; ++oy) {
-                const int cy = ly + oy;
-                if (cy < 0 || cy >= T) continue;
-                const float mxy = m * wx[ox] * wy[oy];
+            for (int ox = 0; ox < 2; ++ox) {
+                const int cx = lx + ox;
+                if (cx < 0 || cx >= T) continue;
 #pragma unroll
-                for (int oz = 0; oz < 2; ++oz) {
-                    const int cz = lz + oz;
-                    if (cz < 0 || cz >= T) continue;
-                    const float w = mxy * wz[oz];
-                    // signed 32.32 fixed point (two's complement sums wrap consistently)
-                    const unsigned long long q = (unsigned long long)__float2ll_rn(w * 4294967296.0f);
-                    atomicAdd(&acc[(cx * T + cy) * T + cz], q);
+                for (int oy = 0; oy < 2; ++oy) {
+                    const int cy = ly + oy;
+                    if (cy < 0 || cy >= T) continue;
+                    const float mxy = m * wx[ox] * wy[oy];
+#pragma unroll
+                    for (int oz = 0; oz < 2; ++oz) {
+                        const int cz = lz + oz;
+                        if (cz < 0 || cz >= T) continue;
+                        const float w = mxy * wz[oz];
+                        // signed 32.32 fixed point (two's complement sums wrap consistently)
+                        const unsigned long long q = (unsigned long long)__float2ll_rn(w * 4294967296.0f);
+                        atomicAdd(&acc[(cx * T + cy) * T + cz], q);
+                    }
                 }
             }
         }

@@ -24,7 +24,8 @@ EXPORTS = [
     "hy_gravitational_constant", "hy_rho_crit0", "hy_pm_profile", "hy_pm_profile_read",
     "hy_pm_get_tuning", "hy_pm_set_tuning", "hy_pm_autotune", "hy_ensemble_create", "hy_ensemble_step",
     "hy_ensemble_check", "hy_ensemble_destroy", "hy_pm_kdk_vjp", "hy_pm_density_vjp",
-    "hy_ic_gaussian_modes", "hy_ic_last_error", "hy_pm_power_spectrum",
+    "hy_ic_gaussian_modes", "hy_ic_last_error", "hy_pm_power_spectrum", "hy_pm_force_meshes",
+    "hy_pm_interpolate",
 ]
 
 HY_FFT_PASSES = 5
@@ -76,7 +77,8 @@ _BY_CODE = {1: HyInvalidArgument, 2: HyLogicError, 3: HyOverflowError, 4: HyUnde
 class hy_pm_desc(C.Structure):
     _fields_ = [("box_size", C.c_double), ("mesh_size", C.c_uint64), ("deconvolve_cic", C.c_int),
                 ("precision", C.c_int), ("device", C.c_int), ("stream", C.c_void_p),
-                ("max_particles", C.c_uint64), ("boxes", C.c_uint32)]
+                ("max_particles", C.c_uint64), ("boxes", C.c_uint32), ("method", C.c_int),
+                ("split_scale", C.c_double)]
 
 
 class hy_pm_info(C.Structure):
@@ -100,6 +102,8 @@ class hy_ensemble_member(C.Structure):
 
 
 HY_PK_MAX_BINS = 1024
+HY_METHOD_PURE_PM = 0
+HY_METHOD_TREEPM_LONG_RANGE = 1
 
 
 class hy_pk_options(C.Structure):
@@ -170,6 +174,8 @@ def lib():
     fn("hy_ic_last_error", C.c_char_p)
     fn("hy_pm_power_spectrum", i, vp, u64, vp, vp, vp, vp, d, C.POINTER(hy_pk_options),
        C.POINTER(hy_pk_bin), u32)
+    fn("hy_pm_force_meshes", i, vp, vp, vp, vp, vp)
+    fn("hy_pm_interpolate", i, vp, u64, vp, vp, vp, vp, vp)
     _lib = L
     return L
 

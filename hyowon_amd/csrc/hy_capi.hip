@@ -143,6 +143,13 @@ struct hy_pm_ctx {
     void* adj_phi = nullptr;                         // adjoint workspace (allocated on first use)
     void* adj_a[3] = {nullptr, nullptr, nullptr};
     double* adj_max = nullptr;
+    int method = HY_METHOD_PURE_PM;
+    double split_scale = 0.0;
+    void* B2 = nullptr;          // TreePM long range: x-derivative spectrum
+    void* R2 = nullptr;          // TreePM long range: y and z force meshes (R holds x)
+    void* R3 = nullptr;
+    void* kc = nullptr;          // k component per index
+    void* kd = nullptr;          // k component per index, 0 on the Nyquist index
     void* pk_aux = nullptr;      // interlaced partner spectrum (allocated on first use)
     double* pk_pos = nullptr;    // shifted positions, 3 * cap (allocated on first use)
     double* pk_tab = nullptr;    // k components, windows, alias factors (3N), shell edges
@@ -238,6 +245,8 @@ void raise_errors(uint32_t e) {
     if (e & ERR_MOMENTUM_NONFINITE) fail(HY_EOVERFLOW, "Leapfrog kick input is not finite");
     if (e & ERR_UPDATE_NONFINITE) fail(HY_EOVERFLOW, "Leapfrog K1/drift update is not finite and representable");
     if (e & ERR_SPECTRUM_NONFINITE) fail(HY_ERUNTIME, "FourierDensityBuilder Fourier mode is non-finite");
+    if (e & ERR_INTERP_UNDERFLOW)
+        fail(HY_EUNDERFLOW, "CIC interpolation lost a non-zero CIC weight or field-weight product in core::Real");
 }
 
 // Input mass identity (src/gravity/pm_solver.cpp:120-130,665-684).
@@ -445,6 +454,74 @@ void launch_fft(hy_pm_ctx* c, const hy::FftKernels<T>& K) {
 void run_fft(hy_pm_ctx* c) {
     if (c->prec == HY_PREC_FP64) launch_fft<double>(c, c->fft64);
     else launch_fft<float>(c, c->fft32);
+}
+
+// TreePM long-range force meshes from the density source in R: r2c, y, then
+// the spectral-gradient passes (src/gravity/pm_solver.cpp:768-810):
+//   x: B2 = X^-1(-i k_x G phi), B = X^-1(G phi)
+//   F_x = Z^-1 Y^-1 B2 -> R,  F_y = Z^-1 Y^-1(-i k_y B) -> R2,  F_z = Z^-1(-i k_z Y^-1 B) -> R3
+template <typename T>
+void launch_fft_sg(hy_pm_ctx* c, const hy::FftKernels<T>& K) {
+    using C = hy::cplx<T>;
+    hy::FftArgs<T> a = fft_args<T>(c);
+    a.kc = static_cast<const T*>(c->kc);
+    a.kd = static_cast<const T*>(c->kd);
+    a.rs = c->split_scale;
+    const uint64_t N = (uint64_t)c->N, NH = N / 2 + 1;
+    launch_pass(c, K, a, hy::FFT_Z_R2C, c->fft_var[hy::FFT_Z_R2C], 0, (int)NH);
+    launch_pass(c, K, a, hy::FFT_Y_FWD, c->fft_var[hy::FFT_Y_FWD], 0, (int)NH);
+    auto sg = [&](int which, const hy::FftArgs<T>& aa) {
+        const hy::FftPass<T>& P = K.sg[which];
+        const uint64_t blocks = which == 0 ? NH * N / (uint64_t)P.rows
+                                           : (which == 1 ? NH * (N / (uint64_t)P.rows) : N * (N / (uint64_t)P.rows));
+        hipLaunchKernelGGL(P.fn, dim3((uint32_t)blocks, 1), dim3(P.threads), P.lds, c->stream, aa);
+        launch_check(which == 0 ? "k_fft_x_green (spectral gradient)"
+                                : (which == 1 ? "k_fft_y_bwd (spectral gradient)" : "k_fft_z_c2r (spectral gradient)"));
+    };
+    a.kz0 = 0;
+    hy::FftArgs<T> b = a;
+    b.deriv = 1;
+    b.xout = static_cast<C*>(c->B2);
+    sg(0, b);
+    sg(0, a);
+    b = a;
+    b.B = static_cast<C*>(c->B2);
+    sg(1, b);
+    sg(2, b);  // F_x -> R
+    b = a;
+    b.deriv = 2;
+    sg(1, b);
+    b.deriv = 0;
+    b.R = static_cast<T*>(c->R2);
+    sg(2, b);  // F_y -> R2
+    b = a;
+    sg(1, b);
+    b.deriv = 3;
+    b.R = static_cast<T*>(c->R3);
+    sg(2, b);  // F_z -> R3
+}
+
+void run_fft_sg(hy_pm_ctx* c) {
+    if (c->prec == HY_PREC_FP64) launch_fft_sg<double>(c, c->fft64);
+    else launch_fft_sg<float>(c, c->fft32);
+}
+
+// CIC interpolation of `meshes` meshes (single box): out (written) or add (added).
+template <typename T>
+void launch_interp(hy_pm_ctx* c, hy::InterpArgs<T> a) {
+    a.g = c->geom;
+    a.err = c->err;
+    const bool o32 = (uint64_t)c->N * c->N * c->N * sizeof(T) <= (1ull << 32);
+    const dim3 grid((uint32_t)blocks_for(a.n, 256));
+    if (o32) hipLaunchKernelGGL((hy::k_interp<T, true>), grid, dim3(256), 0, c->stream, a);
+    else hipLaunchKernelGGL((hy::k_interp<T, false>), grid, dim3(256), 0, c->stream, a);
+    launch_check("k_interp");
+}
+
+void require_pure_pm(const hy_pm_ctx* c, const char* what) {
+    if (c->method != HY_METHOD_PURE_PM)
+        fail(HY_EINVAL, std::string(what) + " takes a pure-PM context (the TreePM long-range force needs the "
+                                            "short-range tree force beside it; use hy_pm_force)");
 }
 
 // Lattice block schedule for n = S^3 particles when a 256-particle block
@@ -688,6 +765,17 @@ int hy_pm_create(const hy_pm_desc* desc, hy_pm_ctx** out) {
         c->geom.upper = std::nextafter((double)N, 0.0);
         c->nbox = desc->boxes == 0 ? 1u : desc->boxes;
         if (c->nbox > 65535) fail(HY_EINVAL, "box count exceeds 65535");
+        // PMForceMethod named constructors (include/cosmo_nbody/mesh/pm_force_method.hpp:20-34)
+        if (desc->method == HY_METHOD_TREEPM_LONG_RANGE) {
+            if (!std::isfinite(desc->split_scale) || desc->split_scale <= 0.0)
+                fail(HY_EINVAL, "TreePM long-range PM method requires a finite positive split scale");
+            if (c->nbox != 1) fail(HY_EINVAL, "TreePM long-range contexts take a single box");
+            c->desc.deconvolve_cic = 1;
+        } else if (desc->method != HY_METHOD_PURE_PM) {
+            fail(HY_EINVAL, "unknown PM method");
+        }
+        c->method = desc->method;
+        c->split_scale = desc->split_scale;
         c->geom.nbox = c->nbox;
         c->geom.nper = 0;
         c->geom.N = N;
@@ -757,6 +845,24 @@ int hy_pm_create(const hy_pm_desc* desc, hy_pm_ctx** out) {
         upload(c->tw, tw);
         upload(c->lap, lap);
         upload(c->win, win);
+        if (c->method == HY_METHOD_TREEPM_LONG_RANGE) {
+            // spectral-gradient workspace and k tables (k_component of
+            // include/cosmo_nbody/mesh/mesh_geometry.hpp:112-115; the derivative
+            // axis drops its Nyquist index, src/gravity/pm_solver.cpp:781-787)
+            c->B2 = c->alloc(ccount * 2 * rb);
+            c->R2 = c->alloc(cells * rb);
+            c->R3 = c->alloc(cells * rb);
+            c->kc = c->alloc((size_t)N * rb);
+            c->kd = c->alloc((size_t)N * rb);
+            std::vector<double> kc(N), kd(N);
+            for (int m = 0; m < N; ++m) {
+                const int64_t mode = m <= N / 2 ? m : (int64_t)m - N;
+                kc[m] = 2.0 * std::numbers::pi * (double)mode / L;
+                kd[m] = (N % 2 == 0 && m == N / 2) ? 0.0 : kc[m];
+            }
+            upload(c->kc, kc);
+            upload(c->kd, kd);
+        }
 
         const uint64_t cap = std::max<uint64_t>(c->cap, 1);
         c->keys = (uint32_t*)c->alloc(cap * 4);
@@ -858,12 +964,57 @@ int hy_pm_potential(hy_pm_ctx* c, const void* source_in, void* phi_out) {
     return guarded([&] {
         if (!c || !source_in || !phi_out) fail(HY_EINVAL, "hy_pm_potential: null argument");
         require_single_box(c, "hy_pm_potential");
+        require_pure_pm(c, "hy_pm_potential");
         set_device(c);
         const size_t bytes = (size_t)c->N * c->N * c->N * c->real_bytes;
         reset_status(c);
         hip_check(hipMemcpyAsync(c->R, source_in, bytes, hipMemcpyDeviceToDevice, c->stream), "copy source");
         run_fft(c);
         hip_check(hipMemcpyAsync(phi_out, c->R, bytes, hipMemcpyDeviceToDevice, c->stream), "copy phi");
+        const auto st = fetch_status(c);
+        raise_errors(st.err);
+    });
+}
+
+int hy_pm_force_meshes(hy_pm_ctx* c, const void* source_in, void* fx_out, void* fy_out, void* fz_out) {
+    return guarded([&] {
+        if (!c || !source_in || !fx_out || !fy_out || !fz_out) fail(HY_EINVAL, "hy_pm_force_meshes: null argument");
+        if (c->method != HY_METHOD_TREEPM_LONG_RANGE)
+            fail(HY_EINVAL, "hy_pm_force_meshes takes a TreePM long-range context");
+        set_device(c);
+        const size_t bytes = (size_t)c->N * c->N * c->N * c->real_bytes;
+        reset_status(c);
+        hip_check(hipMemcpyAsync(c->R, source_in, bytes, hipMemcpyDeviceToDevice, c->stream), "copy source");
+        run_fft_sg(c);
+        hip_check(hipMemcpyAsync(fx_out, c->R, bytes, hipMemcpyDeviceToDevice, c->stream), "copy F_x");
+        hip_check(hipMemcpyAsync(fy_out, c->R2, bytes, hipMemcpyDeviceToDevice, c->stream), "copy F_y");
+        hip_check(hipMemcpyAsync(fz_out, c->R3, bytes, hipMemcpyDeviceToDevice, c->stream), "copy F_z");
+        const auto st = fetch_status(c);
+        raise_errors(st.err);
+    });
+}
+
+int hy_pm_interpolate(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z,
+                      const void* mesh, double* out) {
+    return guarded([&] {
+        if (!c || !x || !y || !z || !mesh || !out) fail(HY_EINVAL, "hy_pm_interpolate: null argument");
+        require_single_box(c, "hy_pm_interpolate");
+        set_device(c);
+        reset_status(c);
+        if (n > 0) {
+            auto interp = [&](auto tag) {
+                using T = decltype(tag);
+                hy::InterpArgs<T> a{};
+                a.n = n;
+                a.x = x; a.y = y; a.z = z;
+                a.f[0] = static_cast<const T*>(mesh);
+                a.add[0] = out;
+                a.meshes = 1;
+                launch_interp(c, a);
+            };
+            if (c->prec == HY_PREC_FP64) interp(double{});
+            else interp(float{});
+        }
         const auto st = fetch_status(c);
         raise_errors(st.err);
     });
@@ -944,24 +1095,44 @@ int hy_pm_force(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, cons
         reset_status(c);
         if (dm) hipLaunchKernelGGL(hy::k_check_mass, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, dm, c->err);
         run_deposit(c, n, dx_, dy_, dz_, dm, uniform, M, true);
-        run_fft(c);
         auto fill = [&](auto& a) {
             a.g = c->geom; a.n = n;
             a.x_in = dx_; a.y_in = dy_; a.z_in = dz_;
             a.err = c->err;
         };
-        if (c->prec == HY_PREC_FP64) {
-            hy::GatherArgs<double, double> a{};
-            fill(a);
-            a.phi = static_cast<const double*>(c->R);
-            a.g_x = (double*)c->gx; a.g_y = (double*)c->gy; a.g_z = (double*)c->gz;
-            launch_gather(c, a);
+        if (c->method == HY_METHOD_TREEPM_LONG_RANGE) {
+            run_fft_sg(c);
+            auto interp = [&](auto tag) {
+                using T = decltype(tag);
+                hy::InterpArgs<T> a{};
+                a.n = n;
+                a.x = dx_; a.y = dy_; a.z = dz_;
+                a.f[0] = static_cast<const T*>(c->R);
+                a.f[1] = static_cast<const T*>(c->R2);
+                a.f[2] = static_cast<const T*>(c->R3);
+                a.out[0] = static_cast<T*>(c->gx);
+                a.out[1] = static_cast<T*>(c->gy);
+                a.out[2] = static_cast<T*>(c->gz);
+                a.meshes = 3;
+                launch_interp(c, a);
+            };
+            if (c->prec == HY_PREC_FP64) interp(double{});
+            else interp(float{});
         } else {
-            hy::GatherArgs<float, float> a{};
-            fill(a);
-            a.phi = static_cast<const float*>(c->R);
-            a.g_x = (float*)c->gx; a.g_y = (float*)c->gy; a.g_z = (float*)c->gz;
-            launch_gather(c, a);
+            run_fft(c);
+            if (c->prec == HY_PREC_FP64) {
+                hy::GatherArgs<double, double> a{};
+                fill(a);
+                a.phi = static_cast<const double*>(c->R);
+                a.g_x = (double*)c->gx; a.g_y = (double*)c->gy; a.g_z = (double*)c->gz;
+                launch_gather(c, a);
+            } else {
+                hy::GatherArgs<float, float> a{};
+                fill(a);
+                a.phi = static_cast<const float*>(c->R);
+                a.g_x = (float*)c->gx; a.g_y = (float*)c->gy; a.g_z = (float*)c->gz;
+                launch_gather(c, a);
+            }
         }
         mark(c, HY_STAGE_GATHER_KDK + 1);
         end_call(c);
@@ -998,6 +1169,7 @@ int hy_pm_kdk(hy_pm_ctx* c, uint64_t n, const double* x_in, const double* y_in, 
               double* acc_y, double* acc_z, uint32_t flags) {
     return guarded([&] {
         if (!c) fail(HY_EINVAL, "hy_pm_kdk: null context");
+        require_pure_pm(c, "hy_pm_kdk");
         if (n == 0) fail(HY_ERUNTIME, "PMSolver exact input mass identity must be finite and positive");
         if (!x_in || !y_in || !z_in) fail(HY_EINVAL, "hy_pm_kdk: null position");
         const bool has_prev = (flags & HY_HAS_PREV) != 0, has_next = (flags & HY_HAS_NEXT) != 0;
@@ -1254,6 +1426,7 @@ int hy_pm_kdk_vjp(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, co
         if (!c || !x || !y || !z || !lx || !ly || !lz || !lpx || !lpy || !lpz)
             fail(HY_EINVAL, "hy_pm_kdk_vjp: null argument");
         require_single_box(c, "hy_pm_kdk_vjp");
+        require_pure_pm(c, "hy_pm_kdk_vjp");
         if (n == 0) fail(HY_ERUNTIME, "PMSolver exact input mass identity must be finite and positive");
         const bool has_prev = (flags & HY_HAS_PREV) != 0, has_next = (flags & HY_HAS_NEXT) != 0;
         set_device(c);
@@ -1606,6 +1779,7 @@ int hy_ensemble_create(const hy_ensemble_member* members, uint32_t count, hy_ens
             const hy_ensemble_member& m = members[i];
             if (!m.ctx || !m.x || !m.y || !m.z || !m.px || !m.py || !m.pz || m.n == 0)
                 fail(HY_EINVAL, "hy_ensemble_create: incomplete member");
+            require_pure_pm(m.ctx, "hy_ensemble_create");
             if (i == 0) {
                 e->stream = m.ctx->stream;
                 e->device = m.ctx->desc.device;

@@ -29,6 +29,7 @@ enum : uint32_t {
     ERR_MOMENTUM_NONFINITE = 1u << 7,  // overflow_error: kick input not finite
     ERR_STALE_KEYS = 1u << 8,          // reused bucket keys do not match the positions
     ERR_SPECTRUM_NONFINITE = 1u << 9,  // runtime_error: a Fourier mode or shell power is not finite
+    ERR_INTERP_UNDERFLOW = 1u << 10,   // underflow_error: a CIC interpolation product rounded to zero
 };
 
 struct Geom {

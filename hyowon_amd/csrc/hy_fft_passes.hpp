@@ -1,6 +1,9 @@
 // The five pencil passes of the PM Poisson solve (gfx950): r2c along z,
 // forward y, forward x + Green + backward x, backward y, c2r along z.  Each is
 // templated on the mesh size N and a launch variant V (rows per workgroup).
+// The x, y^-1 and z^-1 passes also come as SG = 1 kernels for the TreePM
+// long-range force: continuum Green with the Gaussian split and a spectral
+// -i k derivative applied in the pass of its axis.
 #pragma once
 
 #include "hy_device.hpp"
@@ -29,7 +32,19 @@ struct FftArgs {
     uint64_t box_real;   // elements per box of R (N^3) and of A, B (N^2 (N/2+1))
     uint64_t box_cplx;
     uint32_t* err;
+    // spectral-gradient passes (TreePM long range), read only by the SG = 1 kernels
+    const T* kc;         // k component per index
+    const T* kd;         // k component per index, 0 on the Nyquist index (derivative axis)
+    cplx<T>* xout;       // x pass output (nullptr: in place on B)
+    double rs;           // Gaussian split scale r_s
+    int deriv;           // 0, or 1 / 2 / 3: multiply by -i k_x / k_y / k_z in the x / y^-1 / z^-1 pass
 };
+
+// -i k v
+template <typename C, typename T>
+__device__ __forceinline__ C mul_mi(C v, T k) {
+    return C{v.y * k, -v.x * k};
+}
 
 // Boxes of one geometry are laid out box-major; grid row blockIdx.y is the box.
 template <typename T>
@@ -122,7 +137,8 @@ __global__ void __launch_bounds__((RowsCfg<T, N, N, V>::THREADS)) k_fft_y_fwd(Ff
 }
 
 // x pass: forward x FFT, Green multiply (with 1/N^3), backward x FFT, in place on B.
-template <typename T, int N, int V>
+// SG = 1: TreePM Green, optional -i k_x, output to xout when given.
+template <typename T, int N, int V, int SG = 0>
 __global__ void __launch_bounds__((RowsCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_fft_x_green(FftArgs<T> a) {
     fft_box(a);
     using C = cplx<T>;
@@ -148,7 +164,9 @@ __global__ void __launch_bounds__((RowsCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) 
         const uint64_t q = q0 + r;
         const int kz = (int)(q / N), ky = (int)(q % N);
         T f;
-        if (kx == 0 && ky == 0 && kz == 0) {
+        if constexpr (SG) {
+            f = green_factor_treepm(a, kx, ky, kz);
+        } else if (kx == 0 && ky == 0 && kz == 0) {
             f = (T)0;
         } else {
             const T D = a.green_lap[kx] + a.green_lap[ky] + a.green_lap[kz];
@@ -162,12 +180,14 @@ __global__ void __launch_bounds__((RowsCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) 
         C v = lds[r * RS + lds_slot(kx)];
         v.x *= f;
         v.y *= f;
+        if (SG && a.deriv == 1) v = mul_mi(v, a.kd[kx]);
         lds[r * RS + lds_slot(kx)] = v;
     }
     __syncthreads();
     fft_rows<C, N, ROWS, RS, +1>(lds, a.tw, 1, tid);
+    C* orow = (SG && a.xout) ? a.xout + q0 * N : rows;
     staged_copy<ROWS * N, NT, C>(
-        tid, [&](int l) { return lds[(l / N) * RS + lds_slot(l % N)]; }, [&](int l, C v) { rows[l] = v; });
+        tid, [&](int l) { return lds[(l / N) * RS + lds_slot(l % N)]; }, [&](int l, C v) { orow[l] = v; });
 }
 
 // x pass, forward only (spectral analysis): in place on B, unnormalised.
@@ -192,8 +212,8 @@ __global__ void __launch_bounds__((RowsCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) 
         tid, [&](int l) { return lds[(l / N) * RS + lds_slot(l % N)]; }, [&](int l, C v) { rows[l] = v; });
 }
 
-// y pass backward: B[kz][ky][x] -> A rows over y
-template <typename T, int N, int V>
+// y pass backward: B[kz][ky][x] -> A rows over y (SG = 1: optional -i k_y on load)
+template <typename T, int N, int V, int SG = 0>
 __global__ void __launch_bounds__((RowsCfg<T, N, N, V>::THREADS)) k_fft_y_bwd(FftArgs<T> a) {
     fft_box(a);
     using C = cplx<T>;
@@ -208,7 +228,12 @@ __global__ void __launch_bounds__((RowsCfg<T, N, N, V>::THREADS)) k_fft_y_bwd(Ff
     const int tid = threadIdx.x;
     const C* src = a.B + (uint64_t)kz * N * N + x0;
     staged_copy<ROWS * N, NT, C>(
-        tid, [&](int l) { return src[(uint64_t)(l / ROWS) * N + (l % ROWS)]; },
+        tid,
+        [&](int l) {
+            C v = src[(uint64_t)(l / ROWS) * N + (l % ROWS)];
+            if (SG && a.deriv == 2) v = mul_mi(v, a.kd[l / ROWS]);
+            return v;
+        },
         [&](int l, C v) { lds[(l % ROWS) * RS + lds_slot(l / ROWS)] = v; });
     __syncthreads();
     fft_rows<C, N, ROWS, RS, +1>(lds, a.tw, 1, tid);
@@ -218,7 +243,8 @@ __global__ void __launch_bounds__((RowsCfg<T, N, N, V>::THREADS)) k_fft_y_bwd(Ff
 }
 
 // z pass backward (c2r): A[x][kz][y] -> R rows; flags a non-finite potential.
-template <typename T, int N, int V>
+// SG = 1: optional -i k_z on load.
+template <typename T, int N, int V, int SG = 0>
 __global__ void __launch_bounds__((RowsCfg<T, N / 2, N, V>::THREADS)) k_fft_z_c2r(FftArgs<T> a) {
     fft_box(a);
     using C = cplx<T>;
@@ -234,7 +260,12 @@ __global__ void __launch_bounds__((RowsCfg<T, N / 2, N, V>::THREADS)) k_fft_z_c2
     const int tid = threadIdx.x;
     const C* src = a.A + (uint64_t)x * NH * N + y0;
     staged_copy<ROWS * NH, NT, C>(
-        tid, [&](int l) { return src[(uint64_t)(l / ROWS) * N + (l % ROWS)]; },
+        tid,
+        [&](int l) {
+            C v = src[(uint64_t)(l / ROWS) * N + (l % ROWS)];
+            if (SG && a.deriv == 3) v = mul_mi(v, a.kd[l / ROWS]);
+            return v;
+        },
         [&](int l, C v) {
             const int r = l % ROWS, k = l / ROWS;
             if (k == 0 || k == LEN) v.y = (T)0;  // c2r reads only the independent half
@@ -280,6 +311,20 @@ __global__ void __launch_bounds__((RowsCfg<T, N / 2, N, V>::THREADS)) k_fft_z_c2
 // exchange, and a length-B DFT per row written straight back (or through a
 // transposing LDS tile where the pass changes the memory layout).
 // ===========================================================================
+// TreePM long range (src/mesh/green_function.cpp:165-203): continuum
+// 4 pi G / (-k^2), the Gaussian split exp(-k^2 r_s^2), CIC deconvolution.
+template <typename T>
+__device__ __forceinline__ T green_factor_treepm(const FftArgs<T>& a, int kx, int ky, int kz) {
+    if (kx == 0 && ky == 0 && kz == 0) return (T)0;
+    const T qx = a.kc[kx], qy = a.kc[ky], qz = a.kc[kz];
+    const T k2 = qx * qx + qy * qy + qz * qz;
+    T ff = (T)a.green_num / (-k2);
+    ff *= exp(-k2 * (T)a.rs * (T)a.rs);
+    const T w = a.green_win[kx] * a.green_win[ky] * a.green_win[kz];
+    ff /= (w * w);
+    return ff * (T)a.norm;
+}
+
 template <typename T>
 __device__ __forceinline__ T green_factor(const FftArgs<T>& a, int kx, int ky, int kz) {
     if (kx == 0 && ky == 0 && kz == 0) return (T)0;
@@ -393,7 +438,8 @@ __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_fwd(FftAr
 }
 
 // x pass: forward x FFT, Green multiply (with 1/N^3), backward x FFT, in place on B.
-template <typename T, int N, int V>
+// SG = 1: TreePM Green, optional -i k_x, output to xout when given.
+template <typename T, int N, int V, int SG = 0>
 __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_tl_x_green(FftArgs<T> a) {
     fft_box(a);
     using C = cplx<T>;
@@ -428,9 +474,10 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
         dft_reg<B, -1>(v);  // v[k2] = X[t + A k2]
 #pragma unroll
         for (int k2 = 0; k2 < B; ++k2) {
-            const T f = green_factor(a, t + A * k2, ky, kz);
+            const T f = SG ? green_factor_treepm(a, t + A * k2, ky, kz) : green_factor(a, t + A * k2, ky, kz);
             v[k2].x *= f;
             v[k2].y *= f;
+            if (SG && a.deriv == 1) v[k2] = mul_mi(v[k2], a.kd[t + A * k2]);
         }
         // backward transform of the held stride-A column (A and B exchanged)
         dft_reg<B, +1>(v);
@@ -443,8 +490,9 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
 #pragma unroll
         for (int n = 0; n < A; ++n) v[n] = xr[t * (A + 1) + n];
         dft_reg<A, +1>(v);
+        C* orow = (SG && a.xout) ? a.xout + q * N : row;
 #pragma unroll
-        for (int k2 = 0; k2 < A; ++k2) row[t + B * k2] = v[k2];
+        for (int k2 = 0; k2 < A; ++k2) orow[t + B * k2] = v[k2];
     }
 }
 
@@ -483,8 +531,9 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
     }
 }
 
-// y pass backward: B[kz][ky][x] -> A rows over y through a transposing tile.
-template <typename T, int N, int V>
+// y pass backward: B[kz][ky][x] -> A rows over y through a transposing tile
+// (SG = 1: optional -i k_y on load).
+template <typename T, int N, int V, int SG = 0>
 __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_bwd(FftArgs<T> a) {
     fft_box(a);
     using C = cplx<T>;
@@ -502,7 +551,12 @@ __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_bwd(FftAr
     const int x0 = xb * ROWS;
     const C* src = a.B + (uint64_t)kz * N * N + x0;
     staged_copy<ROWS * N, NT, C>(
-        tid, [&](int l) { return src[(uint64_t)(l / ROWS) * N + (l % ROWS)]; },
+        tid,
+        [&](int l) {
+            C val = src[(uint64_t)(l / ROWS) * N + (l % ROWS)];
+            if (SG && a.deriv == 2) val = mul_mi(val, a.kd[l / ROWS]);
+            return val;
+        },
         [&](int l, C val) { lds[(l / ROWS) * TS + (l % ROWS)] = val; });
     __syncthreads();
     C v[TL::EMAX];
@@ -529,7 +583,8 @@ __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_bwd(FftAr
 }
 
 // z pass backward (c2r): A[x][kz][y] -> R rows; flags a non-finite potential.
-template <typename T, int N, int V>
+// SG = 1: optional -i k_z on load.
+template <typename T, int N, int V, int SG = 0>
 __global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_c2r(FftArgs<T> a) {
     fft_box(a);
     using C = cplx<T>;
@@ -547,7 +602,12 @@ __global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_c2r(F
     const int y0 = yb * ROWS;
     const C* src = a.A + (uint64_t)x * NH * N + y0;
     staged_copy<ROWS * NH, NT, C>(
-        tid, [&](int l) { return src[(uint64_t)(l / ROWS) * N + (l % ROWS)]; },
+        tid,
+        [&](int l) {
+            C val = src[(uint64_t)(l / ROWS) * N + (l % ROWS)];
+            if (SG && a.deriv == 3) val = mul_mi(val, a.kd[l / ROWS]);
+            return val;
+        },
         [&](int l, C val) {
             const int rr = l % ROWS, k = l / ROWS;
             if (k == 0 || k == LEN) val.y = (T)0;  // c2r reads only the independent half

@@ -18,6 +18,7 @@
 #pragma once
 
 #include "hy_device.hpp"
+#include <type_traits>
 #include "hy_fft.hpp"
 
 namespace hy {
@@ -1079,6 +1080,95 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
             a.next_ranks[p] = rank;
         }
     }
+}
+
+// CIC interpolation of up to three meshes of one box
+// (src/mesh/mass_assignment_gather.inc:95-181,402-451): v = sum over the x, y,
+// z corner offsets, in that order, of f * ((wx * wy) * wz).  A product that
+// rounds to zero although its factors are non-zero raises ERR_INTERP_UNDERFLOW,
+// as the reference's underflow_error.  out[m] receives v (force mode, a
+// non-finite v raises ERR_FORCE_NONFINITE); add[m] receives += v.
+template <typename T>
+struct InterpArgs {
+    Geom g;
+    uint64_t n;
+    const double* x;
+    const double* y;
+    const double* z;
+    const T* f[3];
+    T* out[3];
+    double* add[3];
+    int meshes;
+    uint32_t* err;
+};
+
+template <typename T, bool O32>
+__global__ void __launch_bounds__(256) k_interp(InterpArgs<T> a) {
+    const uint64_t p = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    const Geom& g = a.g;
+    const int N = g.N;
+    int ix, iy, iz;
+    double fx, fy, fz;
+    if (!(cic_axis(a.x[p], g, ix, fx) && cic_axis(a.y[p], g, iy, fy) && cic_axis(a.z[p], g, iz, fz))) {
+        atomicOr(a.err, ERR_POSITION_NONFINITE);
+        return;
+    }
+    using Off = typename std::conditional<O32, uint32_t, uint64_t>::type;
+    const Off es = O32 ? (Off)sizeof(T) : (Off)1;
+    const Off NN = (Off)N * (Off)N;
+    const T wx[2] = {(T)(1.0 - fx), (T)fx};
+    const T wy[2] = {(T)(1.0 - fy), (T)fy};
+    const T wz[2] = {(T)(1.0 - fz), (T)fz};
+    const Off xo[2] = {(Off)ix * NN * es, (Off)wrap_cell(ix + 1, N) * NN * es};
+    const Off yo[2] = {(Off)iy * (Off)N * es, (Off)wrap_cell(iy + 1, N) * (Off)N * es};
+    const bool zrun = iz + 1 < N;
+    const Off z0 = (Off)iz * es, z1 = (Off)wrap_cell(iz + 1, N) * es;
+    typedef T V2 __attribute__((ext_vector_type(2), aligned(sizeof(T))));
+    bool lost = false, bad = false;
+    for (int m = 0; m < a.meshes; ++m) {
+        const T* f = a.f[m];
+        auto at = [&](Off o) -> const char* {
+            return O32 ? reinterpret_cast<const char*>(f) + o : reinterpret_cast<const char*>(f + o);
+        };
+        T F[2][2][2];
+#pragma unroll
+        for (int ox = 0; ox < 2; ++ox)
+#pragma unroll
+            for (int oy = 0; oy < 2; ++oy) {
+                const Off o = xo[ox] + yo[oy];
+                if (zrun) {
+                    const V2 v = *reinterpret_cast<const V2*>(at(o + z0));
+                    F[ox][oy][0] = v.x;
+                    F[ox][oy][1] = v.y;
+                } else {
+                    F[ox][oy][0] = *reinterpret_cast<const T*>(at(o + z0));
+                    F[ox][oy][1] = *reinterpret_cast<const T*>(at(o + z1));
+                }
+            }
+        T v = (T)0;
+#pragma unroll
+        for (int ox = 0; ox < 2; ++ox)
+#pragma unroll
+            for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+                for (int oz = 0; oz < 2; ++oz) {
+                    const T xy = wx[ox] * wy[oy];
+                    const T w = xy * wz[oz];
+                    const T fv = F[ox][oy][oz];
+                    if (wx[ox] != (T)0 && wy[oy] != (T)0 && wz[oz] != (T)0 && w == (T)0 && fv != (T)0) lost = true;
+                    const T c = fv * w;
+                    if (fv != (T)0 && w != (T)0 && c == (T)0) lost = true;
+                    v += c;
+                }
+        if (a.out[m]) {
+            if (!isfinite(v)) bad = true;
+            a.out[m][p] = v;
+        }
+        if (a.add[m]) a.add[m][p] += (double)v;
+    }
+    if (lost) atomicOr(a.err, ERR_INTERP_UNDERFLOW);
+    if (bad) atomicOr(a.err, ERR_FORCE_NONFINITE);
 }
 
 // Transactional commit for the reference-shaped force entry: acc += g.

@@ -14,7 +14,8 @@ from typing import Optional, Sequence
 import numpy as np
 
 from ._lib import (FFT_PASS_NAMES, HY_ASYNC, HY_FFT_PASSES, HY_HAS_NEXT, HY_HAS_PREV,
-                   HY_HOST_POINTERS, HY_PREC_FP32_MIXED, HY_PREC_FP64, HY_REUSE_KEYS, check,
+                   HY_HOST_POINTERS, HY_METHOD_PURE_PM, HY_METHOD_TREEPM_LONG_RANGE,
+                   HY_PREC_FP32_MIXED, HY_PREC_FP64, HY_REUSE_KEYS, check,
                    hy_fft_tuning, hy_pk_bin, hy_pk_options, hy_pm_desc, hy_pm_info, lib)
 
 PRECISIONS = {"fp64": HY_PREC_FP64, "fp32": HY_PREC_FP32_MIXED, "fp32_mixed": HY_PREC_FP32_MIXED}
@@ -144,14 +145,22 @@ class PMSolver:
     pure PM (discrete-Laplacian Green function, centered-gradient gather).
 
     precision: "fp64" (binary64 everywhere) or "fp32" (binary32 mesh, FFT,
-    forces and momenta; binary64 positions and time integrals)."""
+    forces and momenta; binary64 positions and time integrals).
+
+    method: "pure_pm" (default) or "treepm_long_range" with split_scale r_s
+    (mesh::PMForceMethod::treepm_long_range: continuum Green x
+    exp(-k^2 r_s^2), CIC compensation, spectral gradient; compute_forces,
+    force_meshes and interpolate only)."""
+
+    METHODS = {"pure_pm": HY_METHOD_PURE_PM, "treepm_long_range": HY_METHOD_TREEPM_LONG_RANGE}
 
     def __init__(self, box_size: float, mesh_size: int, deconvolve_cic: bool = False,
                  precision: str = "fp32", max_particles: int = 0, device: int = 0,
-                 stream=None, boxes: int = 1):
+                 stream=None, boxes: int = 1, method: str = "pure_pm", split_scale: float = 0.0):
         self.box_size = float(box_size)
         self.mesh_size = int(mesh_size)
         self.precision = PRECISIONS[precision]
+        self.method = method
         if stream is None:
             # run on the caller's current torch stream when it is not the
             # default stream (the private stream is ordered with the default one)
@@ -165,7 +174,7 @@ class PMSolver:
         d = hy_pm_desc(box_size=self.box_size, mesh_size=self.mesh_size,
                        deconvolve_cic=int(bool(deconvolve_cic)), precision=self.precision,
                        device=device, stream=stream, max_particles=int(max_particles),
-                       boxes=int(boxes))
+                       boxes=int(boxes), method=self.METHODS[method], split_scale=float(split_scale))
         h = C.c_void_p()
         check(lib().hy_pm_create(C.byref(d), C.byref(h)))
         self._h = h
@@ -253,6 +262,19 @@ class PMSolver:
         out = torch.empty_like(src)
         check(lib().hy_pm_potential(self._h, _ptr(src), _ptr(out)))
         return out
+
+    def force_meshes(self, source):
+        """TreePM long-range force meshes (F_x, F_y, F_z) of a density-source mesh."""
+        torch = _torch()
+        src = source.to(self._torch_mesh_dtype()).contiguous()
+        out = [torch.empty_like(src) for _ in range(3)]
+        check(lib().hy_pm_force_meshes(self._h, _ptr(src), *(_ptr(o) for o in out)))
+        return out
+
+    def interpolate(self, mesh, x, y, z, out) -> None:
+        """out += CIC interpolation of mesh at the positions (CICMassAssignment::interpolate_add)."""
+        m = mesh.to(self._torch_mesh_dtype()).contiguous()
+        check(lib().hy_pm_interpolate(self._h, len(x), _ptr(x), _ptr(y), _ptr(z), _ptr(m), _ptr(out)))
 
     def gather(self, phi, x, y, z, acc_x, acc_y, acc_z) -> None:
         p = phi.to(self._torch_mesh_dtype()).contiguous()

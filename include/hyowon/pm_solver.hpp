@@ -15,6 +15,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <limits>
 #include <optional>
 #include <span>
 #include <stdexcept>
@@ -38,22 +39,58 @@ inline void throw_status(int rc) {
 
 enum class Precision { FP64 = HY_PREC_FP64, FP32Mixed = HY_PREC_FP32_MIXED };
 
+namespace mesh {
+
+// mesh::PMForceMethod (include/cosmo_nbody/mesh/pm_force_method.hpp:10-60):
+// pure PM, or the TreePM long-range force (continuum Green, Gaussian split,
+// spectral gradient, CIC compensation).
+class PMForceMethod {
+public:
+    static PMForceMethod pure_pm(bool deconvolve_cic) noexcept {
+        return PMForceMethod(HY_METHOD_PURE_PM, 0.0, deconvolve_cic);
+    }
+    static PMForceMethod treepm_long_range(double split_scale) {
+        if (!(split_scale > 0.0) || split_scale == std::numeric_limits<double>::infinity())
+            throw std::invalid_argument("TreePM long-range PM method requires a finite positive split scale");
+        return PMForceMethod(HY_METHOD_TREEPM_LONG_RANGE, split_scale, true);
+    }
+    int kind() const noexcept { return kind_; }
+    double split_scale() const noexcept { return split_scale_; }
+    bool deconvolves_cic() const noexcept { return deconvolve_cic_; }
+
+private:
+    PMForceMethod(int kind, double split_scale, bool deconvolve_cic)
+        : kind_(kind), split_scale_(split_scale), deconvolve_cic_(deconvolve_cic) {}
+    int kind_;
+    double split_scale_;
+    bool deconvolve_cic_;
+};
+
+}  // namespace mesh
+
 namespace gravity {
 
-// Pure PM (discrete-Laplacian Green function, centered-gradient CIC gather).
+// gravity::PMSolver: pure PM (discrete-Laplacian Green function,
+// centered-gradient CIC gather) or the TreePM long-range force.
 class PMSolver {
 public:
     PMSolver(double box_size, std::size_t mesh_size, bool deconvolve_cic,
              std::size_t max_particles, Precision precision = Precision::FP32Mixed,
-             int device = 0, void* stream = nullptr) {
+             int device = 0, void* stream = nullptr)
+        : PMSolver(box_size, mesh_size, mesh::PMForceMethod::pure_pm(deconvolve_cic), max_particles, precision,
+                   device, stream) {}
+    PMSolver(double box_size, std::size_t mesh_size, mesh::PMForceMethod method, std::size_t max_particles,
+             Precision precision = Precision::FP32Mixed, int device = 0, void* stream = nullptr) {
         hy_pm_desc d{};
         d.box_size = box_size;
         d.mesh_size = mesh_size;
-        d.deconvolve_cic = deconvolve_cic ? 1 : 0;
+        d.deconvolve_cic = method.deconvolves_cic() ? 1 : 0;
         d.precision = static_cast<int>(precision);
         d.device = device;
         d.stream = stream;
         d.max_particles = max_particles;
+        d.method = method.kind();
+        d.split_scale = method.split_scale();
         throw_status(hy_pm_create(&d, &ctx_));
     }
     ~PMSolver() {

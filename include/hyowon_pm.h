@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define HY_ABI_VERSION 2
+#define HY_ABI_VERSION 3
 
 typedef enum {
     HY_OK = 0,
@@ -50,6 +50,12 @@ typedef enum {
     HY_PREC_FP64 = 0,        /* binary64 mesh, FFT, forces, momenta */
     HY_PREC_FP32_MIXED = 1   /* binary32 mesh/FFT/forces/momenta, binary64 positions */
 } hy_precision;
+
+/* PM discretisation (include/cosmo_nbody/mesh/pm_force_method.hpp:10-60) */
+typedef enum {
+    HY_METHOD_PURE_PM = 0,            /* discrete Laplacian Green, centered real-space gradient */
+    HY_METHOD_TREEPM_LONG_RANGE = 1   /* continuum Green x exp(-k^2 r_s^2), W^-2, spectral gradient */
+} hy_method;
 
 /* Flags */
 #define HY_HOST_POINTERS  0x1u  /* particle arrays live in host memory */
@@ -72,6 +78,9 @@ typedef struct {
                                1 = one box).  Particle arrays of hy_pm_force / hy_pm_kdk then
                                concatenate the boxes (n / boxes particles each, uniform mass);
                                every box has its own mesh and its own mass identity. */
+    int method;             /* hy_method; HY_METHOD_TREEPM_LONG_RANGE deconvolves always, takes
+                               a single box and serves hy_pm_force / hy_pm_force_meshes only */
+    double split_scale;     /* r_s of HY_METHOD_TREEPM_LONG_RANGE (finite, positive) */
 } hy_pm_desc;
 
 typedef struct {
@@ -136,6 +145,17 @@ int hy_pm_density(hy_pm_ctx* ctx, uint64_t n, const double* x, const double* y, 
 /* Poisson solve of a density-source mesh: r2c, Green, c2r with 1/N^3
  * (src/mesh/fft_backend.cpp:1088-1145, src/mesh/green_function.cpp:209-241). */
 int hy_pm_potential(hy_pm_ctx* ctx, const void* source_in, void* phi_out);
+
+/* TreePM long-range force meshes of a density-source mesh: r2c, continuum
+ * Green x exp(-k^2 r_s^2) / W^2, then per axis -i k phi (k = 0 on the Nyquist
+ * index) and c2r with 1/N^3 (src/mesh/green_function.cpp:165-203,
+ * src/gravity/pm_solver.cpp:768-810).  HY_METHOD_TREEPM_LONG_RANGE contexts. */
+int hy_pm_force_meshes(hy_pm_ctx* ctx, const void* source_in, void* fx_out, void* fy_out, void* fz_out);
+
+/* CIC interpolation of a mesh (context precision) added into out (binary64):
+ * CICMassAssignment::interpolate_add (src/mesh/mass_assignment_gather.inc:181-196,402-451). */
+int hy_pm_interpolate(hy_pm_ctx* ctx, uint64_t n, const double* x, const double* y, const double* z,
+                      const void* mesh, double* out);
 
 /* Fused centered-gradient gather of a given potential, adding into ax/ay/az
  * (src/mesh/mass_assignment_gradient.cpp:46-226). */

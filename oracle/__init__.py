@@ -116,6 +116,10 @@ def lib():
         L.fn("or_pk_window_axis", C.c_longdouble, _d, _d)
         L.fn("or_pk_raw_power", _d, _d, _d, _d)
         L.fn("or_pk_raw_power_minus", _d, _d, _d, _d, _d)
+        L.fn("or_green_apply_treepm", C.c_int, _u64, _d, _d, _pd)
+        L.fn("or_interpolate_add", C.c_int, _u64, _pd, _pd, _pd, _pd, _d, _u64, _pd)
+        L.fn("or_pm_force_treepm", C.c_int, _u64, _pd, _pd, _pd, _pd, _d, _d, _u64, _d, _pd, _pd,
+             _pd, _pd)
         L.fn("or_set_threads", None, C.c_int)
         L.fn("or_get_threads", C.c_int)
         L.fn("or_gravitational_constant", _d)
@@ -151,6 +155,8 @@ def ref():
         L.fn("ref_pk_volume_times", C.c_int, _d, _d, _pd)
         L.fn("ref_density_direct_available", C.c_int, _u64, _d, _d, C.POINTER(C.c_int))
         L.fn("ref_green_apply", C.c_int, _u64, _d, C.c_int, _pd)
+        L.fn("ref_green_apply_treepm", C.c_int, _u64, _d, _d, _pd)
+        L.fn("ref_interpolate_add", C.c_int, _u64, _pd, _pd, _pd, _pd, _d, _u64, _pd)
         L.fn("ref_gather", C.c_int, _u64, _pd, _pd, _pd, _pd, _d, _u64, _pd, _pd, _pd)
         L.fn("ref_mass_conservation", C.c_int, _d, _d, _u64)
         L.fn("ref_exact_sum", _d, _pd, _u64)
@@ -370,3 +376,34 @@ def pk_edges(minimum, maximum, bins):
     out = np.zeros(bins + 1, np.float64)
     _check(lib().or_pk_edges(minimum, maximum, bins, _ptr(out)), "or_pk_edges")
     return out
+
+
+def green_apply_treepm(cplx, L, split_scale):
+    N = cplx.shape[0]
+    c = np.ascontiguousarray(cplx, np.complex128).copy()
+    _check(lib().or_green_apply_treepm(N, L, split_scale, c.ctypes.data_as(_pd)), "or_green_apply_treepm")
+    return c
+
+
+def interpolate_add(mesh, x, y, z, L, out=None):
+    N = mesh.shape[0]
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    out = np.zeros(len(x)) if out is None else _f64(out).copy()
+    m = _f64(mesh).reshape(-1)
+    _check(lib().or_interpolate_add(len(x), _ptr(m), _ptr(x), _ptr(y), _ptr(z), L, N, _ptr(out)),
+           "or_interpolate_add")
+    return out
+
+
+def pm_force_treepm(x, y, z, L, N, split_scale, masses=None, uniform_mass=1.0, meshes=False):
+    """TreePM long-range PM force (spectral gradient); returns (ax, ay, az[, force meshes])."""
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    n = len(x)
+    m = _f64(masses) if masses is not None else None
+    ax, ay, az = np.zeros(n), np.zeros(n), np.zeros(n)
+    mo = np.zeros(3 * N ** 3) if meshes else None
+    _check(lib().or_pm_force_treepm(n, _ptr(x), _ptr(y), _ptr(z), _ptr(m), uniform_mass, L, N, split_scale,
+                                    _ptr(ax), _ptr(ay), _ptr(az), _ptr(mo)), "or_pm_force_treepm")
+    if meshes:
+        return ax, ay, az, mo.reshape(3, N, N, N)
+    return ax, ay, az

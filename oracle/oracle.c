@@ -818,6 +818,147 @@ int or_pm_force(uint64_t n, const double* x, const double* y, const double* z,
 }
 
 /* ------------------------------------------------------------------------- */
+/* TreePM long-range PM: continuum Green with the Gaussian split, spectral     */
+/* gradient, CIC interpolation (src/mesh/green_function.cpp:165-203,           */
+/* src/gravity/pm_solver.cpp:768-810, src/mesh/mass_assignment_gather.inc:     */
+/* 95-181,402-451).                                                            */
+/* ------------------------------------------------------------------------- */
+static double k_component(uint64_t i, uint64_t N, double L) {
+    return 2.0 * OR_PI * (double)mode_number(i, N) / L;
+}
+
+int or_green_apply_treepm(uint64_t N, double L, double split_scale, double* out) {
+    if (!isfinite(split_scale) || split_scale <= 0.0) return OR_EINVAL;
+    green_tables t;
+    if (green_tables_make(&t, N, L) != OR_OK) return OR_ERUNTIME;
+    const uint64_t NH = N / 2 + 1;
+    cplx* c = (cplx*)out;
+    int failed = 0;
+#pragma omp parallel for schedule(static) num_threads(OR_NT) reduction(|: failed)
+    for (int64_t ix = 0; ix < (int64_t)N; ++ix)
+        for (uint64_t iy = 0; iy < N; ++iy)
+            for (uint64_t iz = 0; iz < NH; ++iz) {
+                double f = 0.0;
+                if (!(ix == 0 && iy == 0 && iz == 0)) {
+                    const double kx = k_component((uint64_t)ix, N, L), ky = k_component(iy, N, L),
+                                 kz = k_component(iz, N, L);
+                    const double k2 = kx * kx + ky * ky + kz * kz;
+                    f = 4.0 * OR_PI * t.G / (-k2);
+                    f *= exp(-k2 * split_scale * split_scale);
+                    const double w = t.win[ix] * t.win[iy] * t.win[iz];
+                    f /= (w * w);
+                }
+                if (!isfinite(f)) failed = 1;
+                cplx* v = &c[iz + NH * (iy + N * (uint64_t)ix)];
+                v->re *= f;
+                v->im *= f;
+            }
+    green_tables_free(&t);
+    return failed ? OR_EOVERFLOW : OR_OK;
+}
+
+/* out += CIC interpolation of field: sum over (x, y, z) corner offsets of
+ * field * ((wx * wy) * wz) in that order.  Returns OR_EUNDERFLOW when a
+ * non-zero weight or field-weight product rounds to zero. */
+int or_interpolate_add(uint64_t n, const double* field, const double* x, const double* y,
+                       const double* z, double L, uint64_t N, double* out) {
+    const double dx = L / (double)N;
+    for (uint64_t p = 0; p < n; ++p)
+        if (!isfinite(x[p]) || !isfinite(y[p]) || !isfinite(z[p])) return OR_EINVAL;
+    int lost = 0, bad = 0;
+#pragma omp parallel for schedule(static) num_threads(OR_NT) reduction(|: lost, bad)
+    for (int64_t pi = 0; pi < (int64_t)n; ++pi) {
+        const uint64_t p = (uint64_t)pi;
+        int64_t ix, iy, iz;
+        double fx, fy, fz;
+        if (or_cic_coordinate(x[p], L, dx, N, &ix, &fx) != OR_OK
+            || or_cic_coordinate(y[p], L, dx, N, &iy, &fy) != OR_OK
+            || or_cic_coordinate(z[p], L, dx, N, &iz, &fz) != OR_OK) {
+            bad = 1;
+            continue;
+        }
+        const double wx[2] = {1.0 - fx, fx}, wy[2] = {1.0 - fy, fy}, wz[2] = {1.0 - fz, fz};
+        double v = 0.0;
+        for (int ox = 0; ox < 2; ++ox) {
+            const uint64_t gx = wrap_index(ix + ox, N);
+            for (int oy = 0; oy < 2; ++oy) {
+                const uint64_t gy = wrap_index(iy + oy, N);
+                for (int oz = 0; oz < 2; ++oz) {
+                    const uint64_t gz = wrap_index(iz + oz, N);
+                    const double xy = wx[ox] * wy[oy];
+                    const double w = xy * wz[oz];
+                    const double f = field[(gx * N + gy) * N + gz];
+                    if (wx[ox] != 0.0 && wy[oy] != 0.0 && wz[oz] != 0.0 && w == 0.0 && f != 0.0) lost = 1;
+                    const double c = f * w;
+                    if (f != 0.0 && w != 0.0 && c == 0.0) lost = 1;
+                    v += c;
+                }
+            }
+        }
+        out[p] += v;
+    }
+    if (bad) return OR_ELOGIC;
+    if (lost) return OR_EUNDERFLOW;
+    return OR_OK;
+}
+
+/* The long-range PM force of TreePM (PMForceMethod::treepm_long_range):
+ * deposit, density source, r2c, continuum Green x exp(-k^2 r_s^2) / W^2, then
+ * per axis -i k_axis phi (k_axis = 0 on the Nyquist index), c2r with 1/N^3 and
+ * CIC interpolation added into a_axis.  mesh_out (optional, 3 N^3) receives
+ * the three force meshes. */
+int or_pm_force_treepm(uint64_t n, const double* x, const double* y, const double* z,
+                       const double* masses, double uniform_mass, double L, uint64_t N,
+                       double split_scale, double* ax, double* ay, double* az, double* mesh_out) {
+    const uint64_t cells = N * N * N, NH = N / 2 + 1, cm = N * N * NH;
+    if (!masses && (!isfinite(uniform_mass) || uniform_mass <= 0.0)) return OR_EINVAL;
+    double* mesh = (double*)malloc(cells * sizeof(double));
+    double* spec = (double*)malloc(cm * 2 * sizeof(double));
+    double* tmp = (double*)malloc(cm * 2 * sizeof(double));
+    if (!mesh || !spec || !tmp) { free(mesh); free(spec); free(tmp); return OR_ERUNTIME; }
+    int err = or_deposit(n, x, y, z, masses, uniform_mass, L, N, mesh);
+    double input_mass = 0.0;
+    if (err == OR_OK) {
+        input_mass = masses ? or_exact_sum(masses, n) : or_exact_repeated(uniform_mass, n);
+        if (!isfinite(input_mass) || input_mass <= 0.0 || n == 0) err = OR_ERUNTIME;
+    }
+    if (err == OR_OK) {
+        const double deposited = or_exact_sum(mesh, cells);
+        if (!isfinite(deposited) || deposited <= 0.0) err = OR_ERUNTIME;
+        else err = or_require_cic_mass_conservation(input_mass, deposited, n);
+    }
+    if (err == OR_OK) err = or_density_contrast(mesh, N, L / (double)N, input_mass);
+    if (err == OR_OK) err = or_fft_r2c_3d(N, mesh, spec);
+    if (err == OR_OK) err = or_green_apply_treepm(N, L, split_scale, spec);
+    double* acc[3] = {ax, ay, az};
+    for (int axis = 0; axis < 3 && err == OR_OK; ++axis) {
+        for (uint64_t ix = 0; ix < N; ++ix) {
+            double kx = k_component(ix, N, L);
+            if (N % 2 == 0 && ix == N / 2) kx = 0.0;
+            for (uint64_t iy = 0; iy < N; ++iy) {
+                double ky = k_component(iy, N, L);
+                if (N % 2 == 0 && iy == N / 2) ky = 0.0;
+                for (uint64_t iz = 0; iz < NH; ++iz) {
+                    double kz = k_component(iz, N, L);
+                    if (N % 2 == 0 && iz == N / 2) kz = 0.0;
+                    const double k = axis == 0 ? kx : (axis == 1 ? ky : kz);
+                    const uint64_t i = 2 * (iz + NH * (iy + N * ix));
+                    tmp[i] = spec[i + 1] * k;
+                    tmp[i + 1] = -spec[i] * k;
+                }
+            }
+        }
+        err = or_fft_c2r_3d(N, tmp, mesh);
+        if (err == OR_OK && mesh_out) memcpy(mesh_out + (uint64_t)axis * cells, mesh, cells * sizeof(double));
+        if (err == OR_OK) err = or_interpolate_add(n, mesh, x, y, z, L, N, acc[axis]);
+    }
+    free(mesh);
+    free(spec);
+    free(tmp);
+    return err;
+}
+
+/* ------------------------------------------------------------------------- */
 /* KDK particle update: include/cosmo_nbody/time/particle_update.hpp:20-116    */
 /* ------------------------------------------------------------------------- */
 static double add_periodic_displacement(double pos, double disp, double L) {

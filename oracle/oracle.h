@@ -190,6 +190,20 @@ void or_normal_pair(uint64_t seed, int32_t kx, int32_t ky, int32_t kz, uint32_t 
 int or_gaussian_modes(uint64_t N, uint64_t seed, uint64_t max_mode, const double* variance_by_m2,
                       uint32_t flags, double* out);
 
+/* TreePM long-range PM (PMForceMethod::treepm_long_range):
+ * or_green_apply_treepm: continuum Green 4 pi G / (-k^2) * exp(-k^2 r_s^2) / W^2
+ *   (src/mesh/green_function.cpp:165-203).
+ * or_interpolate_add: CIC interpolation added into out
+ *   (src/mesh/mass_assignment_gather.inc:95-181,402-451).
+ * or_pm_force_treepm: the long-range force with the spectral gradient
+ *   (src/gravity/pm_solver.cpp:768-810); mesh_out (optional) gets 3 N^3. */
+int or_green_apply_treepm(uint64_t N, double L, double split_scale, double* cplx);
+int or_interpolate_add(uint64_t n, const double* field, const double* x, const double* y,
+                       const double* z, double L, uint64_t N, double* out);
+int or_pm_force_treepm(uint64_t n, const double* x, const double* y, const double* z,
+                       const double* masses, double uniform_mass, double L, uint64_t N,
+                       double split_scale, double* ax, double* ay, double* az, double* mesh_out);
+
 /* Matter power spectrum (src/analysis/matter_power_spectrum.cpp:57-241 over
  * src/analysis/fourier_density_field.cpp:112-343).  masses == NULL selects the
  * uniform mass.  out: num_bins rows of 8 doubles (k_low, k_mean, k_high, p_raw,

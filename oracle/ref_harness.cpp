@@ -218,6 +218,28 @@ int ref_green_apply(uint64_t N, double L, int deconvolve, double* cplx) {
     });
 }
 
+int ref_green_apply_treepm(uint64_t N, double L, double split_scale, double* cplx) {
+    return guarded([&] {
+        const cn::mesh::MeshGeometry geom(L, N);
+        const cn::mesh::GreenFunction green(geom, cn::mesh::PMForceMethod::treepm_long_range(split_scale));
+        cn::mesh::ComplexField field(reinterpret_cast<std::complex<double>*>(cplx),
+                                     geom.complex_size(), cn::mesh::external_mesh_storage);
+        green.apply(field);
+    });
+}
+
+int ref_interpolate_add(uint64_t n, const double* mesh, const double* x, const double* y,
+                        const double* z, double L, uint64_t N, double* out) {
+    return guarded([&] {
+        const cn::mesh::MeshGeometry geom(L, N);
+        cn::mesh::CICMassAssignment cic(geom);
+        cn::mesh::RealField field(const_cast<double*>(mesh), geom.real_size(),
+                                  cn::mesh::external_mesh_storage);
+        cic.interpolate_add(field, nullptr, std::span<const double>(x, n), std::span<const double>(y, n),
+                            std::span<const double>(z, n), std::span<double>(out, n));
+    });
+}
+
 int ref_gather(uint64_t n, const double* phi, const double* x, const double* y, const double* z,
                double L, uint64_t N, double* ax, double* ay, double* az) {
     return guarded([&] {

@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol(hy):
     assert len(syms) >= 20
     for s in syms:
         assert hasattr(lib, s), s
-    assert lib.hy_abi_version() == 2
+    assert lib.hy_abi_version() == 3
 
 
 def test_python_binding_covers_the_header(hy):

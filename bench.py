@@ -37,9 +37,12 @@ def stage_bytes(stage: str, n_p: int, N: int, real: int) -> float:
     half = N * N * NH * 2 * real  # one half-spectrum array
     pos = 24.0 * n_p
     mom = 3.0 * real * n_p
+    fp32 = real == 4
     return {
-        "bucket": pos + 20.0 * n_p,                 # positions + key/rank write/read + index write
-        "deposit": pos + real * M,                  # positions re-read + mesh write
+        # keys + ranks read; FP32: positions read, 16-B deposit records written; FP64: 4-B order written
+        "bucket": (8.0 * n_p + pos + 16.0 * n_p) if fp32 else 12.0 * n_p,
+        # FP32: records read; FP64: order + positions read; mesh written
+        "deposit": (16.0 * n_p if fp32 else 4.0 * n_p + pos) + real * M,
         "fft_z_r2c": real * M + half,               # real mesh in, half spectrum out
         "fft_y_fwd": 2.0 * half,
         "fft_x_green": 2.0 * half,
@@ -56,7 +59,7 @@ def step_alg_bytes(n_p: int, N: int, real: int) -> float:
 
 
 STAGE_KERNEL = {"bucket": "hy::k_scatter_records", "deposit": "void hy::k_deposit_fixed<16>",
-                "gather_kdk": "hy::k_gather_kdk<float, float>"}
+                "gather_kdk": "void hy::k_gather_kdk<float, float, true>"}
 FFT_KERNEL = {"fft_z_r2c": "k_fft_z_r2c", "fft_y_fwd": "k_fft_y_fwd", "fft_x_green": "k_fft_x_green",
               "fft_y_bwd": "k_fft_y_bwd", "fft_z_c2r": "k_fft_z_c2r"}
 
@@ -71,7 +74,8 @@ def measured_traffic(stage: str, n_side: int, N: int, prec: str, plan: dict):
         prof = json.load(f)
     if stage in FFT_KERNEL:
         v = plan["variant"][stage[4:]]
-        name = f"void hy::{FFT_KERNEL[stage].replace('k_fft_', 'k_tl_')}<float, {N}, {v}>"
+        sg = ", 0" if stage in ("fft_x_green", "fft_y_bwd", "fft_z_c2r") else ""  # spectral-gradient flag
+        name = f"void hy::{FFT_KERNEL[stage].replace('k_fft_', 'k_tl_')}<float, {N}, {v}{sg}>"
     else:
         name = STAGE_KERNEL.get(stage)
     k = prof.get("kernels", {}).get(name)
@@ -407,6 +411,56 @@ def run_pk(args):
     return out
 
 
+def run_treepm(args):
+    """TreePM long-range PM force (HY_METHOD_TREEPM_LONG_RANGE, r_s = 1.25 cells)
+    on the config-2 particle load (512^3 1LPT particles, 1024^3 mesh, L = 512):
+    one hy_pm_force per call (deposit, r2c, TreePM Green, three spectral
+    derivatives and c2r transforms, CIC interpolation of three force meshes),
+    timed beside the pure-PM force refresh of the same particles."""
+    import torch
+    import hyowon_amd as H
+    from hyowon_amd import ic
+    n_side, N, L = args.n_side, args.mesh, args.box
+    n = n_side ** 3
+    a0 = 1.0 / (1.0 + args.z_start)
+    dev = torch.device("cuda:0")
+    pos, mom = ic.lpt_ics(n_side, L, a0, seed=1, device=dev, rng="philox", max_mode=n_side // 2 - 1)
+    del mom
+    m = ic.particle_mass(n_side, L)
+    rs = 1.25 * L / N
+    res = {}
+    for prec in args.precisions:
+        for method in ("treepm_long_range", "pure_pm"):
+            s = H.PMSolver(L, N, precision=prec, max_particles=n, method=method, split_scale=rs)
+            acc = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(3)]
+            reps = args.steps if prec == args.precisions[0] else max(2, args.fp64_steps)
+            for _ in range(max(1, args.warmup)):
+                s.compute_forces(*pos, None, m, *acc)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                s.compute_forces(*pos, None, m, *acc)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / reps
+            res[f"{prec}_{method}"] = ms
+            print(f"{prec} {method}: {ms:.2f} ms per force", file=sys.stderr, flush=True)
+            s.close()
+            del acc
+            torch.cuda.empty_cache()
+    main_prec = args.precisions[0]
+    ms = res[f"{main_prec}_treepm_long_range"]
+    return {
+        "metric": "TreePM long-range PM force evaluations/s (512^3 particles, 1024^3 mesh)",
+        "value": n / (ms * 1e-3), "unit": "particle-forces/s", "n_gpus": 1, "ms_per_step": ms,
+        "higher_is_better": True, "dtype": "f32 mesh/FFT/forces" if main_prec == "fp32" else "f64",
+        "data": "synthetic 1LPT ICs (Philox mode-keyed field, BBKS, sigma8=0.8, z=49, seed 1, K=n/2-1)",
+        "config": {"workload": "hy_pm_force with PMForceMethod::treepm_long_range(r_s = 1.25 dx) on the "
+                               "config-2 particle load: 512^3 particles, 1024^3 mesh, L=512",
+                   "particles": n, "mesh": N, "split_scale": rs},
+        "ms_per_force": {k: round(v, 3) for k, v in res.items()},
+    }
+
+
 def run_adjoint(args):
     """Config-4 workload: 128^3 particles from a 128^3 white-noise field (seed
     7), 256^3 mesh, L = 256 Mpc/h, 16 KDK steps z = 19 -> 0; loss = mean
@@ -540,9 +594,12 @@ def main():
                     help="FP32-vs-FP64 force, final-position and P(k) errors (full run)")
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
                     help="measure the FFT launch plan before timing (untimed)")
-    ap.add_argument("--workload", default="config2", choices=["config2", "ensemble", "adjoint", "pk"],
+    ap.add_argument("--workload", default="config2",
+                    choices=["config2", "ensemble", "adjoint", "pk", "treepm"],
                     help="config2: the 512^3 / 1024^3 KDK step (default); ensemble: the 192-box "
-                         "graph-launched ensemble step; adjoint: the differentiable 128^3 / 256^3 run")
+                         "graph-launched ensemble step; adjoint: the differentiable 128^3 / 256^3 run; "
+                         "pk: the power-spectrum estimator on the config-2 particles; treepm: the "
+                         "TreePM long-range force on the config-2 particles")
     ap.add_argument("--fft-plan", default="",
                     help="pin the FFT launch plan 'v0,v1,v2,v3,v4[:plane_group]' instead of autotuning")
     ap.add_argument("--rocfft", action=argparse.BooleanOptionalAction, default=True,
@@ -566,6 +623,9 @@ def main():
         return
     if args.workload == "pk":
         print(json.dumps(run_pk(args)), flush=True)
+        return
+    if args.workload == "treepm":
+        print(json.dumps(run_treepm(args)), flush=True)
         return
     n, N, res = run_gpu(args)
     main_prec = args.precisions[0]

@@ -203,9 +203,9 @@ def accuracy(args, pos, mom, m, K1, K2, D, nsteps):
     """FP32-mixed against binary64 on the same ICs: RMS relative force error
     sqrt(sum|g32-g64|^2 / sum|g64|^2), median and p99 of |dg|/|g| per particle,
     then the full log-a run in both precisions (final positions as periodic
-    minimum-image distances in cells, and the CIC power spectrum below half
-    the Nyquist wavenumber).  The binary64 GPU step is itself pinned to the
-    oracle by the parity tests."""
+    minimum-image distances in cells, and the matter power spectrum below half
+    the Nyquist wavenumber from hy_pm_power_spectrum).  The binary64 GPU step
+    is itself pinned to the oracle by the parity tests."""
     import torch
     import hyowon_amd as H
     n_side, N, L = args.n_side, args.mesh, args.box
@@ -242,11 +242,13 @@ def accuracy(args, pos, mom, m, K1, K2, D, nsteps):
                   reuse_keys=True)
             st, sp = sp, st
         s.check()
-        # CIC density contrast of the final positions (binary64 deposit)
+        # MatterPowerSpectrum of the final positions (binary64 estimator, 32
+        # logarithmic shells up to half the Nyquist wavenumber, interlaced)
         s64 = H.PMSolver(L, N, precision="fp64", max_particles=n)
-        rho = s64.density(st.x, st.y, st.z, None, m)
+        bins = s64.power_spectrum(st.x, st.y, st.z, 32, False, True, 0.5, uniform_mass=m)
         s64.close()
-        final[prec] = (torch.stack([st.x, st.y, st.z]), rho)
+        final[prec] = (torch.stack([st.x, st.y, st.z]),
+                       [(b.k_mean, b.p_corrected, int(b.mode_count)) for b in bins])
         s.close()
         del sp
         torch.cuda.empty_cache()
@@ -257,26 +259,16 @@ def accuracy(args, pos, mom, m, K1, K2, D, nsteps):
     pos_rms = float(torch.sqrt((dist ** 2).mean()))
     pos_max = float(dist.max())
     del dpos, dist
-    pk = {}
-    for prec in ("fp32", "fp64"):
-        rho = final[prec][1]
-        spec = torch.fft.rfftn(rho)
-        p = (spec.real ** 2 + spec.imag ** 2)
-        del spec
-        kf = torch.fft.fftfreq(N, device=dev) * N
-        kz = torch.arange(N // 2 + 1, device=dev, dtype=kf.dtype)
-        kk = torch.sqrt(kf[:, None, None] ** 2 + kf[None, :, None] ** 2 + kz[None, None, :] ** 2)
-        shell = torch.round(kk).long().clamp_max(N // 4)
-        del kk
-        pk[prec] = (torch.bincount(shell.flatten(), weights=p.flatten(), minlength=N // 4 + 1)
-                    / torch.bincount(shell.flatten(), minlength=N // 4 + 1).clamp_min(1))
-        del p, shell
-        torch.cuda.empty_cache()
-    r = (pk["fp32"][1:N // 4] / pk["fp64"][1:N // 4] - 1.0).abs()
+    p32, p64 = final["fp32"][1], final["fp64"][1]
+    rel = [abs(a[1] / b[1] - 1.0) for a, b in zip(p32, p64) if b[2] and b[1] != 0.0]
     return {"force_rms_rel_fp32_vs_fp64": rms, "force_rel_median": q[0], "force_rel_p99": q[1],
             "force_at": f"initial conditions, {n} particles, {N}^3 mesh",
             "final_pos_err_cells_rms": pos_rms, "final_pos_err_cells_max": pos_max,
-            "pk_max_rel_diff_k_lt_half_nyquist": float(r.max()),
+            "pk_max_rel_diff_k_lt_half_nyquist": max(rel),
+            "pk_final_fp32": {"k_mean": [b[0] for b in p32], "p_corrected": [b[1] for b in p32],
+                              "mode_count": [b[2] for b in p32],
+                              "estimator": "hy_pm_power_spectrum, binary64, 32 log shells to 0.5 k_Nyq, "
+                                           "interlaced, no shot-noise subtraction"},
             "run": f"{nsteps} KDK steps z={args.z_start:g}->0 in each precision"}
 
 

@@ -150,6 +150,8 @@ struct hy_pm_ctx {
     void* R3 = nullptr;
     void* kc = nullptr;          // k component per index
     void* kd = nullptr;          // k component per index, 0 on the Nyquist index
+    void* ic_kc = nullptr;       // binary64 k tables of the 1LPT displacements (allocated on first use)
+    void* ic_kd = nullptr;
     void* pk_aux = nullptr;      // interlaced partner spectrum (allocated on first use)
     double* pk_pos = nullptr;    // shifted positions, 3 * cap (allocated on first use)
     double* pk_tab = nullptr;    // k components, windows, alias factors (3N), shell edges
@@ -990,6 +992,64 @@ int hy_pm_force_meshes(hy_pm_ctx* c, const void* source_in, void* fx_out, void* 
         hip_check(hipMemcpyAsync(fy_out, c->R2, bytes, hipMemcpyDeviceToDevice, c->stream), "copy F_y");
         hip_check(hipMemcpyAsync(fz_out, c->R3, bytes, hipMemcpyDeviceToDevice, c->stream), "copy F_z");
         const auto st = fetch_status(c);
+        raise_errors(st.err);
+    });
+}
+
+int hy_ic_lpt(hy_pm_ctx* c, uint64_t n_side, const void* delta_k, double p_factor, double* x, double* y,
+              double* z, double* px, double* py, double* pz) {
+    return guarded([&] {
+        if (!c || !delta_k || !x || !y || !z || !px || !py || !pz) fail(HY_EINVAL, "hy_ic_lpt: null argument");
+        require_single_box(c, "hy_ic_lpt");
+        if (c->prec != HY_PREC_FP64) fail(HY_EINVAL, "hy_ic_lpt takes a binary64 context");
+        const uint64_t N = (uint64_t)c->N;
+        if (n_side == 0 || n_side > N || N % n_side != 0)
+            fail(HY_EINVAL, "1LPT particle lattice must divide the IC mesh");
+        if (!std::isfinite(p_factor)) fail(HY_EINVAL, "Invalid start-scaled LPT normalization");
+        set_device(c);
+        if (!c->ic_kc) {
+            std::vector<double> kc(N), kd(N);
+            for (uint64_t m = 0; m < N; ++m) {
+                const int64_t mode = m <= N / 2 ? (int64_t)m : (int64_t)m - (int64_t)N;
+                kc[m] = 2.0 * std::numbers::pi * (double)mode / c->geom.L;
+                kd[m] = (N % 2 == 0 && m == N / 2) ? 0.0 : kc[m];
+            }
+            c->ic_kc = c->alloc(N * sizeof(double));
+            c->ic_kd = c->alloc(N * sizeof(double));
+            hip_check(hipMemcpy(c->ic_kc, kc.data(), N * sizeof(double), hipMemcpyHostToDevice), "upload");
+            hip_check(hipMemcpy(c->ic_kd, kd.data(), N * sizeof(double), hipMemcpyHostToDevice), "upload");
+        }
+        reset_status(c);
+        const uint64_t NH = N / 2 + 1;
+        const double L = c->geom.L;
+        const double dxp = L / (double)n_side;
+        const double norm = 1.0 / ((double)N * (double)N * (double)N);
+        const uint64_t np = n_side * n_side * n_side;
+        hy::FftArgs<double> a = fft_args<double>(c);
+        const hy::FftKernels<double>& K = c->fft64;
+        double* pos[3] = {x, y, z};
+        double* mom[3] = {px, py, pz};
+        for (int axis = 0; axis < 3; ++axis) {
+            hipLaunchKernelGGL(hy::k_lpt_spectrum, dim3((uint32_t)blocks_for(NH * N * N, 256)), dim3(256), 0,
+                               c->stream, static_cast<const double2*>(delta_k), axis, (int)N,
+                               static_cast<const double*>(c->ic_kc), static_cast<const double*>(c->ic_kd), norm,
+                               static_cast<double2*>(c->B));
+            launch_check("k_lpt_spectrum");
+            const hy::FftPass<double>& P = K.x_inv;
+            a.kz0 = 0;
+            hipLaunchKernelGGL(P.fn, dim3((uint32_t)(NH * N / (uint64_t)P.rows), 1), dim3(P.threads), P.lds,
+                               c->stream, a);
+            launch_check("k_fft_x_dft (inverse)");
+            launch_pass(c, K, a, hy::FFT_Y_BWD, c->fft_var[hy::FFT_Y_BWD], 0, (int)NH);
+            launch_pass(c, K, a, hy::FFT_Z_C2R, c->fft_var[hy::FFT_Z_C2R], 0, (int)NH);
+            hipLaunchKernelGGL(hy::k_lpt_update, dim3((uint32_t)blocks_for(np, 256)), dim3(256), 0, c->stream,
+                               static_cast<const double*>(c->R), axis, (int)N, (int)n_side, (int)(N / n_side), dxp,
+                               L, p_factor, pos[axis], mom[axis], c->err);
+            launch_check("k_lpt_update");
+        }
+        const auto st = fetch_status(c);
+        if (st.err & (hy::ERR_UPDATE_NONFINITE | hy::ERR_FIELD_NONFINITE))
+            fail(HY_ERUNTIME, "1LPT particle update is non-finite or unrepresentable");
         raise_errors(st.err);
     });
 }

@@ -190,9 +190,10 @@ __global__ void __launch_bounds__((RowsCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) 
         tid, [&](int l) { return lds[(l / N) * RS + lds_slot(l % N)]; }, [&](int l, C v) { orow[l] = v; });
 }
 
-// x pass, forward only (spectral analysis): in place on B, unnormalised.
-template <typename T, int N, int V>
-__global__ void __launch_bounds__((RowsCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_fft_x_fwd(FftArgs<T> a) {
+// x pass without a Green multiply: forward (S = -1, spectral analysis) or
+// inverse (S = +1, IC displacements) transform, in place on B, unnormalised.
+template <typename T, int N, int V, int S>
+__global__ void __launch_bounds__((RowsCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_fft_x_dft(FftArgs<T> a) {
     fft_box(a);
     using C = cplx<T>;
     constexpr int NH = N / 2 + 1;
@@ -207,7 +208,7 @@ __global__ void __launch_bounds__((RowsCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) 
     staged_copy<ROWS * N, NT, C>(
         tid, [&](int l) { return rows[l]; }, [&](int l, C v) { lds[(l / N) * RS + lds_slot(l % N)] = v; });
     __syncthreads();
-    fft_rows<C, N, ROWS, RS, -1>(lds, a.tw, 1, tid);
+    fft_rows<C, N, ROWS, RS, S>(lds, a.tw, 1, tid);
     staged_copy<ROWS * N, NT, C>(
         tid, [&](int l) { return lds[(l / N) * RS + lds_slot(l % N)]; }, [&](int l, C v) { rows[l] = v; });
 }
@@ -496,9 +497,10 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
     }
 }
 
-// x pass, forward only (spectral analysis): in place on B, unnormalised.
-template <typename T, int N, int V>
-__global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_tl_x_fwd(FftArgs<T> a) {
+// x pass without a Green multiply: forward (S = -1, spectral analysis) or
+// inverse (S = +1, IC displacements) transform, in place on B, unnormalised.
+template <typename T, int N, int V, int S>
+__global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_tl_x_dft(FftArgs<T> a) {
     fft_box(a);
     using C = cplx<T>;
     constexpr int LEN = N, NH = N / 2 + 1;
@@ -516,8 +518,8 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
     if (t < B) {
 #pragma unroll
         for (int n1 = 0; n1 < A; ++n1) v[n1] = row[t + B * n1];
-        dft_reg<A, -1>(v);
-        tl_twiddle<A, LEN, -1>(v, t, a.tw, 1);
+        dft_reg<A, S>(v);
+        tl_twiddle<A, LEN, S>(v, t, a.tw, 1);
 #pragma unroll
         for (int k1 = 0; k1 < A; ++k1) xr[k1 * (B + 1) + t] = v[k1];
     }
@@ -525,7 +527,7 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
     if (t < A) {
 #pragma unroll
         for (int n2 = 0; n2 < B; ++n2) v[n2] = xr[t * (B + 1) + n2];
-        dft_reg<B, -1>(v);  // v[k2] = X[t + A k2]
+        dft_reg<B, S>(v);  // v[k2] = X[t + A k2]
 #pragma unroll
         for (int k2 = 0; k2 < B; ++k2) row[t + A * k2] = v[k2];
     }

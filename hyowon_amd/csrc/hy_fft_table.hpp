@@ -22,6 +22,7 @@ template <typename T>
 struct FftKernels {
     FftPass<T> pass[FFT_PASSES][kFftVariants];
     FftPass<T> x_fwd;  // forward-only x pass of the spectral estimators (in place on B)
+    FftPass<T> x_inv;  // inverse-only x pass of the IC displacements (in place on B)
     FftPass<T> sg[3];  // spectral-gradient x, y^-1 and z^-1 passes (TreePM long range; variant 0)
     int variants = 0;  // launch variants compiled for this size (variant 0 is the default)
 };
@@ -51,7 +52,8 @@ void fill_variant(FftKernels<T>& k) {
         k.pass[FFT_Y_BWD][V] = {k_tl_y_bwd<T, N, V>, RY::ROWS, RY::THREADS, RY::LDS_BYTES};
         k.pass[FFT_Z_C2R][V] = {k_tl_z_c2r<T, N, V>, RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES};
         if constexpr (V == 0) {
-            k.x_fwd = {k_tl_x_fwd<T, N, V>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
+            k.x_fwd = {k_tl_x_dft<T, N, V, -1>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
+            k.x_inv = {k_tl_x_dft<T, N, V, +1>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
             k.sg[0] = {k_tl_x_green<T, N, V, 1>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
             k.sg[1] = {k_tl_y_bwd<T, N, V, 1>, RY::ROWS, RY::THREADS, RY::LDS_BYTES};
             k.sg[2] = {k_tl_z_c2r<T, N, V, 1>, RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES};
@@ -66,7 +68,8 @@ void fill_variant(FftKernels<T>& k) {
         k.pass[FFT_Y_BWD][V] = {k_fft_y_bwd<T, N, V>, RY::ROWS, RY::THREADS, RY::LDS_BYTES};
         k.pass[FFT_Z_C2R][V] = {k_fft_z_c2r<T, N, V>, RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES};
         if constexpr (V == 0) {
-            k.x_fwd = {k_fft_x_fwd<T, N, V>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
+            k.x_fwd = {k_fft_x_dft<T, N, V, -1>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
+            k.x_inv = {k_fft_x_dft<T, N, V, +1>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
             k.sg[0] = {k_fft_x_green<T, N, V, 1>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
             k.sg[1] = {k_fft_y_bwd<T, N, V, 1>, RY::ROWS, RY::THREADS, RY::LDS_BYTES};
             k.sg[2] = {k_fft_z_c2r<T, N, V, 1>, RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES};

@@ -1171,6 +1171,57 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs<T> a) {
     if (bad) atomicOr(a.err, ERR_FORCE_NONFINITE);
 }
 
+// 1LPT displacement spectrum of one axis (src/ic/lpt_displacement.cpp:306-340):
+// psi_k = (-Im d, Re d) * k_d / k^2 with k_d = 0 on the Nyquist index of the
+// axis and psi_k = 0 where k^2 = 0, times 1/N^3 (the inverse transform's
+// normalisation), written into B [kz][ky][kx] from the r2c half spectrum
+// dk [x][y][kz] (binary64).
+__global__ void __launch_bounds__(256) k_lpt_spectrum(const double2* __restrict__ dk, int axis, int N,
+                                                      const double* __restrict__ kc, const double* __restrict__ kd,
+                                                      double norm, double2* __restrict__ B) {
+    const int NH = N / 2 + 1;
+    const uint64_t total = (uint64_t)NH * N * N;
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const int kx = (int)(e % (uint64_t)N), ky = (int)((e / (uint64_t)N) % (uint64_t)N), kz = (int)(e / ((uint64_t)N * N));
+    const double qx = kc[kx], qy = kc[ky], qz = kc[kz];
+    const double k2 = qx * qx + qy * qy + qz * qz;
+    double2 out = {0.0, 0.0};
+    if (k2 != 0.0) {
+        const double dkv = axis == 0 ? kd[kx] : (axis == 1 ? kd[ky] : kd[kz]);
+        const double f = dkv / k2;
+        const double2 d = dk[((uint64_t)kx * N + ky) * NH + kz];
+        out = {-d.y * f * norm, d.x * f * norm};
+    }
+    B[e] = out;
+}
+
+// 1LPT particle update of one axis (src/ic/lpt_displacement.cpp:99-129,
+// 341-384; lattice src/ic/initial_conditions.cpp:296-312): particle
+// (i, j, k) of the n^3 lattice takes psi at mesh cell (i s, j s, k s),
+// position wrap(q + psi) with q = (i, j or k) * dx, momentum p_factor * psi.
+__global__ void __launch_bounds__(256) k_lpt_update(const double* __restrict__ psi, int axis, int N, int n, int s,
+                                                    double dx, double L, double p_factor, double* __restrict__ pos,
+                                                    double* __restrict__ mom, uint32_t* __restrict__ err) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t total = (uint64_t)n * n * n;
+    if (p >= total) return;
+    const uint64_t i = p / ((uint64_t)n * n), j = (p / (uint64_t)n) % (uint64_t)n, k = p % (uint64_t)n;
+    const double v = psi[((i * s) * (uint64_t)N + j * s) * (uint64_t)N + k * s];
+    const uint64_t q = axis == 0 ? i : (axis == 1 ? j : k);
+    const double disp = 1.0 * v;
+    const double inc = p_factor * v;
+    const double cand = (double)q * dx + disp;
+    const double mcand = 0.0 + inc;
+    const double w = wrap_pos(cand, L);
+    if (!isfinite(v) || !isfinite(disp) || !isfinite(inc) || !isfinite(cand) || !isfinite(mcand) || !isfinite(w)) {
+        atomicOr(err, ERR_UPDATE_NONFINITE);
+        return;
+    }
+    pos[p] = w;
+    mom[p] = mcand;
+}
+
 // Transactional commit for the reference-shaped force entry: acc += g.
 template <typename T>
 __global__ void k_commit_force(uint64_t n, const T* __restrict__ gx, const T* __restrict__ gy,

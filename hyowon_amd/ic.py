@@ -187,6 +187,35 @@ def lpt_from_delta_k(delta_k, box: float, a_start: float, cosmo: Cosmology | Non
     return pos, mom
 
 
+def lpt_from_delta_k_hip(delta_k, box: float, a_start: float, cosmo: Cosmology | None = None,
+                         n_side: int | None = None):
+    """1LPT lattice ICs from delta_k on the GPU kernels (hy_ic_lpt): the
+    reference's first-order displacement (src/ic/lpt_displacement.cpp) on
+    libhyowon_pm's own FFT passes.  delta_k: complex128 CUDA tensor
+    [N][N][N/2+1]; n_side (default N) must divide N.  Returns (pos, mom) as
+    lists of three float64 tensors of n_side^3 particles."""
+    import torch
+    from ._lib import check, lib
+    from .pm import PMSolver
+
+    cosmo = cosmo or Cosmology()
+    N = delta_k.shape[0]
+    n = N if n_side is None else int(n_side)
+    _, f = cosmo.growth(a_start)
+    H = 100.0 * float(cosmo.E(a_start))
+    p_factor = a_start * a_start * H * f
+    dev = delta_k.device
+    dk = delta_k.to(torch.complex128).contiguous()
+    out = [torch.empty(n ** 3, dtype=torch.float64, device=dev) for _ in range(6)]
+    s = PMSolver(box, N, precision="fp64", max_particles=0, device=dev.index or 0)
+    try:
+        torch.cuda.synchronize(dev)
+        check(lib().hy_ic_lpt(s.handle, n, dk.data_ptr(), p_factor, *(o.data_ptr() for o in out)))
+    finally:
+        s.close()
+    return out[:3], out[3:]
+
+
 def lpt_ics(n_side: int, box: float, a_start: float, seed: int = 1,
             cosmo: Cosmology | None = None, device=None, rng: str = "torch",
             max_mode: int | None = None):
@@ -195,14 +224,14 @@ def lpt_ics(n_side: int, box: float, a_start: float, seed: int = 1,
     rng="torch": seeded torch white noise (fast synthetic ICs);
     rng="philox": the reference's mode-keyed Philox Gaussian field, drawn on
     the GPU by hy_ic_gaussian_modes, with the Cartesian band limit max_mode
-    (default n_side // 2)."""
+    (default n_side // 2), displaced by hy_ic_lpt."""
     import torch  # data generation only
 
     device = device or ("cuda" if torch.cuda.is_available() else "cpu")
     if rng == "philox":
         dev = torch.device(device)
         dk = philox_modes(n_side, box, a_start, seed, cosmo, max_mode=max_mode, device=dev.index or 0)
-        return lpt_from_delta_k(dk, box, a_start, cosmo)
+        return lpt_from_delta_k_hip(dk, box, a_start, cosmo)
     noise = white_noise(n_side, seed, device)
     return lpt_from_noise(noise, box, a_start, cosmo)
 

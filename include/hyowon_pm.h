@@ -285,6 +285,17 @@ int hy_ic_gaussian_modes(uint64_t N, uint64_t seed, uint64_t max_mode, const dou
                          void* out, uint32_t flags, int device, void* stream);
 const char* hy_ic_last_error(void);
 
+/* 1LPT lattice ICs (src/ic/lpt_displacement.cpp:202-384 first order, lattice
+ * src/ic/initial_conditions.cpp:296-312) on a binary64 context whose mesh is
+ * the IC mesh: per axis psi = c2r(i k_axis / k^2 delta_k) (k_axis = 0 on the
+ * Nyquist index, 1/N^3), then for particle (i, j, k) of the n_side^3 lattice
+ * (n_side divides N) x = wrap(q + psi(i s, j s, k s)), p = p_factor * psi with
+ * q = index * L / n_side and p_factor = a^2 H(a) f(a).  delta_k: device
+ * complex128 half spectrum [N][N][N/2+1] (hy_ic_gaussian_modes' layout);
+ * outputs: device binary64, n_side^3 each.  Synchronous. */
+int hy_ic_lpt(hy_pm_ctx* ctx, uint64_t n_side, const void* delta_k, double p_factor, double* x, double* y,
+              double* z, double* px, double* py, double* pz);
+
 /* Matter power spectrum on the context's mesh (N = the estimator's mesh_size,
  * L = the periodic domain's box size). */
 #define HY_PK_MAX_BINS 1024

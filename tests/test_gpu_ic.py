@@ -72,3 +72,49 @@ def test_philox_lpt_displacements_match_numpy(gpu):
     _, f = cosmo.growth(a)
     vel = a * a * 100.0 * float(cosmo.E(a)) * f
     assert np.allclose(mom[0].cpu().numpy(), vel * psi.reshape(-1), rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [16, 32, 48, 128])
+def test_hip_lpt_matches_numpy(gpu, n):
+    """hy_ic_lpt on the library's FFT passes: lattice displacements equal
+    numpy's inverse r2c transform of i k_x / k^2 delta_k (Nyquist index of the
+    derivative axis zeroed) on every axis, positions wrap(q + psi), momenta
+    a^2 H f psi."""
+    from hyowon_amd import ic
+    L, a = 2.0 * n, 1.0 / 50.0
+    dk = ic.philox_modes(n, L, a, seed=11)
+    pos, mom = ic.lpt_from_delta_k_hip(dk, L, a)
+    D = dk.cpu().numpy()
+    m = np.fft.fftfreq(n, 1.0 / n)
+    mz = np.arange(n // 2 + 1)
+    k1 = m * 2 * np.pi / L
+    kzz = mz * 2 * np.pi / L
+    k2 = k1[:, None, None] ** 2 + k1[None, :, None] ** 2 + kzz[None, None, :] ** 2
+    k2[0, 0, 0] = 1.0
+    cosmo = ic.Cosmology()
+    _, f = cosmo.growth(a)
+    vel = a * a * 100.0 * float(cosmo.E(a)) * f
+    q = np.arange(n) * (L / n)
+    for axis in range(3):
+        kd = [k1[:, None, None], k1[None, :, None], kzz[None, None, :]][axis]
+        kd = np.where(np.abs([m[:, None, None], m[None, :, None], mz[None, None, :]][axis]) == n // 2, 0.0, kd)
+        psi = np.fft.irfftn(1j * kd / k2 * D, s=(n, n, n), axes=(0, 1, 2))
+        qa = [q[:, None, None], q[None, :, None], q[None, None, :]][axis]
+        x_ref = np.mod(qa + psi, L).reshape(-1)
+        d = pos[axis].cpu().numpy() - x_ref
+        d -= L * np.round(d / L)
+        assert np.abs(d).max() < 1e-10 * L
+        assert np.allclose(mom[axis].cpu().numpy(), vel * psi.reshape(-1), rtol=1e-9, atol=1e-12 * np.abs(vel * psi).max())
+        assert (pos[axis] >= 0).all() and (pos[axis] < L).all()
+
+
+def test_hip_lpt_sub_sampled_lattice(gpu):
+    """A 16^3 lattice on a 32^3 IC mesh takes psi at every second cell."""
+    from hyowon_amd import ic
+    n, N, L, a = 16, 32, 40.0, 1.0 / 20.0
+    dk = ic.philox_modes(N, L, a, seed=4)
+    pos_full, _ = ic.lpt_from_delta_k_hip(dk, L, a)
+    pos_sub, _ = ic.lpt_from_delta_k_hip(dk, L, a, n_side=n)
+    for axis in range(3):
+        full = pos_full[axis].cpu().numpy().reshape(N, N, N)[::2, ::2, ::2].reshape(-1)
+        assert np.array_equal(pos_sub[axis].cpu().numpy(), full)

@@ -105,7 +105,11 @@ def philox_modes(n_side: int, box: float, a_start: float, seed: int, cosmo: Cosm
     from ._lib import check, lib
     var = torch.as_tensor(variance_by_m2(n_side, box, a_start, cosmo), device=f"cuda:{device}")
     out = torch.empty((n_side, n_side, n_side // 2 + 1), dtype=torch.complex128, device=f"cuda:{device}")
-    K = n_side // 2 if max_mode is None else int(max_mode)
+    # the particle-lattice band limit (src/ic/particle_lattice_bandlimit.cpp:39-80):
+    # |m_axis| <= K, by default and at most (n - 1) // 2 for an n^3 lattice on this mesh
+    K = (n_side - 1) // 2 if max_mode is None else int(max_mode)
+    if K > (n_side - 1) // 2:
+        raise ValueError("IC Fourier support exceeds the safe particle-lattice support")
     torch.cuda.synchronize(device)
     rc = lib().hy_ic_gaussian_modes(n_side, int(seed), K, var.data_ptr(), out.data_ptr(), 0, device, None)
     if rc != 0:
@@ -224,7 +228,7 @@ def lpt_ics(n_side: int, box: float, a_start: float, seed: int = 1,
     rng="torch": seeded torch white noise (fast synthetic ICs);
     rng="philox": the reference's mode-keyed Philox Gaussian field, drawn on
     the GPU by hy_ic_gaussian_modes, with the Cartesian band limit max_mode
-    (default n_side // 2), displaced by hy_ic_lpt."""
+    (default (n_side - 1) // 2, the reference's), displaced by hy_ic_lpt."""
     import torch  # data generation only
 
     device = device or ("cuda" if torch.cuda.is_available() else "cpu")

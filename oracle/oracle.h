@@ -7,7 +7,8 @@
  *
  * It restates, in plain C (binary64, -ffp-contract=off, OpenMP over independent
  * planes/lines), the reference's pure-PM force refresh and kick-drift-kick
- * update.  Every function cites the reference file:line it follows (paths are
+ * update, the mode-keyed Gaussian IC field, the matter power spectrum and the
+ * TreePM long-range PM force.  Every function cites the reference file:line it follows (paths are
  * relative to the reference source tree).
  *
  * Pinning (what makes each function trustworthy; see tests/test_oracle_*.py):
@@ -30,6 +31,22 @@
  *   or_time_boundaries / or_geometric_mean : reference TimeStepper via oracle/_ref.
  *   or_pm_force       : composition of the above; checked against the _ref
  *       composition (reference deposit/Green/gather + this FFT).
+ *   or_philox4x32_10 / or_uniform_pair / or_normal_pair : Random123
+ *       known-answer vectors and the reference's philox_rng.cpp via oracle/_ref,
+ *       bit-exact; or_gaussian_modes: Hermitian symmetry, band limit and
+ *       per-mode variance statistics.
+ *   or_pk_edges / or_pk_bin_index / or_pk_window_axis / or_pk_raw_power /
+ *   or_pk_raw_power_minus : the reference's header-only spectral helpers via
+ *       oracle/_ref, bit-exact (the subtraction's exact branch included).
+ *   or_power_spectrum : closed forms (alternating planes, period-4 pattern
+ *       with its interlacing partner), brute-force shell counts and mean |k|,
+ *       mass-scale and translation invariance, the Poisson expectation of the
+ *       aliased shot noise; the normalised deposit via oracle/_ref.
+ *   or_green_apply_treepm / or_interpolate_add : reference GreenFunction
+ *       (TreePM branch) and CICMassAssignment::interpolate_add via oracle/_ref,
+ *       bit-exact.
+ *   or_pm_force_treepm : the Gaussian-filtered Newtonian pair force (2%) and
+ *       momentum conservation.
  */
 #ifndef HYOWON_ORACLE_H
 #define HYOWON_ORACLE_H

@@ -46,6 +46,41 @@ __device__ __forceinline__ C mul_mi(C v, T k) {
     return C{v.y * k, -v.x * k};
 }
 
+// a / b: binary64 exactly; binary32 as a product with the hardware
+// reciprocal (within 1 ulp; the binary32 solve carries no bit-for-bit
+// contract), two instructions instead of the correctly rounded division.
+template <typename T>
+__device__ __forceinline__ T green_div(T a, T b) {
+    if constexpr (sizeof(T) == 4) return a * __builtin_amdgcn_rcpf(b);
+    else return a / b;
+}
+
+// TreePM long range (src/mesh/green_function.cpp:165-203): continuum
+// 4 pi G / (-k^2), the Gaussian split exp(-k^2 r_s^2), CIC deconvolution.
+template <typename T>
+__device__ __forceinline__ T green_factor_treepm(const FftArgs<T>& a, int kx, int ky, int kz) {
+    if (kx == 0 && ky == 0 && kz == 0) return (T)0;
+    const T qx = a.kc[kx], qy = a.kc[ky], qz = a.kc[kz];
+    const T k2 = qx * qx + qy * qy + qz * qz;
+    T ff = green_div((T)a.green_num, -k2);
+    ff *= exp(-k2 * (T)a.rs * (T)a.rs);
+    const T w = a.green_win[kx] * a.green_win[ky] * a.green_win[kz];
+    ff = green_div(ff, w * w);
+    return ff * (T)a.norm;
+}
+
+template <typename T>
+__device__ __forceinline__ T green_factor(const FftArgs<T>& a, int kx, int ky, int kz) {
+    if (kx == 0 && ky == 0 && kz == 0) return (T)0;
+    const T D = a.green_lap[kx] + a.green_lap[ky] + a.green_lap[kz];
+    T ff = green_div((T)a.green_num, D);
+    if (a.deconvolve) {
+        const T w = a.green_win[kx] * a.green_win[ky] * a.green_win[kz];
+        ff = green_div(ff, w * w);
+    }
+    return ff * (T)a.norm;
+}
+
 // Boxes of one geometry are laid out box-major; grid row blockIdx.y is the box.
 template <typename T>
 __device__ __forceinline__ void fft_box(FftArgs<T>& a) {
@@ -169,13 +204,7 @@ __global__ void __launch_bounds__((RowsCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) 
         } else if (kx == 0 && ky == 0 && kz == 0) {
             f = (T)0;
         } else {
-            const T D = a.green_lap[kx] + a.green_lap[ky] + a.green_lap[kz];
-            T ff = (T)a.green_num / D;
-            if (a.deconvolve) {
-                const T w = a.green_win[kx] * a.green_win[ky] * a.green_win[kz];
-                ff /= (w * w);
-            }
-            f = ff * (T)a.norm;
+            f = green_factor(a, kx, ky, kz);
         }
         C v = lds[r * RS + lds_slot(kx)];
         v.x *= f;
@@ -312,31 +341,6 @@ __global__ void __launch_bounds__((RowsCfg<T, N / 2, N, V>::THREADS)) k_fft_z_c2
 // exchange, and a length-B DFT per row written straight back (or through a
 // transposing LDS tile where the pass changes the memory layout).
 // ===========================================================================
-// TreePM long range (src/mesh/green_function.cpp:165-203): continuum
-// 4 pi G / (-k^2), the Gaussian split exp(-k^2 r_s^2), CIC deconvolution.
-template <typename T>
-__device__ __forceinline__ T green_factor_treepm(const FftArgs<T>& a, int kx, int ky, int kz) {
-    if (kx == 0 && ky == 0 && kz == 0) return (T)0;
-    const T qx = a.kc[kx], qy = a.kc[ky], qz = a.kc[kz];
-    const T k2 = qx * qx + qy * qy + qz * qz;
-    T ff = (T)a.green_num / (-k2);
-    ff *= exp(-k2 * (T)a.rs * (T)a.rs);
-    const T w = a.green_win[kx] * a.green_win[ky] * a.green_win[kz];
-    ff /= (w * w);
-    return ff * (T)a.norm;
-}
-
-template <typename T>
-__device__ __forceinline__ T green_factor(const FftArgs<T>& a, int kx, int ky, int kz) {
-    if (kx == 0 && ky == 0 && kz == 0) return (T)0;
-    const T D = a.green_lap[kx] + a.green_lap[ky] + a.green_lap[kz];
-    T ff = (T)a.green_num / D;
-    if (a.deconvolve) {
-        const T w = a.green_win[kx] * a.green_win[ky] * a.green_win[kz];
-        ff /= (w * w);
-    }
-    return ff * (T)a.norm;
-}
 
 // z pass: real rows (as LEN = N/2 complex) -> half spectrum, transposed into A.
 template <typename T, int N, int V>

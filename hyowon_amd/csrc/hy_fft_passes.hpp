@@ -40,6 +40,17 @@ struct FftArgs {
     int deriv;           // 0, or 1 / 2 / 3: multiply by -i k_x / k_y / k_z in the x / y^-1 / z^-1 pass
 };
 
+// Streaming store of a complex value (non-temporal).  Used for the passes'
+// contiguous row outputs, which nothing re-reads before the next pass; the
+// transposing stores keep the default policy (their partial lines merge in L2).
+template <typename C>
+__device__ __forceinline__ void st_stream(C* p, C v) {
+    using E = decltype(v.x);
+    typedef E V2 __attribute__((ext_vector_type(2)));
+    const V2 w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<V2*>(p));
+}
+
 // -i k v
 template <typename C, typename T>
 __device__ __forceinline__ C mul_mi(C v, T k) {
@@ -497,7 +508,7 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
         dft_reg<A, +1>(v);
         C* orow = (SG && a.xout) ? a.xout + q * N : row;
 #pragma unroll
-        for (int k2 = 0; k2 < A; ++k2) orow[t + B * k2] = v[k2];
+        for (int k2 = 0; k2 < A; ++k2) st_stream(&orow[t + B * k2], v[k2]);
     }
 }
 
@@ -533,7 +544,7 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
         for (int n2 = 0; n2 < B; ++n2) v[n2] = xr[t * (B + 1) + n2];
         dft_reg<B, S>(v);  // v[k2] = X[t + A k2]
 #pragma unroll
-        for (int k2 = 0; k2 < B; ++k2) row[t + A * k2] = v[k2];
+        for (int k2 = 0; k2 < B; ++k2) st_stream(&row[t + A * k2], v[k2]);
     }
 }
 
@@ -584,7 +595,7 @@ __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_bwd(FftAr
         dft_reg<B, +1>(v);
         C* dst = a.A + ((uint64_t)(x0 + r) * NH + kz) * N;
 #pragma unroll
-        for (int k2 = 0; k2 < B; ++k2) dst[t + A * k2] = v[k2];
+        for (int k2 = 0; k2 < B; ++k2) st_stream(&dst[t + A * k2], v[k2]);
     }
 }
 
@@ -650,7 +661,7 @@ __global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_c2r(F
 #pragma unroll
         for (int k2 = 0; k2 < B; ++k2) {
             if (!isfinite(v[k2].x) || !isfinite(v[k2].y)) mybad = 1;
-            dst[t + A * k2] = v[k2];
+            st_stream(&dst[t + A * k2], v[k2]);
         }
     }
     if (__ballot(mybad) && lane_id() == 0) atomicOr(a.err, ERR_FIELD_NONFINITE);

@@ -908,13 +908,14 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
         double fx = 0.0, fy = 0.0, fz = 0.0;
         // the whole particle state is loaded up front: one memory round trip
         // for it, then one for the potential stencil
-        const double xp = a.x_in[p], yp = a.y_in[p], zp = a.z_in[p];
+        const double xp = __builtin_nontemporal_load(&a.x_in[p]), yp = __builtin_nontemporal_load(&a.y_in[p]),
+                     zp = __builtin_nontemporal_load(&a.z_in[p]);
         const bool kick = a.has_prev || a.has_next;
         PT qx0 = 0, qy0 = 0, qz0 = 0;
         if (kick) {
-            qx0 = a.px_in[p];
-            qy0 = a.py_in[p];
-            qz0 = a.pz_in[p];
+            qx0 = __builtin_nontemporal_load(&a.px_in[p]);
+            qy0 = __builtin_nontemporal_load(&a.py_in[p]);
+            qz0 = __builtin_nontemporal_load(&a.pz_in[p]);
         }
         if (!(cic_axis(xp, g, ix, fx) && cic_axis(yp, g, iy, fy) && cic_axis(zp, g, iz, fz))) {
             fail = ERR_POSITION_NONFINITE;
@@ -1052,12 +1053,14 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
                             || !isfinite(ny) || !isfinite(nz)) {
                             fail = ERR_UPDATE_NONFINITE;
                         } else {
-                            a.px_out[p] = qx;
-                            a.py_out[p] = qy;
-                            a.pz_out[p] = qz;
-                            a.x_out[p] = nx;
-                            a.y_out[p] = ny;
-                            a.z_out[p] = nz;
+                            // the particle streams are read and written once per step:
+                            // non-temporal, so they do not evict the potential stencil
+                            __builtin_nontemporal_store(qx, &a.px_out[p]);
+                            __builtin_nontemporal_store(qy, &a.py_out[p]);
+                            __builtin_nontemporal_store(qz, &a.pz_out[p]);
+                            __builtin_nontemporal_store(nx, &a.x_out[p]);
+                            __builtin_nontemporal_store(ny, &a.y_out[p]);
+                            __builtin_nontemporal_store(nz, &a.z_out[p]);
                             if (a.next_counts) {
                                 // bucket of the drifted position for the next deposit
                                 int bx, by, bz;
@@ -1076,8 +1079,8 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
     if (a.next_counts) {  // uniform across the grid: every lane reaches the ballots
         const uint32_t rank = aggregated_rank(a.next_counts, next_key, live && key_ok);
         if (live) {
-            a.next_keys[p] = key_ok ? next_key : 0xffffffffu;
-            a.next_ranks[p] = rank;
+            __builtin_nontemporal_store(key_ok ? next_key : 0xffffffffu, &a.next_keys[p]);
+            __builtin_nontemporal_store(rank, &a.next_ranks[p]);
         }
     }
 }

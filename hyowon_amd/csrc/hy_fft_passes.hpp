@@ -51,6 +51,16 @@ __device__ __forceinline__ void st_stream(C* p, C v) {
     __builtin_nontemporal_store(w, reinterpret_cast<V2*>(p));
 }
 
+// Streaming load of a complex value (non-temporal), for the x passes' row
+// inputs (on the z and y passes it measured slower).
+template <typename C>
+__device__ __forceinline__ C ld_stream(const C* p) {
+    using E = decltype(p->x);
+    typedef E V2 __attribute__((ext_vector_type(2)));
+    const V2 w = __builtin_nontemporal_load(reinterpret_cast<const V2*>(p));
+    return C{w.x, w.y};
+}
+
 // -i k v
 template <typename C, typename T>
 __device__ __forceinline__ C mul_mi(C v, T k) {
@@ -474,7 +484,7 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
     C v[TL::EMAX];
     if (t < B) {
 #pragma unroll
-        for (int n1 = 0; n1 < A; ++n1) v[n1] = row[t + B * n1];
+        for (int n1 = 0; n1 < A; ++n1) v[n1] = ld_stream(&row[t + B * n1]);
         dft_reg<A, -1>(v);
         tl_twiddle<A, LEN, -1>(v, t, a.tw, 1);
 #pragma unroll
@@ -532,7 +542,7 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
     C v[TL::EMAX];
     if (t < B) {
 #pragma unroll
-        for (int n1 = 0; n1 < A; ++n1) v[n1] = row[t + B * n1];
+        for (int n1 = 0; n1 < A; ++n1) v[n1] = ld_stream(&row[t + B * n1]);
         dft_reg<A, S>(v);
         tl_twiddle<A, LEN, S>(v, t, a.tw, 1);
 #pragma unroll

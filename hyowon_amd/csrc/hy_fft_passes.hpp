@@ -76,30 +76,41 @@ __device__ __forceinline__ T green_div(T a, T b) {
     else return a / b;
 }
 
-// TreePM long range (src/mesh/green_function.cpp:165-203): continuum
-// 4 pi G / (-k^2), the Gaussian split exp(-k^2 r_s^2), CIC deconvolution.
+// Green factor from per-axis table values: p = the discrete Laplacian
+// eigenvalue per axis (pure PM, src/mesh/green_function.cpp) or the wave
+// number k per axis (SG: TreePM long range, src/mesh/green_function.cpp:
+// 165-203: continuum 4 pi G / (-k^2), the Gaussian split exp(-k^2 r_s^2), CIC
+// deconvolution), w = the per-axis CIC window.  The caller handles k = 0.
+template <typename T, int SG>
+__device__ __forceinline__ T green_axes(const FftArgs<T>& a, T px, T py, T pz, T wx, T wy, T wz) {
+    if constexpr (SG) {
+        const T k2 = px * px + py * py + pz * pz;
+        T ff = green_div((T)a.green_num, -k2);
+        ff *= exp(-k2 * (T)a.rs * (T)a.rs);
+        const T w = wx * wy * wz;
+        ff = green_div(ff, w * w);
+        return ff * (T)a.norm;
+    } else {
+        T ff = green_div((T)a.green_num, px + py + pz);
+        if (a.deconvolve) {
+            const T w = wx * wy * wz;
+            ff = green_div(ff, w * w);
+        }
+        return ff * (T)a.norm;
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ T green_factor_treepm(const FftArgs<T>& a, int kx, int ky, int kz) {
     if (kx == 0 && ky == 0 && kz == 0) return (T)0;
-    const T qx = a.kc[kx], qy = a.kc[ky], qz = a.kc[kz];
-    const T k2 = qx * qx + qy * qy + qz * qz;
-    T ff = green_div((T)a.green_num, -k2);
-    ff *= exp(-k2 * (T)a.rs * (T)a.rs);
-    const T w = a.green_win[kx] * a.green_win[ky] * a.green_win[kz];
-    ff = green_div(ff, w * w);
-    return ff * (T)a.norm;
+    return green_axes<T, 1>(a, a.kc[kx], a.kc[ky], a.kc[kz], a.green_win[kx], a.green_win[ky], a.green_win[kz]);
 }
 
 template <typename T>
 __device__ __forceinline__ T green_factor(const FftArgs<T>& a, int kx, int ky, int kz) {
     if (kx == 0 && ky == 0 && kz == 0) return (T)0;
-    const T D = a.green_lap[kx] + a.green_lap[ky] + a.green_lap[kz];
-    T ff = green_div((T)a.green_num, D);
-    if (a.deconvolve) {
-        const T w = a.green_win[kx] * a.green_win[ky] * a.green_win[kz];
-        ff = green_div(ff, w * w);
-    }
-    return ff * (T)a.norm;
+    return green_axes<T, 0>(a, a.green_lap[kx], a.green_lap[ky], a.green_lap[kz], a.green_win[kx],
+                            a.green_win[ky], a.green_win[kz]);
 }
 
 // Boxes of one geometry are laid out box-major; grid row blockIdx.y is the box.
@@ -481,10 +492,39 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
     const uint64_t q = (uint64_t)a.kz0 * N + (uint64_t)b * ROWS + r;  // row q = kz*N + ky
     const int kz = (int)(q / N), ky = (int)(q % N);
     C* row = a.B + q * N;
+    // The Green factors of this thread's spectrum column X[t + A k2] depend on
+    // the row alone: their table loads go out first, the row loads right
+    // behind them, so the table latency hides under the row's.  (The TreePM
+    // factor, with its exponential, is formed in place after the forward
+    // transform instead: held across it, it would halve the occupancy.)
+    const T* ptab = SG ? a.kc : a.green_lap;
+    T px[B], wx[B], py = (T)0, pz = (T)0, wy = (T)0, wz = (T)0;
+    if (!SG && t < A) {
+        py = ptab[ky];
+        pz = ptab[kz];
+        wy = a.green_win[ky];
+        wz = a.green_win[kz];
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) {
+            px[k2] = ptab[t + A * k2];
+            wx[k2] = a.green_win[t + A * k2];
+        }
+    }
     C v[TL::EMAX];
     if (t < B) {
 #pragma unroll
         for (int n1 = 0; n1 < A; ++n1) v[n1] = ld_stream(&row[t + B * n1]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    T f[B];
+    if (!SG && t < A) {
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) {
+            const int kx = t + A * k2;
+            f[k2] = (kx == 0 && ky == 0 && kz == 0) ? (T)0 : green_axes<T, 0>(a, px[k2], py, pz, wx[k2], wy, wz);
+        }
+    }
+    if (t < B) {
         dft_reg<A, -1>(v);
         tl_twiddle<A, LEN, -1>(v, t, a.tw, 1);
 #pragma unroll
@@ -500,9 +540,9 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
         dft_reg<B, -1>(v);  // v[k2] = X[t + A k2]
 #pragma unroll
         for (int k2 = 0; k2 < B; ++k2) {
-            const T f = SG ? green_factor_treepm(a, t + A * k2, ky, kz) : green_factor(a, t + A * k2, ky, kz);
-            v[k2].x *= f;
-            v[k2].y *= f;
+            if constexpr (SG) f[k2] = green_factor_treepm(a, t + A * k2, ky, kz);
+            v[k2].x *= f[k2];
+            v[k2].y *= f[k2];
             if (SG && a.deriv == 1) v[k2] = mul_mi(v[k2], a.kd[t + A * k2]);
         }
         // backward transform of the held stride-A column (A and B exchanged)

@@ -239,6 +239,25 @@ struct TwoLevel {
 
 // Level-1 twiddle: v[k1] *= W_LEN^(S col k1), twiddles from the table
 // tw[m * stride] = exp(-2 pi i m / LEN).
+// Level-1 twiddles split into the loads, issued with a pass's row loads so
+// their latency hides under the row fetch, and the multiplies, which match
+// tl_twiddle's arithmetic.
+template <int A, int LEN, typename C>
+HY_HD void tl_twiddle_load(C* w, int col, const C* tw, int stride) {
+#pragma unroll
+    for (int k1 = 1; k1 < A; ++k1) w[k1] = tw[((col * k1) % LEN) * stride];
+}
+
+template <int A, int S, typename C>
+HY_HD void tl_twiddle_apply(C* v, const C* w) {
+#pragma unroll
+    for (int k1 = 1; k1 < A; ++k1) {
+        C x = w[k1];
+        if constexpr (S > 0) x.y = -x.y;
+        v[k1] = cmul(v[k1], x);
+    }
+}
+
 // TB > 0: the table loads go out in batches of TB ahead of their multiplies
 // (a scheduling barrier keeps the compiler from interleaving them with the
 // uses), so a row pays (A - 1) / TB memory round trips for its twiddles rather

@@ -392,12 +392,16 @@ __global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_r2c(F
     const int yb = b % (N / ROWS), x = b / (N / ROWS);
     const int y0 = yb * ROWS;
     const C* zrow = reinterpret_cast<const C*>(a.R + ((uint64_t)x * N + y0 + r) * N);
-    C v[TL::EMAX];
+    C v[TL::EMAX], w[A];
     if (t < B) {
 #pragma unroll
         for (int n1 = 0; n1 < A; ++n1) v[n1] = zrow[t + B * n1];
+        tl_twiddle_load<A, LEN>(w, t, a.tw, 2);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (t < B) {
         dft_reg<A, -1>(v);
-        tl_twiddle<A, LEN, -1>(v, t, a.tw, 2);
+        tl_twiddle_apply<A, -1>(v, w);
 #pragma unroll
         for (int k1 = 0; k1 < A; ++k1) xr[k1 * (B + 1) + t] = v[k1];
     }
@@ -447,12 +451,16 @@ __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_fwd(FftAr
     const int xb = b % (N / ROWS), kz = a.kz0 + (int)(b / (N / ROWS));
     const int x0 = xb * ROWS;
     const C* src = a.A + ((uint64_t)(x0 + r) * NH + kz) * N;
-    C v[TL::EMAX];
+    C v[TL::EMAX], w[A];
     if (t < B) {
 #pragma unroll
         for (int n1 = 0; n1 < A; ++n1) v[n1] = src[t + B * n1];
+        tl_twiddle_load<A, LEN>(w, t, a.tw, 1);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (t < B) {
         dft_reg<A, -1>(v);
-        tl_twiddle<A, LEN, -1>(v, t, a.tw, 1);
+        tl_twiddle_apply<A, -1>(v, w);
 #pragma unroll
         for (int k1 = 0; k1 < A; ++k1) xr[k1 * (B + 1) + t] = v[k1];
     }
@@ -617,6 +625,8 @@ __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_bwd(FftAr
     const int xb = b % (N / ROWS), kz = a.kz0 + (int)(b / (N / ROWS));
     const int x0 = xb * ROWS;
     const C* src = a.B + (uint64_t)kz * N * N + x0;
+    C w[A];
+    if (t < B) tl_twiddle_load<A, LEN>(w, t, a.tw, 1);
     staged_copy<ROWS * N, NT, C>(
         tid,
         [&](int l) {
@@ -634,7 +644,7 @@ __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_bwd(FftAr
     __syncthreads();
     if (t < B) {
         dft_reg<A, +1>(v);
-        tl_twiddle<A, LEN, +1>(v, t, a.tw, 1);
+        tl_twiddle_apply<A, +1>(v, w);
 #pragma unroll
         for (int k1 = 0; k1 < A; ++k1) xr[k1 * (B + 1) + t] = v[k1];
     }
@@ -668,6 +678,14 @@ __global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_c2r(F
     const int yb = b % (N / ROWS), x = b / (N / ROWS);
     const int y0 = yb * ROWS;
     const C* src = a.A + (uint64_t)x * NH * N + y0;
+    // the untangle twiddles W^k of this thread's column (k = t + B n1) and its
+    // level-1 twiddles, in flight with the row loads
+    C wu[A], w[A];
+    if (t < B) {
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) wu[n1] = a.tw[t + B * n1];
+        tl_twiddle_load<A, LEN>(w, t, a.tw, 2);
+    }
     staged_copy<ROWS * NH, NT, C>(
         tid,
         [&](int l) {
@@ -690,14 +708,14 @@ __global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_c2r(F
             const C xk = lds[r * NAT + k];
             const C xm = lds[r * NAT + LEN - k];
             const C s1 = cadd(xk, cconj(xm));
-            const C d1 = cmul(csub(xk, cconj(xm)), cconj(a.tw[k]));
+            const C d1 = cmul(csub(xk, cconj(xm)), cconj(wu[n1]));
             v[n1] = C{s1.x - d1.y, s1.y + d1.x};
         }
     }
     __syncthreads();
     if (t < B) {
         dft_reg<A, +1>(v);
-        tl_twiddle<A, LEN, +1>(v, t, a.tw, 2);
+        tl_twiddle_apply<A, +1>(v, w);
 #pragma unroll
         for (int k1 = 0; k1 < A; ++k1) xr[k1 * (B + 1) + t] = v[k1];
     }

@@ -87,6 +87,17 @@ __device__ __forceinline__ bool cic_axis(double pos, const Geom& g, int& base, d
     return true;
 }
 
+// cic_axis over the three axes of a position already loaded, all three
+// evaluated (no short circuit), so a caller's three coordinate loads go out
+// together instead of each waiting on the previous axis's check.
+__device__ __forceinline__ bool cic_axes(double px, double py, double pz, const Geom& g, int& bx, int& by, int& bz,
+                                         double& fx, double& fy, double& fz) {
+    const bool ox = cic_axis(px, g, bx, fx);
+    const bool oy = cic_axis(py, g, by, fy);
+    const bool oz = cic_axis(pz, g, bz, fz);
+    return ox && oy && oz;
+}
+
 __device__ __forceinline__ int wrap_cell(int i, int N) {
     return i < 0 ? i + N : (i >= N ? i - N : i);
 }

@@ -416,21 +416,37 @@ __global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_r2c(F
 #pragma unroll
         for (int k2 = 0; k2 < B; ++k2) lds[r * NAT + t + A * k2] = v[k2];
     }
+    // untangle: X[k] = E + W^k O, E = (Z[k]+conj Z[L-k])/2, O = -i/2 (Z[k]-conj Z[L-k]);
+    // thread tid forms the outputs l = tid + i NT (k = l / ROWS), its twiddles
+    // W^k loaded ahead of the barrier
+    constexpr int COUNT = ROWS * NH, PER = (COUNT + NT - 1) / NT;
+    C wk[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int l = tid + i * NT;
+        if (COUNT % NT == 0 || l < COUNT) wk[i] = a.tw[l / ROWS];
+    }
     __syncthreads();
-    // untangle: X[k] = E + W^k O, E = (Z[k]+conj Z[L-k])/2, O = -i/2 (Z[k]-conj Z[L-k])
     C* dst = a.A + (uint64_t)x * NH * N + y0;
-    staged_copy<ROWS * NH, NT, C>(
-        tid,
-        [&](int l) {
+    C out[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int l = tid + i * NT;
+        if (COUNT % NT == 0 || l < COUNT) {
             const int rr = l % ROWS, k = l / ROWS;
             const C zk = lds[rr * NAT + (k % LEN)];
             const C zc = cconj(lds[rr * NAT + (LEN - k) % LEN]);
             const C e = {(T)0.5 * (zk.x + zc.x), (T)0.5 * (zk.y + zc.y)};
             const C d = csub(zk, zc);
             const C o = {(T)0.5 * d.y, (T)-0.5 * d.x};
-            return cadd(e, cmul(a.tw[k], o));
-        },
-        [&](int l, C val) { dst[(uint64_t)(l / ROWS) * N + (l % ROWS)] = val; });
+            out[i] = cadd(e, cmul(wk[i], o));
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int l = tid + i * NT;
+        if (COUNT % NT == 0 || l < COUNT) dst[(uint64_t)(l / ROWS) * N + (l % ROWS)] = out[i];
+    }
 }
 
 // y pass forward: A rows over y -> B[kz][ky][x] through a transposing tile.

@@ -569,9 +569,13 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
             v[k2].y *= f[k2];
             if (SG && a.deriv == 1) v[k2] = mul_mi(v[k2], a.kd[t + A * k2]);
         }
-        // backward transform of the held stride-A column (A and B exchanged)
+        // backward transform of the held stride-A column (A and B exchanged);
+        // its level-1 twiddles load while the length-B DFT runs
+        C w2[B];
+        tl_twiddle_load<B, LEN>(w2, t, a.tw, 1);
+        __builtin_amdgcn_sched_barrier(0);
         dft_reg<B, +1>(v);
-        tl_twiddle<B, LEN, +1>(v, t, a.tw, 1);
+        tl_twiddle_apply<B, +1>(v, w2);
 #pragma unroll
         for (int k1 = 0; k1 < B; ++k1) xr[k1 * (A + 1) + t] = v[k1];
     }

@@ -484,17 +484,17 @@ __device__ __forceinline__ void build_cell_starts(KP keys, uint32_t M, uint32_t*
 
 // Source ranges of a destination tile: range 0 is the own tile (all eight
 // classes, contiguous); ranges 1..19 are the boundary classes c of the seven
-// lower neighbour tiles at offset d in {0,1}^3 with (c & d) == d.
+// lower neighbour tiles at offset d in {0,1}^3 with (c & d) == d, in the order
+//   (0,0) (1,1) (1,3) (1,5) (1,7) (2,2) (2,3) (2,6) (2,7) (3,3) (3,7)
+//   (4,4) (4,5) (4,6) (4,7) (5,5) (5,7) (6,6) (6,7) (7,7).
+// The d and c of range t are 3-bit fields t of two immediates (a per-lane
+// table lookup would cost each tile a memory round trip before its offsets).
+constexpr uint64_t kRangeD = 0xfb5b248da491248ull;
+constexpr uint64_t kRangeC = 0xff7bf59dfcd7ac8ull;
 struct RangeSlot { uint8_t d, c; };
-__constant__ const RangeSlot kRangeSlots[20] = {
-    {0, 0},
-    {1, 1}, {1, 3}, {1, 5}, {1, 7},
-    {2, 2}, {2, 3}, {2, 6}, {2, 7},
-    {3, 3}, {3, 7},
-    {4, 4}, {4, 5}, {4, 6}, {4, 7},
-    {5, 5}, {5, 7},
-    {6, 6}, {6, 7},
-    {7, 7}};
+__device__ __forceinline__ RangeSlot range_slot(int t) {
+    return RangeSlot{(uint8_t)((kRangeD >> (3 * t)) & 7u), (uint8_t)((kRangeC >> (3 * t)) & 7u)};
+}
 
 // Fills rs (start in order[]), rc (prefix of counts, rc[20] = total), rd (d),
 // rcls (class) with one global round trip per range, in parallel.
@@ -505,7 +505,7 @@ __device__ __forceinline__ void load_source_ranges(const uint32_t* __restrict__ 
     if (t < 64) {  // the first wave: one lane per range, then a shuffle scan of the counts
         uint32_t cnt = 0;
         if (t < 20) {
-            const RangeSlot sl = kRangeSlots[t];
+            const RangeSlot sl = range_slot(t);
             const int d = sl.d;
             const int sx = (tx - ((d >> 2) & 1) + nT) % nT, sy = (ty - ((d >> 1) & 1) + nT) % nT,
                       sz = (tz - (d & 1) + nT) % nT;

@@ -258,38 +258,28 @@ HY_HD void tl_twiddle_apply(C* v, const C* w) {
     }
 }
 
-// TB > 0: the table loads go out in batches of TB ahead of their multiplies
-// (a scheduling barrier keeps the compiler from interleaving them with the
+// The table loads go out in batches of TB ahead of their multiplies (a
+// scheduling barrier keeps the compiler from interleaving them with the
 // uses), so a row pays (A - 1) / TB memory round trips for its twiddles rather
-// than up to one per twiddle.  TB = 0 leaves the schedule to the compiler
-// (for kernels already at their register limit).
+// than up to one per twiddle.
 template <int A, int LEN, int S, int TB = 16, typename C>
 HY_HD void tl_twiddle(C* v, int col, const C* tw, int stride) {
-    if constexpr (TB == 0) {
 #pragma unroll
-        for (int k1 = 1; k1 < A; ++k1) {
-            C w = tw[((col * k1) % LEN) * stride];
-            if constexpr (S > 0) w.y = -w.y;
-            v[k1] = cmul(v[k1], w);
+    for (int k0 = 1; k0 < A; k0 += TB) {
+        C w[TB];
+#pragma unroll
+        for (int j = 0; j < TB; ++j) {
+            if (k0 + j < A) w[j] = tw[((col * (k0 + j)) % LEN) * stride];
         }
-    } else {
-#pragma unroll
-        for (int k0 = 1; k0 < A; k0 += TB) {
-            C w[TB];
-#pragma unroll
-            for (int j = 0; j < TB; ++j) {
-                if (k0 + j < A) w[j] = tw[((col * (k0 + j)) % LEN) * stride];
-            }
 #ifdef __HIP_DEVICE_COMPILE__
-            __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_sched_barrier(0);
 #endif
 #pragma unroll
-            for (int j = 0; j < TB; ++j) {
-                if (k0 + j < A) {
-                    C x = w[j];
-                    if constexpr (S > 0) x.y = -x.y;
-                    v[k0 + j] = cmul(v[k0 + j], x);
-                }
+        for (int j = 0; j < TB; ++j) {
+            if (k0 + j < A) {
+                C x = w[j];
+                if constexpr (S > 0) x.y = -x.y;
+                v[k0 + j] = cmul(v[k0 + j], x);
             }
         }
     }

@@ -607,12 +607,16 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const uint64_t q = (uint64_t)a.kz0 * N + (uint64_t)b * ROWS + r;
     C* row = a.B + q * N;
-    C v[TL::EMAX];
+    C v[TL::EMAX], w[A];
     if (t < B) {
 #pragma unroll
         for (int n1 = 0; n1 < A; ++n1) v[n1] = ld_stream(&row[t + B * n1]);
+        tl_twiddle_load<A, LEN>(w, t, a.tw, 1);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (t < B) {
         dft_reg<A, S>(v);
-        tl_twiddle<A, LEN, S>(v, t, a.tw, 1);
+        tl_twiddle_apply<A, S>(v, w);
 #pragma unroll
         for (int k1 = 0; k1 < A; ++k1) xr[k1 * (B + 1) + t] = v[k1];
     }

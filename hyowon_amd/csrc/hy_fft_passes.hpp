@@ -141,6 +141,12 @@ __device__ __forceinline__ void staged_copy(int tid, Load load, Store store) {
     }
 }
 
+// Binary32 complex pair (16 bytes) for the transposed reads of the inverse
+// passes: elements l and l + 1 of a tile row are adjacent in memory and
+// 16-byte aligned (l even, rows start at multiples of ROWS), so one lane loads
+// both.
+typedef float F4 __attribute__((ext_vector_type(4)));
+
 // z pass: real rows -> half spectrum, written transposed into A.
 template <typename T, int N, int V>
 __global__ void __launch_bounds__((RowsCfg<T, N / 2, N, V>::THREADS)) k_fft_z_r2c(FftArgs<T> a) {
@@ -651,14 +657,30 @@ __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_bwd(FftAr
     const C* src = a.B + (uint64_t)kz * N * N + x0;
     C w[A];
     if (t < B) tl_twiddle_load<A, LEN>(w, t, a.tw, 1);
-    staged_copy<ROWS * N, NT, C>(
-        tid,
-        [&](int l) {
-            C val = src[(uint64_t)(l / ROWS) * N + (l % ROWS)];
-            if (SG && a.deriv == 2) val = mul_mi(val, a.kd[l / ROWS]);
-            return val;
-        },
-        [&](int l, C val) { lds[(l / ROWS) * TS + (l % ROWS)] = val; });
+    if constexpr (sizeof(T) == 4 && !SG && ROWS % 2 == 0 && (ROWS * N / 2) % NT == 0) {
+        constexpr int PER = ROWS * N / 2 / NT;
+        F4 q[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int l = 2 * (tid + i * NT);
+            q[i] = *reinterpret_cast<const F4*>(&src[(uint64_t)(l / ROWS) * N + (l % ROWS)]);
+        }
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int l = 2 * (tid + i * NT);
+            lds[(l / ROWS) * TS + (l % ROWS)] = C{q[i].x, q[i].y};
+            lds[(l / ROWS) * TS + (l % ROWS) + 1] = C{q[i].z, q[i].w};
+        }
+    } else {
+        staged_copy<ROWS * N, NT, C>(
+            tid,
+            [&](int l) {
+                C val = src[(uint64_t)(l / ROWS) * N + (l % ROWS)];
+                if (SG && a.deriv == 2) val = mul_mi(val, a.kd[l / ROWS]);
+                return val;
+            },
+            [&](int l, C val) { lds[(l / ROWS) * TS + (l % ROWS)] = val; });
+    }
     __syncthreads();
     C v[TL::EMAX];
     if (t < B) {
@@ -710,18 +732,39 @@ __global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_c2r(F
         for (int n1 = 0; n1 < A; ++n1) wu[n1] = a.tw[t + B * n1];
         tl_twiddle_load<A, LEN>(w, t, a.tw, 2);
     }
-    staged_copy<ROWS * NH, NT, C>(
-        tid,
-        [&](int l) {
-            C val = src[(uint64_t)(l / ROWS) * N + (l % ROWS)];
-            if (SG && a.deriv == 3) val = mul_mi(val, a.kd[l / ROWS]);
-            return val;
-        },
-        [&](int l, C val) {
-            const int rr = l % ROWS, k = l / ROWS;
-            if (k == 0 || k == LEN) val.y = (T)0;  // c2r reads only the independent half
-            lds[rr * NAT + k] = val;
-        });
+    const auto put = [&](int l, C val) {
+        const int rr = l % ROWS, k = l / ROWS;
+        if (k == 0 || k == LEN) val.y = (T)0;  // c2r reads only the independent half
+        lds[rr * NAT + k] = val;
+    };
+    if constexpr (sizeof(T) == 4 && !SG && ROWS % 2 == 0) {
+        constexpr int PAIRS = ROWS * NH / 2, PER = (PAIRS + NT - 1) / NT;
+        F4 q[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int m = tid + i * NT;
+            const int l = 2 * m;
+            if (PAIRS % NT == 0 || m < PAIRS) q[i] = *reinterpret_cast<const F4*>(&src[(uint64_t)(l / ROWS) * N + (l % ROWS)]);
+        }
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int m = tid + i * NT;
+            const int l = 2 * m;
+            if (PAIRS % NT == 0 || m < PAIRS) {
+                put(l, C{q[i].x, q[i].y});
+                put(l + 1, C{q[i].z, q[i].w});
+            }
+        }
+    } else {
+        staged_copy<ROWS * NH, NT, C>(
+            tid,
+            [&](int l) {
+                C val = src[(uint64_t)(l / ROWS) * N + (l % ROWS)];
+                if (SG && a.deriv == 3) val = mul_mi(val, a.kd[l / ROWS]);
+                return val;
+            },
+            put);
+    }
     __syncthreads();
     C v[TL::EMAX];
     if (t < B) {

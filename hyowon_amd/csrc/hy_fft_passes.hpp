@@ -425,33 +425,61 @@ __global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_r2c(F
     // untangle: X[k] = E + W^k O, E = (Z[k]+conj Z[L-k])/2, O = -i/2 (Z[k]-conj Z[L-k]);
     // thread tid forms the outputs l = tid + i NT (k = l / ROWS), its twiddles
     // W^k loaded ahead of the barrier
-    constexpr int COUNT = ROWS * NH, PER = (COUNT + NT - 1) / NT;
-    C wk[PER];
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-        const int l = tid + i * NT;
-        if (COUNT % NT == 0 || l < COUNT) wk[i] = a.tw[l / ROWS];
-    }
-    __syncthreads();
+    const auto untangle = [&](int l, C w) {
+        const int rr = l % ROWS, k = l / ROWS;
+        const C zk = lds[rr * NAT + (k % LEN)];
+        const C zc = cconj(lds[rr * NAT + (LEN - k) % LEN]);
+        const C e = {(T)0.5 * (zk.x + zc.x), (T)0.5 * (zk.y + zc.y)};
+        const C d = csub(zk, zc);
+        const C o = {(T)0.5 * d.y, (T)-0.5 * d.x};
+        return cadd(e, cmul(w, o));
+    };
     C* dst = a.A + (uint64_t)x * NH * N + y0;
-    C out[PER];
+    if constexpr (sizeof(T) == 4 && ROWS % 2 == 0) {
+        // pairs (k, rr) and (k, rr + 1), rr even: one twiddle, one 16-byte store
+        constexpr int PAIRS = ROWS * NH / 2, PER = (PAIRS + NT - 1) / NT;
+        C wk[PER];
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-        const int l = tid + i * NT;
-        if (COUNT % NT == 0 || l < COUNT) {
-            const int rr = l % ROWS, k = l / ROWS;
-            const C zk = lds[rr * NAT + (k % LEN)];
-            const C zc = cconj(lds[rr * NAT + (LEN - k) % LEN]);
-            const C e = {(T)0.5 * (zk.x + zc.x), (T)0.5 * (zk.y + zc.y)};
-            const C d = csub(zk, zc);
-            const C o = {(T)0.5 * d.y, (T)-0.5 * d.x};
-            out[i] = cadd(e, cmul(wk[i], o));
+        for (int i = 0; i < PER; ++i) {
+            const int m = tid + i * NT;
+            if (PAIRS % NT == 0 || m < PAIRS) wk[i] = a.tw[(2 * m) / ROWS];
         }
-    }
+        __syncthreads();
+        F4 q[PER];
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-        const int l = tid + i * NT;
-        if (COUNT % NT == 0 || l < COUNT) dst[(uint64_t)(l / ROWS) * N + (l % ROWS)] = out[i];
+        for (int i = 0; i < PER; ++i) {
+            const int m = tid + i * NT, l = 2 * m;
+            if (PAIRS % NT == 0 || m < PAIRS) {
+                const C c0 = untangle(l, wk[i]), c1 = untangle(l + 1, wk[i]);
+                q[i] = F4{c0.x, c0.y, c1.x, c1.y};
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int m = tid + i * NT, l = 2 * m;
+            if (PAIRS % NT == 0 || m < PAIRS)
+                *reinterpret_cast<F4*>(&dst[(uint64_t)(l / ROWS) * N + (l % ROWS)]) = q[i];
+        }
+    } else {
+        constexpr int COUNT = ROWS * NH, PER = (COUNT + NT - 1) / NT;
+        C wk[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int l = tid + i * NT;
+            if (COUNT % NT == 0 || l < COUNT) wk[i] = a.tw[l / ROWS];
+        }
+        __syncthreads();
+        C out[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int l = tid + i * NT;
+            if (COUNT % NT == 0 || l < COUNT) out[i] = untangle(l, wk[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int l = tid + i * NT;
+            if (COUNT % NT == 0 || l < COUNT) dst[(uint64_t)(l / ROWS) * N + (l % ROWS)] = out[i];
+        }
     }
 }
 

@@ -745,13 +745,32 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
     __syncthreads();
 
     double mass_acc = 0.0;
-    for (int c = threadIdx.x; c < cells; c += blockDim.x) {
-        const long long q = (long long)acc[c];
-        mass_acc += (double)q;
-        const double mcell = (double)q * Q.fixed_to_mass;
-        const int lz = c % T, ly = (c / T) % T, lx = c / (T * T);
-        const uint64_t gx = (uint64_t)(tx * T + lx), gy = (uint64_t)(ty * T + ly), gz = (uint64_t)(tz * T + lz);
-        mesh[(gx * g.N + gy) * g.N + gz] = contrast_of((double)mcell, P) ;
+    if constexpr (TE > 0 && TE % 4 == 0) {
+        // four z-adjacent cells per lane, one 16-byte store (a tile's z-runs
+        // start at multiples of TE cells)
+        typedef float F4 __attribute__((ext_vector_type(4)));
+        for (int c4 = threadIdx.x; c4 < cells / 4; c4 += blockDim.x) {
+            const int c = 4 * c4;
+            const int lz = c % T, ly = (c / T) % T, lx = c / (T * T);
+            float o[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long long q = (long long)acc[c + j];
+                mass_acc += (double)q;
+                o[j] = contrast_of((double)q * Q.fixed_to_mass, P);
+            }
+            const uint64_t gx = (uint64_t)(tx * T + lx), gy = (uint64_t)(ty * T + ly), gz = (uint64_t)(tz * T + lz);
+            *reinterpret_cast<F4*>(&mesh[(gx * g.N + gy) * g.N + gz]) = F4{o[0], o[1], o[2], o[3]};
+        }
+    } else {
+        for (int c = threadIdx.x; c < cells; c += blockDim.x) {
+            const long long q = (long long)acc[c];
+            mass_acc += (double)q;
+            const double mcell = (double)q * Q.fixed_to_mass;
+            const int lz = c % T, ly = (c / T) % T, lx = c / (T * T);
+            const uint64_t gx = (uint64_t)(tx * T + lx), gy = (uint64_t)(ty * T + ly), gz = (uint64_t)(tz * T + lz);
+            mesh[(gx * g.N + gy) * g.N + gz] = contrast_of((double)mcell, P);
+        }
     }
     // deterministic per-tile mass in units of the fixed-point quantum
     for (int off = 32; off > 0; off >>= 1) mass_acc += __shfl_down(mass_acc, off);

@@ -23,7 +23,7 @@ EXPORTS = [
     "hy_kick_factor", "hy_geometric_mean", "hy_time_boundaries", "hy_exact_sum",
     "hy_gravitational_constant", "hy_rho_crit0", "hy_pm_profile", "hy_pm_profile_read",
     "hy_pm_get_tuning", "hy_pm_set_tuning", "hy_pm_autotune", "hy_ensemble_create", "hy_ensemble_step",
-    "hy_ensemble_check", "hy_ensemble_destroy", "hy_pm_kdk_vjp", "hy_pm_density_vjp",
+    "hy_ensemble_check", "hy_ensemble_reset", "hy_ensemble_destroy", "hy_pm_kdk_vjp", "hy_pm_density_vjp",
     "hy_ic_gaussian_modes", "hy_ic_last_error", "hy_pm_power_spectrum", "hy_pm_force_meshes",
     "hy_pm_interpolate", "hy_ic_lpt",
 ]
@@ -167,6 +167,7 @@ def lib():
     fn("hy_ensemble_create", i, C.POINTER(hy_ensemble_member), u32, C.POINTER(vp))
     fn("hy_ensemble_step", i, vp, d, d, d, u32)
     fn("hy_ensemble_check", i, vp)
+    fn("hy_ensemble_reset", i, vp)
     fn("hy_ensemble_destroy", i, vp)
     fn("hy_pm_kdk_vjp", i, vp, u64, vp, vp, vp, d, vp, vp, vp, vp, vp, vp, d, d, d, u32)
     fn("hy_pm_density_vjp", i, vp, u64, vp, vp, vp, d, vp, d, vp, vp, vp)

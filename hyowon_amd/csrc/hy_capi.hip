@@ -249,6 +249,11 @@ void raise_errors(uint32_t e) {
     if (e & ERR_SPECTRUM_NONFINITE) fail(HY_ERUNTIME, "FourierDensityBuilder Fourier mode is non-finite");
     if (e & ERR_INTERP_UNDERFLOW)
         fail(HY_EUNDERFLOW, "CIC interpolation lost a non-zero CIC weight or field-weight product in core::Real");
+    if (e & ERR_GATHER_UNDERFLOW)
+        fail(HY_EUNDERFLOW, "CIC centered-gradient gather lost a non-zero CIC weight, gradient, or weighted "
+                            "contribution in core::Real");
+    if (e & ERR_ROUTING) fail(HY_ELOGIC, "CIC bucket routing does not cover each particle exactly once");
+    if (e & ERR_MASS_CONSERVATION) fail(HY_ERUNTIME, "CIC deposited mass violates the forward-error bound");
 }
 
 // Input mass identity (src/gravity/pm_solver.cpp:120-130,665-684).
@@ -374,7 +379,22 @@ DepositPlan plan_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double
     return {Min, P};
 }
 
-void launch_deposit(hy_pm_ctx* c, const hy::DepParams& P) {
+// Forward-error factor of the mass-conservation bound for n particles per box
+// (src/math/cic_mass_conservation.cpp:25-63): gamma(32 n + 1) + 64 eps, plus
+// 1e-4 for the binary32 mesh.
+long double mass_tol_factor(const hy_pm_ctx* c, uint64_t n) {
+    const long double eps = std::numeric_limits<double>::epsilon();
+    const long double nu = (32.0L * static_cast<long double>(n) + 1.0L) * eps;
+    if (!(nu < 0.25L)) fail(HY_EOVERFLOW, "CIC mass conservation forward-error bound is not informative at this particle count");
+    const long double gamma = nu / (1.0L - nu);
+    long double f = gamma + 64.0L * eps;
+    if (c->prec == HY_PREC_FP32_MIXED) f += 1e-4L;
+    return f;
+}
+
+// status_check: also evaluate the routing identity and the mass bound on the
+// device (error bits reported by the synchronous check or hy_pm_check).
+void launch_deposit(hy_pm_ctx* c, const hy::DepParams& P, bool status_check = true) {
     if (c->prec == HY_PREC_FP64)
         hipLaunchKernelGGL((c->geom.T == 16 ? hy::k_deposit<double, 16> : hy::k_deposit<double, 0>),
                            dim3(c->ntiles * c->nbox), dim3(hy::DEP_THREADS), 0, c->stream, P,
@@ -392,6 +412,12 @@ void launch_deposit(hy_pm_ctx* c, const hy::DepParams& P) {
     hipLaunchKernelGGL(hy::k_reduce_sum, dim3(c->nbox), dim3(256), 0, c->stream, c->mass_partials,
                        (uint64_t)hy::REDUCE_BLOCKS, c->mass_total);
     launch_check("k_reduce_sum");
+    if (status_check) {
+        const double tol = static_cast<double>(mass_tol_factor(c, P.n / c->nbox));
+        hipLaunchKernelGGL(hy::k_deposit_status, dim3((uint32_t)blocks_for(c->nbox, 256)), dim3(256), 0, c->stream,
+                           c->grand_total, P.n, c->mass_total, c->nbox, P.total_mass, tol, c->err);
+        launch_check("k_deposit_status");
+    }
 }
 
 template <typename T>
@@ -572,13 +598,8 @@ void launch_gather(hy_pm_ctx* c, hy::GatherArgs<T, PT> a) {
 void check_mass(hy_pm_ctx* c, double input, double deposited, uint64_t n) {
     if (!std::isfinite(deposited) || deposited <= 0.0)
         fail(HY_ERUNTIME, "PMSolver total deposited mass must be finite and positive");
-    const long double eps = std::numeric_limits<double>::epsilon();
-    const long double nu = (32.0L * static_cast<long double>(n) + 1.0L) * eps;
-    if (!(nu < 0.25L)) fail(HY_EOVERFLOW, "CIC mass conservation forward-error bound is not informative at this particle count");
-    const long double gamma = nu / (1.0L - nu);
     const long double scale = std::max<long double>(input, deposited);
-    long double tol = (gamma + 64.0L * eps) * scale;
-    if (c->prec == HY_PREC_FP32_MIXED) tol += 1e-4L * scale;
+    const long double tol = mass_tol_factor(c, n) * scale;
     if (std::fabs(static_cast<long double>(deposited) - static_cast<long double>(input)) > tol)
         fail(HY_ERUNTIME, "CIC deposited mass violates the forward-error bound");
 }
@@ -612,7 +633,7 @@ void run_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, con
         if ((st.err & hy::ERR_SCRATCH) && attempt < 3) {
             // grow the fallback scratch (records of the hot tiles) and redo the
             // deposit; errors the retry would clear are raised first
-            raise_errors(st.err & ~hy::ERR_SCRATCH);
+            raise_errors(st.err & ~(hy::ERR_SCRATCH | hy::ERR_MASS_CONSERVATION | hy::ERR_ROUTING));
             const uint64_t grow = c->scratch_cap * 2;
             void* p = nullptr;
             hip_check(hipMalloc(&p, grow * sizeof(uint64_t)), "hipMalloc scratch");
@@ -1472,7 +1493,7 @@ void deposit_values(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, 
                            1.0 / unit, c->keys, c->ranks, c->offsets, c->rec_mass);
         launch_check("k_scatter_values");
     }
-    launch_deposit(c, plan.P);
+    launch_deposit(c, plan.P, false);
 }
 
 }  // namespace
@@ -1821,6 +1842,7 @@ struct hy_ensemble {
     int device = 0;
     double* dfac = nullptr;  // k_prev, k_next, d_next for the next replay
     hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool fresh = false;  // next step re-keys the members eagerly (positions were replaced)
     ~hy_ensemble() {
         for (hipGraphExec_t g : exec)
             if (g) (void)hipGraphExecDestroy(g);
@@ -1863,6 +1885,20 @@ int hy_ensemble_step(hy_ensemble* e, double k_prev, double k_next, double d_next
         if ((has_prev && !std::isfinite(k_prev)) || (has_next && (!std::isfinite(k_next) || !std::isfinite(d_next))))
             fail(HY_EOVERFLOW, "Leapfrog drift/kick factors must be finite");
         hip_check(hipSetDevice(e->device), "hipSetDevice");
+        if (e->fresh) {
+            // positions were replaced outside the graphs: one eager step with
+            // fresh bucket keys, after which the recorded steps' reused keys
+            // are again those of the members' current positions
+            const uint32_t kflags = (has_prev ? HY_HAS_PREV : 0u) | (has_next ? HY_HAS_NEXT : 0u) | HY_ASYNC;
+            for (const hy_ensemble_member& m : e->members) {
+                const int rc = hy_pm_kdk(m.ctx, m.n, m.x, m.y, m.z, m.px, m.py, m.pz, m.x, m.y, m.z, m.px, m.py,
+                                         m.pz, nullptr, m.uniform_mass, k_prev, k_next, d_next, nullptr, nullptr,
+                                         nullptr, kflags);
+                if (rc != HY_OK) fail(rc, "ensemble re-keying step: " + g_last_error);
+            }
+            e->fresh = false;
+            return;
+        }
         hipLaunchKernelGGL(hy::k_set_factors, dim3(1), dim3(64), 0, e->stream, e->dfac, k_prev, k_next, d_next);
         launch_check("k_set_factors");
         const int idx = (has_prev ? 1 : 0) | (has_next ? 2 : 0);
@@ -1905,16 +1941,38 @@ int hy_ensemble_step(hy_ensemble* e, double k_prev, double k_next, double d_next
     });
 }
 
+int hy_ensemble_reset(hy_ensemble* e) {
+    return guarded([&] {
+        if (!e) fail(HY_EINVAL, "hy_ensemble_reset: null ensemble");
+        hip_check(hipSetDevice(e->device), "hipSetDevice");
+        hip_check(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+        for (const hy_ensemble_member& m : e->members) {
+            m.ctx->keys_ready = false;
+            reset_status(m.ctx);  // errors of the replaced run are discarded
+        }
+        hip_check(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+        e->fresh = true;
+    });
+}
+
 int hy_ensemble_check(hy_ensemble* e) {
     if (!e) {
         g_last_error = "hy_ensemble_check: null ensemble";
         return HY_EINVAL;
     }
+    // every member is checked (each report clears that member's error word);
+    // the first error is returned
+    int first = HY_OK;
+    std::string msg;
     for (const hy_ensemble_member& m : e->members) {
         const int rc = hy_pm_check(m.ctx);
-        if (rc != HY_OK) return rc;
+        if (rc != HY_OK && first == HY_OK) {
+            first = rc;
+            msg = g_last_error;
+        }
     }
-    return HY_OK;
+    g_last_error = msg;
+    return first;
 }
 
 int hy_ensemble_destroy(hy_ensemble* e) {

@@ -30,6 +30,9 @@ enum : uint32_t {
     ERR_STALE_KEYS = 1u << 8,          // reused bucket keys do not match the positions
     ERR_SPECTRUM_NONFINITE = 1u << 9,  // runtime_error: a Fourier mode or shell power is not finite
     ERR_INTERP_UNDERFLOW = 1u << 10,   // underflow_error: a CIC interpolation product rounded to zero
+    ERR_MASS_CONSERVATION = 1u << 11,  // runtime_error: deposited mass outside the forward-error bound
+    ERR_ROUTING = 1u << 12,            // logic_error: bucket routing did not cover each particle once
+    ERR_GATHER_UNDERFLOW = 1u << 13,   // underflow_error: a centered-gradient gather product rounded to zero
 };
 
 struct Geom {

@@ -837,6 +837,26 @@ __global__ void k_reduce_sum(const double* __restrict__ v, uint64_t n, double* _
     if (threadIdx.x == 0) *out = sh[0];
 }
 
+// Deposit status on the device, so that calls which do not synchronise stay
+// fail-closed: the routing-count identity of the x-plane bins
+// (src/mesh/mass_assignment_deposit.inc:269-284: every particle routed exactly
+// once) and the mass-conservation bound of src/math/cic_mass_conservation.cpp:
+// 25-63 per box, |deposited - input| <= tol_factor * max(input, deposited)
+// with a finite positive deposited mass.  One thread per box.
+__global__ void k_deposit_status(const uint32_t* __restrict__ grand_total, uint64_t n,
+                                 const double* __restrict__ mass_total, uint32_t nbox, double input_per_box,
+                                 double tol_factor, uint32_t* __restrict__ err) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    uint32_t bits = 0;
+    if (b == 0 && (uint64_t)*grand_total != n) bits |= ERR_ROUTING;
+    const double dep = mass_total[b];
+    const double scale = dep > input_per_box ? dep : input_per_box;
+    if (!isfinite(dep) || !(dep > 0.0) || fabs(dep - input_per_box) > tol_factor * scale)
+        bits |= ERR_MASS_CONSERVATION;
+    if (bits) atomicOr(err, bits);
+}
+
 // ===========================================================================
 // Fused centered-gradient gather + kick/drift
 // ===========================================================================
@@ -1013,6 +1033,11 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
             const bool exact = g.inv_exact != 0;
             auto cdiff = [&](T d) -> T { return exact ? (-d) * inv_two_dx : -d / two_dx; };
             T gx = 0, gy = 0, gz = 0;
+            // lost non-zero products (src/mesh/mass_assignment_gradient.cpp:157-196):
+            // a weight whose three factors are non-zero, a gradient of a non-zero
+            // difference, or a contribution of a non-zero weight and gradient
+            // that rounds to zero
+            bool lost = false;
 #pragma unroll
             for (int ox = 0; ox < 2; ++ox) {
 #pragma unroll
@@ -1027,13 +1052,25 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
                         const T dxv = X[ox + 2][oy][oz] - X[ox][oy][oz];
                         const T dyv = ypl - ymi;
                         const T dzv = zpl - zmi;
-                        gx += w * cdiff(dxv);
-                        gy += w * cdiff(dyv);
-                        gz += w * cdiff(dzv);
+                        const T gxv = cdiff(dxv), gyv = cdiff(dyv), gzv = cdiff(dzv);
+                        const T cx = w * gxv, cy = w * gyv, cz = w * gzv;
+                        const bool w_lost = w == (T)0 && wx[ox] != (T)0 && wy[oy] != (T)0 && wz[oz] != (T)0;
+                        const bool g_lost = (dxv != (T)0 && gxv == (T)0) || (dyv != (T)0 && gyv == (T)0) ||
+                                            (dzv != (T)0 && gzv == (T)0);
+                        const bool g_nonzero = gxv != (T)0 || gyv != (T)0 || gzv != (T)0;
+                        const bool c_lost = (gxv != (T)0 && cx == (T)0) || (gyv != (T)0 && cy == (T)0) ||
+                                            (gzv != (T)0 && cz == (T)0);
+                        lost |= ((w != (T)0 || w_lost) && g_lost) || (w_lost && g_nonzero) ||
+                                (w != (T)0 && c_lost);
+                        gx += cx;
+                        gy += cy;
+                        gz += cz;
                     }
                 }
             }
-            if (!isfinite(gx) || !isfinite(gy) || !isfinite(gz)) {
+            if (lost) {
+                fail = ERR_GATHER_UNDERFLOW;
+            } else if (!isfinite(gx) || !isfinite(gy) || !isfinite(gz)) {
                 fail = ERR_FORCE_NONFINITE;
             } else {
                 if (a.acc_x) {

@@ -62,6 +62,11 @@ class Ensemble:
     def check(self) -> None:
         check(lib().hy_ensemble_check(self._h))
 
+    def reset(self) -> None:
+        """Call after the members' positions were written outside the
+        ensemble: the next step re-keys every member eagerly."""
+        check(lib().hy_ensemble_reset(self._h))
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             lib().hy_ensemble_destroy(self._h)

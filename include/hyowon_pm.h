@@ -249,11 +249,17 @@ int hy_pm_density_vjp(hy_pm_ctx* ctx, uint64_t n, const double* x, const double*
  * hy_ensemble_step for a given (HY_HAS_PREV, HY_HAS_NEXT) combination records
  * the members' fused steps into a HIP graph, with the kick/drift factors read
  * from device memory, instantiates it and launches it; later steps only write
- * the three factors and relaunch the graph.  Run
- * one eager hy_pm_kdk per member with HY_HAS_NEXT before the first graph step
- * so that the recorded step reuses the bucket keys the previous step made.
- * Calls do not synchronise; hy_ensemble_check reports kernel-raised errors of
- * any member. */
+ * the three factors and relaunch the graph.  A run starts with one eager
+ * hy_pm_kdk per member with HY_HAS_NEXT only (force at the initial positions,
+ * K1, drift) before the first graph step, so that the recorded steps reuse the
+ * bucket keys the previous step made.  The recorded steps assume the members'
+ * positions change only through the ensemble: after positions are written by
+ * anything else (new initial conditions in the same buffers), call
+ * hy_ensemble_reset, whose next hy_ensemble_step runs eagerly with fresh keys
+ * (it also discards errors not yet reported); without it the reused keys are
+ * stale and hy_ensemble_check raises logic_error.  Calls do not synchronise;
+ * hy_ensemble_check checks every member, clears their error words and returns
+ * the first kernel-raised error. */
 typedef struct hy_ensemble hy_ensemble;
 typedef struct {
     hy_pm_ctx* ctx;
@@ -264,6 +270,7 @@ typedef struct {
 } hy_ensemble_member;
 int hy_ensemble_create(const hy_ensemble_member* members, uint32_t count, hy_ensemble** out);
 int hy_ensemble_step(hy_ensemble* e, double k_prev, double k_next, double d_next, uint32_t flags);
+int hy_ensemble_reset(hy_ensemble* e);
 int hy_ensemble_check(hy_ensemble* e);
 int hy_ensemble_destroy(hy_ensemble* e);
 

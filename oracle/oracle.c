@@ -732,8 +732,8 @@ int or_gather_centered_gradient_add(uint64_t n, const double* phi,
         if (!isfinite(x[p]) || !isfinite(y[p]) || !isfinite(z[p])) return OR_EINVAL;
         if (!isfinite(ax[p]) || !isfinite(ay[p]) || !isfinite(az[p])) return OR_EINVAL;
     }
-    int failed = 0;
-#pragma omp parallel for schedule(static) num_threads(OR_NT) reduction(|: failed)
+    int failed = 0, lost = 0;
+#pragma omp parallel for schedule(static) num_threads(OR_NT) reduction(|: failed, lost)
     for (int64_t pi = 0; pi < (int64_t)n; ++pi) {
         const uint64_t p = (uint64_t)pi;
         int64_t ix, iy, iz; double fx, fy, fz;
@@ -757,25 +757,37 @@ int or_gather_centered_gradient_add(uint64_t n, const double* phi,
                     const uint64_t cz = wrap_index(iz + oz, N);
                     const uint64_t czp = wrap_index((int64_t)cz + 1, N), czm = wrap_index((int64_t)cz - 1, N);
                     const double w = wx[ox] * wy[oy] * wz[oz];
+                    /* cic_weight_3d (src/mesh/cic_coordinate.hpp:26-38) */
+                    const int w_lost = wx[ox] != 0.0 && wy[oy] != 0.0 && wz[oz] != 0.0 && w == 0.0;
                     const double dxv = phi[(cxp * N + cy) * N + cz] - phi[(cxm * N + cy) * N + cz];
                     const double dyv = phi[(cx * N + cyp) * N + cz] - phi[(cx * N + cym) * N + cz];
                     const double dzv = phi[(cx * N + cy) * N + czp] - phi[(cx * N + cy) * N + czm];
                     const double g_x = -dxv / two_dx;
                     const double g_y = -dyv / two_dx;
                     const double g_z = -dzv / two_dx;
-                    gx += w * g_x;
-                    gy += w * g_y;
-                    gz += w * g_z;
+                    /* lost non-zero products: mass_assignment_gradient.cpp:157-196 */
+                    if ((w != 0.0 || w_lost)
+                        && ((dxv != 0.0 && g_x == 0.0) || (dyv != 0.0 && g_y == 0.0) || (dzv != 0.0 && g_z == 0.0)))
+                        lost = 1;
+                    if (w_lost && (g_x != 0.0 || g_y != 0.0 || g_z != 0.0)) lost = 1;
+                    const double c_x = w * g_x, c_y = w * g_y, c_z = w * g_z;
+                    if (w != 0.0
+                        && ((g_x != 0.0 && c_x == 0.0) || (g_y != 0.0 && c_y == 0.0) || (g_z != 0.0 && c_z == 0.0)))
+                        lost = 1;
+                    gx += c_x;
+                    gy += c_y;
+                    gz += c_z;
                 }
             }
         }
         ax[p] += gx;
         ay[p] += gy;
         az[p] += gz;
-        if (!isfinite(ax[p]) || !isfinite(ay[p]) || !isfinite(az[p])) failed = 2;
+        if (!isfinite(ax[p]) || !isfinite(ay[p]) || !isfinite(az[p])) failed |= 2;
     }
-    if (failed == 1) return OR_ELOGIC;
-    if (failed == 2) return OR_EOVERFLOW;
+    if (failed & 1) return OR_ELOGIC;
+    if (lost) return OR_EUNDERFLOW;
+    if (failed & 2) return OR_EOVERFLOW;
     return OR_OK;
 }
 

@@ -127,3 +127,39 @@ def test_leapfrog_step_loop(oracle_mod, ref_lib):
         assert np.array_equal(got, want)
     for got, want in zip(acc, racc):
         assert np.array_equal(got, want)
+
+
+def _underflow_cases():
+    """Inputs that lose a non-zero product in the centered-gradient gather
+    (src/mesh/mass_assignment_gradient.cpp:157-196), one per condition, plus a
+    control with normal magnitudes."""
+    N, L = 8, 16.0  # dx = 2
+    rng = np.random.default_rng(11)
+    cases = {}
+    # (a) a weight whose three factors are non-zero rounds to zero: fx = fy = 1e-200
+    phi = rng.normal(size=N ** 3)
+    cases["weight"] = (phi, np.array([2e-200]), np.array([2e-200]), np.array([1.0]))
+    # (b) a non-zero potential difference whose centred quotient rounds to zero
+    phi = np.zeros(N ** 3)
+    phi.reshape(N, N, N)[2, 0, 0] = 5e-324  # cell x = 2 against x = 0 around x = 1
+    cases["gradient"] = (phi, np.array([2.5]), np.array([0.5]), np.array([0.5]))
+    # (c) a weighted contribution rounds to zero: w ~ 1e-10, g ~ 1e-316
+    phi = rng.normal(size=N ** 3) * 1e-315
+    cases["contribution"] = (phi, np.array([2e-10]), np.array([0.5]), np.array([0.5]))
+    cases["control"] = (rng.normal(size=N ** 3), np.array([2e-10]), np.array([3.3]), np.array([7.9]))
+    return N, L, cases
+
+
+@pytest.mark.parametrize("which", ["weight", "gradient", "contribution", "control"])
+def test_gather_underflow_errors_match_reference(oracle_mod, ref_lib, which):
+    N, L, cases = _underflow_cases()
+    phi, x, y, z = cases[which]
+    outs = []
+    codes = []
+    for fn in (ref_lib.ref_gather, oracle_mod.lib().or_gather_centered_gradient_add):
+        acc = np.zeros((3, len(x)))
+        codes.append(fn(len(x), p(phi), p(x), p(y), p(z), L, N, p(acc[0]), p(acc[1]), p(acc[2])))
+        outs.append(acc)
+    assert codes[0] == codes[1] == (0 if which == "control" else 4)
+    if which == "control":
+        assert np.array_equal(outs[0], outs[1])

@@ -25,7 +25,7 @@ EXPORTS = [
     "hy_pm_get_tuning", "hy_pm_set_tuning", "hy_pm_autotune", "hy_ensemble_create", "hy_ensemble_step",
     "hy_ensemble_check", "hy_ensemble_reset", "hy_ensemble_destroy", "hy_pm_kdk_vjp", "hy_pm_density_vjp",
     "hy_ic_gaussian_modes", "hy_ic_last_error", "hy_pm_power_spectrum", "hy_pm_force_meshes",
-    "hy_pm_interpolate", "hy_ic_lpt",
+    "hy_pm_interpolate", "hy_ic_lpt", "hy_xplane_order_static", "hy_static_last_error",
 ]
 
 HY_FFT_PASSES = 5
@@ -178,6 +178,8 @@ def lib():
     fn("hy_pm_force_meshes", i, vp, vp, vp, vp, vp)
     fn("hy_pm_interpolate", i, vp, u64, vp, vp, vp, vp, vp)
     fn("hy_ic_lpt", i, vp, u64, vp, d, vp, vp, vp, vp, vp, vp)
+    fn("hy_xplane_order_static", i, d, u64, u64, vp, vp, vp, i, vp)
+    fn("hy_static_last_error", C.c_char_p)
     _lib = L
     return L
 

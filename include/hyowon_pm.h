@@ -174,6 +174,22 @@ int hy_pm_bucket_order(hy_pm_ctx* ctx, uint64_t n, const double* x, const double
 int hy_pm_xplane_order(hy_pm_ctx* ctx, uint64_t n, const double* x, uint32_t* order,
                        uint32_t* offsets);
 
+/* Allocation-free variant of hy_pm_xplane_order.  No reference counterpart:
+ * it serves callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * context: box_size and mesh_size (as hy_pm_create accepts them) give the
+ * geometry; x (device, n binary64 values), order (n) and offsets (mesh_size+1,
+ * uint32) are the caller's.  The kernels' only other memory is one 16 MiB
+ * file-scope device array per GPU (plane x chunk counters, scan sums, status),
+ * resolved once per device; nothing is allocated or freed.  n must be below
+ * 2^32.  Output and errors equal hy_pm_xplane_order's; runs on `stream` (NULL:
+ * the legacy default stream) on `device` and synchronises it.  Calls that
+ * overlap on one GPU share the array and race: serialise them.  The message
+ * of a failure is returned by hy_static_last_error(). */
+int hy_xplane_order_static(double box_size, uint64_t mesh_size, uint64_t n, const double* x, uint32_t* order,
+                           uint32_t* offsets, int device, void* stream);
+const char* hy_static_last_error(void);
+
 /* Waits for outstanding work and reports errors raised by HY_ASYNC calls
  * (each error once: the context's error word is cleared when reported). */
 int hy_pm_check(hy_pm_ctx* ctx);

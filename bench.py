@@ -551,9 +551,10 @@ def cpu_baseline(args):
     Ns = ns * args.mesh // args.n_side
     Ls = args.box * ns / args.n_side
     a0 = 1.0 / (1.0 + args.z_start)
-    pos, mom = ic.lpt_ics(ns, Ls, a0, seed=1, device="cpu")
-    pos = [p.numpy() for p in pos]
-    mom = [p.numpy() for p in mom]
+    # the same Philox mode-keyed field as the GPU workload, drawn on the GPU and copied out
+    pos, mom = ic.lpt_ics(ns, Ls, a0, seed=1, device="cuda", rng="philox", max_mode=ns // 2 - 1)
+    pos = [p.cpu().numpy() for p in pos]
+    mom = [p.cpu().numpy() for p in mom]
     m = ic.particle_mass(ns, Ls)
     b = np.array([a0, a0 * math.exp(math.log(1.0 / a0) / args.schedule_steps)])
     t0 = time.perf_counter()

@@ -94,13 +94,15 @@ def schedule(a_start: float, a_final: float, steps: int, omega_m: float = 0.3, o
 
 
 def class_ics(cls: BoxClass, a_start: float, precision: str, device):
-    """Concatenated 1LPT initial conditions of a class's boxes (one seed per box)."""
+    """Concatenated 1LPT initial conditions of a class's boxes (one seed per
+    box): the reference's Philox mode-keyed field on an IC mesh of the
+    lattice's size, displaced by hy_ic_lpt."""
     from . import ic
     torch = _torch()
     pos = [[], [], []]
     mom = [[], [], []]
     for seed in cls.seeds:
-        p, q = ic.lpt_ics(cls.n_side, cls.box_size, a_start, seed=int(seed), device=device)
+        p, q = ic.lpt_ics(cls.n_side, cls.box_size, a_start, seed=int(seed), device=device, rng="philox")
         for d in range(3):
             pos[d].append(p[d])
             mom[d].append(q[d])

@@ -103,7 +103,7 @@ def test_density_fp32(gpu, oracle_mod, kind, N, L, n):
     assert np.abs(mesh - ref).max() <= 1e-5 * scale
 
 
-@pytest.mark.parametrize("N", [8, 12, 16, 24, 32, 48, 64, 96, 128])
+@pytest.mark.parametrize("N", [8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384])
 @pytest.mark.parametrize("deconv", [False, True])
 def test_potential_fp64(gpu, oracle_mod, N, deconv):
     L = 2.0 * N
@@ -119,7 +119,7 @@ def test_potential_fp64(gpu, oracle_mod, N, deconv):
     assert err < 1e-12, err
 
 
-@pytest.mark.parametrize("N", [16, 48, 64, 128])
+@pytest.mark.parametrize("N", [16, 48, 64, 128, 192, 256])
 def test_potential_fp32(gpu, oracle_mod, N):
     L = 2.0 * N
     rng = np.random.default_rng(N + 1)

@@ -379,10 +379,9 @@ def run_pk(args):
         s.close()
         torch.cuda.empty_cache()
     cosmo = ic.Cosmology()
-    d1, _ = cosmo.growth(a0)
     main_prec = args.precisions[0]
     rows = res[main_prec]["bins"]
-    ratio = [b[1] / (d1 * d1 * float(cosmo.power(np.array([b[0]]))[0])) if b[3] else None for b in rows]
+    ratio = [b[1] / float(cosmo.power(np.array([b[0]]), a0)[0]) if b[3] else None for b in rows]
     out = {
         "metric": "matter power-spectrum estimates/s (512^3 particles, 1024^3 mesh, interlaced)",
         "value": 1e3 / res[main_prec]["ms"], "unit": "estimates/s", "n_gpus": 1,

@@ -26,6 +26,7 @@ EXPORTS = [
     "hy_ensemble_check", "hy_ensemble_reset", "hy_ensemble_destroy", "hy_pm_kdk_vjp", "hy_pm_density_vjp",
     "hy_ic_gaussian_modes", "hy_ic_last_error", "hy_pm_power_spectrum", "hy_pm_force_meshes",
     "hy_pm_interpolate", "hy_ic_lpt", "hy_xplane_order_static", "hy_static_last_error",
+    "hy_growth", "hy_linear_power", "hy_ic_mode_variance", "hy_cubic_spline",
 ]
 
 HY_FFT_PASSES = 5
@@ -106,6 +107,11 @@ HY_METHOD_PURE_PM = 0
 HY_METHOD_TREEPM_LONG_RANGE = 1
 
 
+class hy_cosmology(C.Structure):
+    _fields_ = [("h", C.c_double), ("omega_m", C.c_double), ("omega_b", C.c_double),
+                ("omega_lambda", C.c_double), ("sigma8", C.c_double), ("n_s", C.c_double)]
+
+
 class hy_pk_options(C.Structure):
     _fields_ = [("num_bins", C.c_int), ("subtract_shot_noise", C.c_int), ("interlaced", C.c_int),
                 ("max_k_fraction_nyquist", C.c_double)]
@@ -180,6 +186,11 @@ def lib():
     fn("hy_ic_lpt", i, vp, u64, vp, d, vp, vp, vp, vp, vp, vp)
     fn("hy_xplane_order_static", i, d, u64, u64, vp, vp, vp, i, vp)
     fn("hy_static_last_error", C.c_char_p)
+    pc = C.POINTER(hy_cosmology)
+    fn("hy_growth", i, pc, u64, pd, pd, pd, pd, pd)
+    fn("hy_linear_power", i, pc, C.c_char_p, d, d, u64, pd, pd)
+    fn("hy_ic_mode_variance", i, pc, C.c_char_p, d, d, u64, d, u64, pd)
+    fn("hy_cubic_spline", i, u64, pd, pd, u64, pd, pd)
     _lib = L
     return L
 

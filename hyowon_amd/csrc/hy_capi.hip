@@ -2026,6 +2026,47 @@ int hy_time_boundaries(double a_start, double a_final, double dlna, const double
 }
 
 double hy_exact_sum(const double* v, uint64_t n) { return hy::host::exact_sum(v, n); }
+
+// ---------------------------------------------------------------------------
+// Initial-condition scalars
+// ---------------------------------------------------------------------------
+namespace {
+hy::host::Cosmology cosmology_of(const hy_cosmology* c) {
+    if (!c) fail(HY_EINVAL, "null cosmology");
+    return {c->h, c->omega_m, c->omega_b, c->omega_lambda, c->sigma8, c->n_s};
+}
+}  // namespace
+
+int hy_growth(const hy_cosmology* c, uint64_t n, const double* a, double* D1, double* f1, double* D2, double* f2) {
+    return guarded([&] {
+        if (n && !a) fail(HY_EINVAL, "null scale factors");
+        hy::host::growth(cosmology_of(c), n, a, D1, f1, D2, f2);
+    });
+}
+
+int hy_linear_power(const hy_cosmology* c, const char* table, double table_redshift, double start_redshift,
+                    uint64_t n, const double* k, double* out) {
+    return guarded([&] {
+        if (!table || (n && (!k || !out))) fail(HY_EINVAL, "null argument");
+        hy::host::linear_power(cosmology_of(c), table, table_redshift, start_redshift, n, k, out);
+    });
+}
+
+int hy_ic_mode_variance(const hy_cosmology* c, const char* table, double table_redshift, double start_redshift,
+                        uint64_t mesh_size, double box_size, uint64_t max_mode, double* variance_by_m2) {
+    return guarded([&] {
+        if (!table || !variance_by_m2) fail(HY_EINVAL, "null argument");
+        hy::host::mode_variance(cosmology_of(c), table, table_redshift, start_redshift, mesh_size, box_size,
+                                max_mode, variance_by_m2);
+    });
+}
+
+int hy_cubic_spline(uint64_t n, const double* x, const double* y, uint64_t m, const double* q, double* out) {
+    return guarded([&] {
+        if (!x || !y || (m && (!q || !out))) fail(HY_EINVAL, "null argument");
+        hy::host::cubic_spline(n, x, y, m, q, out);
+    });
+}
 double hy_gravitational_constant(void) { return hy::host::gravitational_constant(); }
 double hy_rho_crit0(void) { return hy::host::rho_crit0(); }
 

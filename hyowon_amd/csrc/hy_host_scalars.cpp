@@ -40,22 +40,24 @@ double rho_crit0() {
 // --------------------------------------------------------------------------
 // Background
 // --------------------------------------------------------------------------
-double hubble(double a, double omega_m, double omega_lambda) {
+// sqrt(Omega_m) a^(-3/2), the matter term of E(a) (src/cosmology/cosmology_model.cpp:25-57)
+double matter_amplitude(double a, double omega_m) {
     if (!std::isfinite(a) || a <= 0.0) throw std::invalid_argument("Scale factor must be finite and positive");
     const double root_a = std::sqrt(a);
     const double root_m = std::sqrt(omega_m);
-    const double root_l = std::sqrt(omega_lambda);
-    double matter;
     if (a <= 1.0) {
         // sqrt(Om) / a / sqrt(a), one rounding to binary64 at the end
         long double q = static_cast<long double>(root_m);
         q /= static_cast<long double>(a);
         q /= static_cast<long double>(root_a);
-        matter = static_cast<double>(q);
-    } else {
-        matter = root_m * ((1.0 / root_a) / a);
+        return static_cast<double>(q);
     }
-    return 100.0 * std::hypot(matter, root_l);
+    return root_m * ((1.0 / root_a) / a);
+}
+
+double hubble(double a, double omega_m, double omega_lambda) {
+    const double matter = matter_amplitude(a, omega_m);
+    return 100.0 * std::hypot(matter, std::sqrt(omega_lambda));
 }
 
 // --------------------------------------------------------------------------

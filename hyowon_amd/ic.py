@@ -1,36 +1,42 @@
-"""Synthetic 1LPT initial conditions for benchmarks and tests.
+"""1LPT initial conditions for benchmarks and tests.
 
 Builds the workload the PM step is measured on: a corner lattice of particles
 (index (i*N+j)*N+k, q = (i,j,k)*L/N; src/ic/initial_conditions.cpp:296-312)
-displaced by the first-order LPT field of a Gaussian density with the BBKS
-spectrum P(k) = A k^n_s T(k)^2, T from Bardeen et al. (1986) with q = k/Gamma,
-normalised to sigma_8 at z=0 and scaled by the linear growth D1(a).
-Momenta p = a^2 H(a) f(a) psi (src/ic/lpt_displacement.cpp:301-380).
+displaced by the first-order LPT field of a Gaussian density
+(src/ic/lpt_displacement.cpp:202-384), momenta p = a^2 H(a) f1(a) psi.
 
-The white noise comes from torch's seeded Gaussian generator, not from the
-reference's Philox mode-keyed generator, so the realisations differ from
-HYOWON's for the same seed; the statistics and the path being timed do not.
+The linear spectrum follows the reference's input path: a two-column table
+"k P(k)" at z = 0 with its semantic header (src/ic/linear_power_spectrum.cpp),
+rescaled by the growth table's (D1(a_start)/D1(1))^2 and interpolated as a
+natural cubic spline in ln k -- all evaluated by libhyowon_pm (hy_linear_power,
+hy_ic_mode_variance, hy_growth).  The table written here is the synthetic BBKS
+spectrum P(k) = A k^n_s T(k)^2 (Bardeen et al. 1986, q = k/Gamma) over
+1e-4..1e2 h/Mpc, A normalised to sigma_8 at z = 0.
+
+rng="philox" draws the reference's mode-keyed Philox field
+(src/ic/random_field.cpp); rng="torch" draws real-space white noise with
+torch's generator (the differentiable runs' noise -> IC map).
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 from dataclasses import dataclass
+from functools import lru_cache
 
 import numpy as np
 from scipy import integrate
 
 
-@dataclass
+@dataclass(frozen=True)
 class Cosmology:
     omega_m: float = 0.3
     omega_lambda: float = 0.7
     h: float = 0.7
+    omega_b: float = 0.05
     sigma8: float = 0.8
     n_s: float = 0.96
     gamma: float = 0.21
-
-    def E(self, a):
-        return np.sqrt(self.omega_m / a ** 3 + self.omega_lambda)
 
     def transfer(self, k):
         q = np.asarray(k, dtype=np.float64) / self.gamma
@@ -46,30 +52,85 @@ class Cosmology:
 
     @property
     def amplitude(self) -> float:
-        R = 8.0
+        """A with sigma_8(z=0) = sigma8 for the closed-form BBKS shape."""
+        return _bbks_amplitude(self.n_s, self.gamma, self.sigma8)
 
-        def integrand(lnk):
-            k = math.exp(lnk)
-            x = k * R
-            w = 3.0 * (math.sin(x) - x * math.cos(x)) / x ** 3 if x > 1e-4 else 1.0 - x * x / 10.0
-            return k ** 3 * float(self._unnormalised(np.array([k]))[0]) * w * w / (2 * math.pi ** 2)
-
-        s2, _ = integrate.quad(integrand, math.log(1e-5), math.log(1e3), limit=400)
-        return self.sigma8 ** 2 / s2
-
-    def power(self, k):
+    def bbks_power(self, k):
+        """Closed-form BBKS P(k) at z = 0 (the table's content)."""
         return self.amplitude * self._unnormalised(k)
 
-    def growth(self, a):
-        """Linear growth D1(a), D1(1) = 1, and f = dlnD/dlna (flat matter + Lambda)."""
-        def unnorm(a_):
-            val, _ = integrate.quad(lambda x: 1.0 / (x * self.E(x)) ** 3, 0.0, a_, limit=200)
-            return 2.5 * self.omega_m * self.E(a_) * val
+    def table(self) -> str:
+        """The spectrum table (text) with the semantic header the reference requires."""
+        return _bbks_table(self)
 
-        d = unnorm(a) / unnorm(1.0)
-        h = 1e-4
-        f = (math.log(unnorm(a * (1 + h))) - math.log(unnorm(a * (1 - h)))) / (math.log1p(h) - math.log1p(-h))
-        return d, f
+    def c_struct(self):
+        from ._lib import hy_cosmology
+        return hy_cosmology(self.h, self.omega_m, self.omega_b, self.omega_lambda, self.sigma8, self.n_s)
+
+    def growth(self, a):
+        """(D1(a), f1(a)) of the reference's growth table (hy_growth)."""
+        from ._lib import check, lib
+        av = np.array([a], np.float64)
+        d1 = np.zeros(1)
+        f1 = np.zeros(1)
+        pd = C.POINTER(C.c_double)
+        check(lib().hy_growth(C.byref(self.c_struct()), 1, av.ctypes.data_as(pd), d1.ctypes.data_as(pd),
+                              f1.ctypes.data_as(pd), None, None))
+        return float(d1[0]), float(f1[0])
+
+    def hubble(self, a: float) -> float:
+        from ._lib import lib
+        return float(lib().hy_hubble(a, self.omega_m, self.omega_lambda))
+
+    def power(self, k, a_start: float):
+        """P(k) at a_start: the table rescaled by the growth and spline-interpolated (hy_linear_power)."""
+        from ._lib import check, lib
+        kk = np.ascontiguousarray(k, np.float64).reshape(-1)
+        out = np.zeros_like(kk)
+        pd = C.POINTER(C.c_double)
+        check(lib().hy_linear_power(C.byref(self.c_struct()), self.table().encode(), 0.0, start_redshift(a_start),
+                                    kk.size, kk.ctypes.data_as(pd), out.ctypes.data_as(pd)))
+        return out.reshape(np.shape(k))
+
+
+def start_redshift(a_start: float) -> float:
+    """z with 1/(1+z) = a_start (the reference derives a_start from z_start)."""
+    return 1.0 / a_start - 1.0
+
+
+@lru_cache(maxsize=8)
+def _bbks_amplitude(n_s: float, gamma: float, sigma8: float) -> float:
+    shape = Cosmology(n_s=n_s, gamma=gamma, sigma8=sigma8)
+    R = 8.0
+
+    def integrand(lnk):
+        k = math.exp(lnk)
+        x = k * R
+        w = 3.0 * (math.sin(x) - x * math.cos(x)) / x ** 3 if x > 1e-4 else 1.0 - x * x / 10.0
+        return k ** 3 * float(shape._unnormalised(np.array([k]))[0]) * w * w / (2 * math.pi ** 2)
+
+    s2, _ = integrate.quad(integrand, math.log(1e-5), math.log(1e3), limit=400)
+    return sigma8 ** 2 / s2
+
+
+@lru_cache(maxsize=8)
+def _bbks_table(c: Cosmology, k_min: float = 1e-4, k_max: float = 1e2, per_decade: int = 100) -> str:
+    decades = math.log10(k_max / k_min)
+    n = int(round(decades * per_decade)) + 1
+    k = k_min * 10.0 ** (np.arange(n) * (decades / (n - 1)))
+    k[-1] = k_max
+    p = c.bbks_power(k)
+    head = [
+        "k_unit=h/Mpc", "power_unit=(Mpc/h)^3", "redshift=0", "species=cold_plus_baryon",
+        "gauge=synchronous", "fidelity=precision_boltzmann", "normalization=sigma8_z0",
+        f"normalization_value={c.sigma8!r}", "nonlinear=false", "massive_neutrino_sum_eV=0",
+        f"h={c.h!r}", f"omega_m={c.omega_m!r}", f"omega_b={c.omega_b!r}",
+        f"omega_lambda={c.omega_lambda!r}", f"n_s={c.n_s!r}",
+    ]
+    lines = [f"# {h}" for h in head]
+    lines.append(f"# BBKS transfer, Gamma={c.gamma!r}, A={c.amplitude!r}")
+    lines += [f"{kv:.17e} {pv:.17e}" for kv, pv in zip(k, p)]
+    return "\n".join(lines) + "\n"
 
 
 def white_noise(n_side: int, seed: int, device=None):
@@ -82,17 +143,28 @@ def white_noise(n_side: int, seed: int, device=None):
 
 def variance_by_m2(n_side: int, box: float, a_start: float, cosmo: Cosmology | None = None,
                    max_mode: int | None = None):
-    """Per-mode variance N^6/L^3 P(k) D1(a)^2 of the r2c modes of an n_side^3
-    IC mesh, indexed by m^2 = mx^2 + my^2 + mz^2 (float64, length 3 K^2 + 1
-    with K = n_side // 2); entry 0 is unused."""
+    """Per-mode variance N^6/L^3 P(|m| 2 pi/L) at a_start of the r2c modes of an
+    n_side^3 IC mesh, indexed by m^2 = mx^2 + my^2 + mz^2 (float64, length
+    3 (n/2)^2 + 1; zero at m^2 = 0 and beyond 3 K^2), from hy_ic_mode_variance:
+    the reference's table + spline + growth rescaling and its scaled product
+    (src/ic/random_field.cpp:108-235)."""
+    from ._lib import check, lib
     cosmo = cosmo or Cosmology()
-    D, _ = cosmo.growth(a_start)
-    K = n_side // 2
-    m2 = np.arange(3 * K * K + 1, dtype=np.float64)
-    k = np.sqrt(m2) * (2.0 * math.pi / box)
-    pk = np.zeros_like(k)
-    pk[1:] = cosmo.power(k[1:])
-    return float(n_side) ** 6 / box ** 3 * pk * D * D
+    K = _band_limit(n_side, max_mode)
+    half = n_side // 2
+    out = np.zeros(3 * half * half + 1, np.float64)
+    check(lib().hy_ic_mode_variance(C.byref(cosmo.c_struct()), cosmo.table().encode(), 0.0, start_redshift(a_start),
+                                    n_side, float(box), K, out.ctypes.data_as(C.POINTER(C.c_double))))
+    return out
+
+
+def _band_limit(n_side: int, max_mode: int | None) -> int:
+    """The particle-lattice band limit (src/ic/particle_lattice_bandlimit.cpp:39-80):
+    |m_axis| <= K, by default and at most (n - 1) // 2 for an n^3 lattice."""
+    K = (n_side - 1) // 2 if max_mode is None else int(max_mode)
+    if K > (n_side - 1) // 2 or K < 1:
+        raise ValueError("IC Fourier support exceeds the safe particle-lattice support")
+    return K
 
 
 def philox_modes(n_side: int, box: float, a_start: float, seed: int, cosmo: Cosmology | None = None,
@@ -103,18 +175,22 @@ def philox_modes(n_side: int, box: float, a_start: float, seed: int, cosmo: Cosm
     convention, so a mode's draw depends only on (seed, integer mode)."""
     import torch
     from ._lib import check, lib
-    var = torch.as_tensor(variance_by_m2(n_side, box, a_start, cosmo), device=f"cuda:{device}")
+    K = _band_limit(n_side, max_mode)
+    var = torch.as_tensor(variance_by_m2(n_side, box, a_start, cosmo, K), device=f"cuda:{device}")
     out = torch.empty((n_side, n_side, n_side // 2 + 1), dtype=torch.complex128, device=f"cuda:{device}")
-    # the particle-lattice band limit (src/ic/particle_lattice_bandlimit.cpp:39-80):
-    # |m_axis| <= K, by default and at most (n - 1) // 2 for an n^3 lattice on this mesh
-    K = (n_side - 1) // 2 if max_mode is None else int(max_mode)
-    if K > (n_side - 1) // 2:
-        raise ValueError("IC Fourier support exceeds the safe particle-lattice support")
     torch.cuda.synchronize(device)
     rc = lib().hy_ic_gaussian_modes(n_side, int(seed), K, var.data_ptr(), out.data_ptr(), 0, device, None)
     if rc != 0:
         raise RuntimeError(lib().hy_ic_last_error().decode())
     return out
+
+
+def momentum_factor(a_start: float, cosmo: Cosmology | None = None) -> float:
+    """a^2 H(a) f1(a), the first-order momentum factor (src/ic/lpt_displacement.cpp:202-221)."""
+    cosmo = cosmo or Cosmology()
+    a = 1.0 / (1.0 + start_redshift(a_start))
+    _, f1 = cosmo.growth(a)
+    return a * a * cosmo.hubble(a) * f1
 
 
 def lpt_from_noise(noise, box: float, a_start: float, cosmo: Cosmology | None = None):
@@ -127,10 +203,6 @@ def lpt_from_noise(noise, box: float, a_start: float, cosmo: Cosmology | None = 
     cosmo = cosmo or Cosmology()
     device = noise.device
     n = noise.shape[0]
-    D, f = cosmo.growth(a_start)
-    H = 100.0 * float(cosmo.E(a_start))
-    vel_fac = a_start * a_start * H * f
-
     kf = 2.0 * math.pi / box
     m = torch.fft.fftfreq(n, d=1.0 / n, device=device, dtype=torch.float64)
     mz = torch.fft.rfftfreq(n, d=1.0 / n, device=device, dtype=torch.float64)
@@ -139,9 +211,9 @@ def lpt_from_noise(noise, box: float, a_start: float, cosmo: Cosmology | None = 
     kz = (mz * kf).view(1, 1, n // 2 + 1)
     k2 = kx * kx + ky * ky + kz * kz
     kmag = torch.sqrt(k2).cpu().numpy()
-    pk = cosmo.power(kmag.ravel()).reshape(kmag.shape)
+    pk = cosmo.power(kmag, a_start)  # already at a_start (growth-rescaled table)
     vcell = (box / n) ** 3
-    amp = torch.as_tensor(np.sqrt(pk / vcell), device=device) * D
+    amp = torch.as_tensor(np.sqrt(pk / vcell), device=device)
     amp[0, 0, 0] = 0.0
     delta_k = torch.fft.rfftn(noise).to(torch.complex128) * amp
     return lpt_from_delta_k(delta_k, box, a_start, cosmo)
@@ -156,9 +228,7 @@ def lpt_from_delta_k(delta_k, box: float, a_start: float, cosmo: Cosmology | Non
     cosmo = cosmo or Cosmology()
     device = delta_k.device
     n = delta_k.shape[0]
-    _, f = cosmo.growth(a_start)
-    H = 100.0 * float(cosmo.E(a_start))
-    vel_fac = a_start * a_start * H * f
+    vel_fac = momentum_factor(a_start, cosmo)
     kf = 2.0 * math.pi / box
     m = torch.fft.fftfreq(n, d=1.0 / n, device=device, dtype=torch.float64)
     mz = torch.fft.rfftfreq(n, d=1.0 / n, device=device, dtype=torch.float64)
@@ -205,9 +275,7 @@ def lpt_from_delta_k_hip(delta_k, box: float, a_start: float, cosmo: Cosmology |
     cosmo = cosmo or Cosmology()
     N = delta_k.shape[0]
     n = N if n_side is None else int(n_side)
-    _, f = cosmo.growth(a_start)
-    H = 100.0 * float(cosmo.E(a_start))
-    p_factor = a_start * a_start * H * f
+    p_factor = momentum_factor(a_start, cosmo)
     dev = delta_k.device
     dk = delta_k.to(torch.complex128).contiguous()
     out = [torch.empty(n ** 3, dtype=torch.float64, device=dev) for _ in range(6)]

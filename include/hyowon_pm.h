@@ -373,6 +373,33 @@ int hy_time_boundaries(double a_start, double a_final, double delta_ln_a, const 
 /* Correctly rounded exact sum of non-negative binary64 values
  * (contract of include/cosmo_nbody/math/exact_positive_sum.hpp). */
 double hy_exact_sum(const double* values, uint64_t count);
+/* Initial-condition scalars (host, binary64).  Cosmology parameters as
+ * config::CosmologyParams (include/cosmo_nbody/config/simulation_parameters.hpp:24-31);
+ * the model is flat matter + Lambda. */
+typedef struct {
+    double h, omega_m, omega_b, omega_lambda, sigma8, n_s;
+} hy_cosmology;
+/* Linear growth D1, f1 = dlnD1/dlna, D2, f2 at scale factors a[0..n) (any output
+ * may be NULL): CosmologyModel::D1/f1/D2/f2 (include/cosmo_nbody/cosmology/
+ * cosmology_model.hpp:46-53; src/cosmology/cosmology_model.cpp:70-509): RK4 in
+ * ln a from the matter era, cubic Hermite interpolation in a. */
+int hy_growth(const hy_cosmology* c, uint64_t n, const double* a, double* D1, double* f1, double* D2, double* f2);
+/* LinearPowerSpectrum::evaluate (include/cosmo_nbody/ic/linear_power_spectrum.hpp:19-28;
+ * src/ic/linear_power_spectrum.cpp:414-664) for a table given as text: "# key=value"
+ * semantic header checked against c and table_redshift, rows "k P(k)" in h/Mpc and
+ * (Mpc/h)^3, growth-rescaled to start_redshift by (D1(a_start)/D1(a_table))^2, ln P
+ * interpolated by a natural cubic spline in ln k.  At least 3 rows. */
+int hy_linear_power(const hy_cosmology* c, const char* table, double table_redshift, double start_redshift,
+                    uint64_t n, const double* k, double* out);
+/* Per-mode variance of the Gaussian IC field, N^6/L^3 P(|m| 2 pi/L) as
+ * src/ic/random_field.cpp:108-235 forms it (scaled mantissa/exponent product),
+ * indexed by m^2 = mx^2+my^2+mz^2 for m^2 <= 3 max_mode^2 (0 beyond, and at 0):
+ * the variance_by_m2 input of hy_ic_gaussian_modes, 3 (N/2)^2 + 1 entries. */
+int hy_ic_mode_variance(const hy_cosmology* c, const char* table, double table_redshift, double start_redshift,
+                        uint64_t mesh_size, double box_size, uint64_t max_mode, double* variance_by_m2);
+/* math::CubicSpline (include/cosmo_nbody/math/cubic_spline.hpp): natural spline
+ * through (x, y) (n >= 3, x increasing), evaluated at q[0..m). */
+int hy_cubic_spline(uint64_t n, const double* x, const double* y, uint64_t m, const double* q, double* out);
 /* G in (Mpc/h)(km/s)^2/(1e10 Msun/h) and rho_crit0 (include/cosmo_nbody/cosmology/units.hpp:26-30). */
 double hy_gravitational_constant(void);
 double hy_rho_crit0(void);

@@ -120,6 +120,7 @@ def lib():
         L.fn("or_interpolate_add", C.c_int, _u64, _pd, _pd, _pd, _pd, _d, _u64, _pd)
         L.fn("or_pm_force_treepm", C.c_int, _u64, _pd, _pd, _pd, _pd, _d, _d, _u64, _d, _pd, _pd,
              _pd, _pd)
+        L.fn("or_lpt_first_order", C.c_int, _u64, _pd, _u64, _d, _d, _pd, _pd, _pd, _pd, _pd, _pd)
         L.fn("or_set_threads", None, C.c_int)
         L.fn("or_get_threads", C.c_int)
         L.fn("or_gravitational_constant", _d)
@@ -171,6 +172,10 @@ def ref():
         L.fn("ref_gravitational_constant", _d)
         L.fn("ref_rho_crit0", _d)
         L.fn("ref_philox4x32_10", None, _pu32, _pu32, _pu32)
+        L.fn("ref_growth", C.c_int, _d, _d, _u64, _pd, _pd, _pd, _pd, _pd)
+        L.fn("ref_cubic_spline", C.c_int, _u64, _pd, _pd, _u64, _pd, _pd)
+        L.fn("ref_linear_power", C.c_int, _pd, C.c_char_p, _d, _d, _d, _u64, _u64, _u64, _u64, _pd, _pd)
+        L.fn("ref_mode_variance", C.c_int, _pd, C.c_char_p, _d, _d, _d, _u64, _u64, _u64, _pd)
         L.fn("ref_uniform_pair", None, _u64, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, _pd)
         L.fn("ref_normal_pair", None, _u64, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, _pd)
         _ref = L.lib
@@ -407,3 +412,15 @@ def pm_force_treepm(x, y, z, L, N, split_scale, masses=None, uniform_mass=1.0, m
     if meshes:
         return ax, ay, az, mo.reshape(3, N, N, N)
     return ax, ay, az
+
+
+def lpt_first_order(delta_k, L, n_side, p_factor):
+    """First-order LPT lattice ICs of an r2c half spectrum (complex128 [N][N][N/2+1]):
+    returns (pos (3, n^3), mom (3, n^3))."""
+    dk = np.ascontiguousarray(delta_k, np.complex128)
+    N = dk.shape[0]
+    n = n_side ** 3
+    out = [np.zeros(n) for _ in range(6)]
+    _check(lib().or_lpt_first_order(N, dk.ctypes.data_as(_pd), n_side, L, p_factor, *[_ptr(o) for o in out]),
+           "or_lpt_first_order")
+    return np.stack(out[:3]), np.stack(out[3:])

@@ -1476,6 +1476,82 @@ int or_gaussian_modes(uint64_t N, uint64_t seed, uint64_t max_mode, const double
 /* ------------------------------------------------------------------------- */
 
 /* log(upper/lower) through log1p of the relative gap (spectral_numeric.hpp:28-47) */
+/* ------------------------------------------------------------------------- */
+/* First-order LPT: src/ic/lpt_displacement.cpp:89-129 (Nyquist rule, particle */
+/* update), 202-221 (factors), 301-384 (per-axis spectrum, c2r, lattice         */
+/* sampling); lattice src/ic/initial_conditions.cpp:296-312.                    */
+/* ------------------------------------------------------------------------- */
+int or_lpt_first_order(uint64_t N, const double* delta_k, uint64_t n_side, double L, double p_factor,
+                       double* x, double* y, double* z, double* px, double* py, double* pz) {
+    if (N < 2 || n_side == 0 || N % n_side != 0 || !(L > 0.0)) return OR_EINVAL;
+    const uint64_t NH = N / 2 + 1;
+    const uint64_t s = N / n_side;
+    const double dx = L / (double)n_side;
+    double* pos[3] = {x, y, z};
+    double* mom[3] = {px, py, pz};
+    const uint64_t np = n_side * n_side * n_side;
+    /* lattice q = index * dx, zero momenta */
+    for (uint64_t i = 0; i < n_side; ++i)
+        for (uint64_t j = 0; j < n_side; ++j)
+            for (uint64_t k = 0; k < n_side; ++k) {
+                const uint64_t p = (i * n_side + j) * n_side + k;
+                x[p] = (double)i * dx;
+                y[p] = (double)j * dx;
+                z[p] = (double)k * dx;
+                px[p] = py[p] = pz[p] = 0.0;
+            }
+    double* psi_k = (double*)malloc(N * N * NH * 2 * sizeof(double));
+    double* psi = (double*)malloc(N * N * N * sizeof(double));
+    if (!psi_k || !psi) { free(psi_k); free(psi); return OR_ERUNTIME; }
+    int bad = 0;
+    for (int dim = 0; dim < 3 && !bad; ++dim) {
+#pragma omp parallel for schedule(static) num_threads(OR_NT)
+        for (int64_t ix = 0; ix < (int64_t)N; ++ix)
+            for (uint64_t iy = 0; iy < N; ++iy) {
+                const double kx = k_component((uint64_t)ix, N, L);
+                const double ky = k_component(iy, N, L);
+                for (uint64_t iz = 0; iz < NH; ++iz) {
+                    const double kz = k_component(iz, N, L);
+                    const double full[3] = {kx, ky, kz};
+                    const uint64_t idx[3] = {(uint64_t)ix, iy, iz};
+                    const double k2 = kx * kx + ky * ky + kz * kz;
+                    const uint64_t c = ((uint64_t)ix * N + iy) * NH + iz;
+                    if (k2 == 0.0) {
+                        psi_k[2 * c] = 0.0;
+                        psi_k[2 * c + 1] = 0.0;
+                        continue;
+                    }
+                    const int nyq = N % 2 == 0 && idx[dim] == N / 2;
+                    const double f = (nyq ? 0.0 : full[dim]) / k2;
+                    const double re = delta_k[2 * c], im = delta_k[2 * c + 1];
+                    psi_k[2 * c] = -im * f;
+                    psi_k[2 * c + 1] = re * f;
+                }
+            }
+        if (or_fft_c2r_3d(N, psi_k, psi) != OR_OK) { bad = 1; break; }
+#pragma omp parallel for schedule(static) num_threads(OR_NT) reduction(|: bad)
+        for (int64_t p = 0; p < (int64_t)np; ++p) {
+            const uint64_t i = (uint64_t)p / (n_side * n_side), j = ((uint64_t)p / n_side) % n_side,
+                           k = (uint64_t)p % n_side;
+            const double v = psi[((i * s) * N + j * s) * N + k * s];
+            const double disp = 1.0 * v;
+            const double inc = p_factor * v;
+            const double cand = pos[dim][p] + disp;
+            const double mc = mom[dim][p] + inc;
+            const double w = or_wrap(cand, L);
+            if (!isfinite(v) || !isfinite(disp) || !isfinite(inc) || !isfinite(cand) || !isfinite(mc) || !isfinite(w)) {
+                bad = 1;
+                continue;
+            }
+            pos[dim][p] = w;
+            mom[dim][p] = mc;
+        }
+    }
+    free(psi_k);
+    free(psi);
+    return bad ? OR_ERUNTIME : OR_OK;
+}
+
 static int pk_log_ratio(double upper, double lower, double* out) {
     if (!isfinite(upper) || !isfinite(lower) || lower <= 0.0 || upper < lower) return OR_EINVAL;
     if (upper == lower) { *out = 0.0; return OR_OK; }

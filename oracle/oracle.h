@@ -232,6 +232,14 @@ int or_pm_force_treepm(uint64_t n, const double* x, const double* y, const doubl
 int or_power_spectrum(uint64_t n, const double* x, const double* y, const double* z,
                       const double* masses, double uniform_mass, double L, uint64_t N, int num_bins,
                       int subtract_shot_noise, int interlaced, double max_fraction, double* out);
+/* or_lpt_first_order: first-order LPT lattice ICs of an r2c density half
+ * spectrum delta_k [N][N][N/2+1] (complex128): per axis psi = c2r(i k_d/k^2
+ * delta_k) (k_d = 0 on the Nyquist index), particle (i,j,k) of the n_side^3
+ * lattice at q = index * L/n_side moves to wrap(q + psi(i s, j s, k s)),
+ * p = p_factor * psi.  Pinned by a single-mode closed form and the discrete
+ * divergence against numpy's FFT (tests/test_oracle_ic_field.py). */
+int or_lpt_first_order(uint64_t N, const double* delta_k, uint64_t n_side, double L, double p_factor,
+                       double* x, double* y, double* z, double* px, double* py, double* pz);
 int or_pk_edges(double minimum, double maximum, int bins, double* edges);
 int or_pk_bin_index(const double* edges, int bins, double v);
 long double or_pk_window_axis(double k, double dx);

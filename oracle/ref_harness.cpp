@@ -17,7 +17,12 @@
 #include "cosmo_nbody/cosmology/cosmology_model.hpp"
 #include "cosmo_nbody/cosmology/drift_kick_factors.hpp"
 #include "cosmo_nbody/cosmology/units.hpp"
+#include "cosmo_nbody/config/simulation_parameters.hpp"
+#include "cosmo_nbody/ic/linear_power_spectrum.hpp"
+#include "cosmo_nbody/ic/particle_lattice_bandlimit.hpp"
 #include "cosmo_nbody/ic/philox_rng.hpp"
+#include "cosmo_nbody/math/cubic_spline.hpp"
+#include "cosmo_nbody/math/scaled_positive_product.hpp"
 #include "cosmo_nbody/math/cic_mass_conservation.hpp"
 #include "cosmo_nbody/math/density_normalization.hpp"
 #include "cosmo_nbody/math/exact_geometric_mean.hpp"
@@ -31,7 +36,10 @@
 #include "cosmo_nbody/time/particle_update.hpp"
 #include "cosmo_nbody/time/time_stepper.hpp"
 
+#include <array>
 #include <complex>
+#include <numbers>
+#include <string>
 #include <cstdint>
 #include <cstring>
 #include <optional>
@@ -74,9 +82,112 @@ cn::config::CosmologyParams cosmology(double om, double ol) {
     return p;
 }
 
+// Full cosmology (the IC spectrum header binds all six parameters).
+cn::config::CosmologyParams cosmology_full(const double* c) {
+    cn::config::CosmologyParams p;
+    p.h = c[0];
+    p.omega_m = c[1];
+    p.omega_b = c[2];
+    p.omega_lambda = c[3];
+    p.sigma8 = c[4];
+    p.n_s = c[5];
+    return p;
+}
+
+// A generate-mode PM configuration carrying the IC spectrum inputs: lattice
+// n_side^3 on an IC mesh ic_mesh, band limit max_mode, table at table_z, start z_start.
+cn::config::SimulationParameters ic_config(const double* c, const char* table_path, double table_z, double z_start,
+                                           double L, uint64_t n_side, uint64_t ic_mesh, uint64_t max_mode) {
+    cn::config::BoxParams b;
+    b.L = L;
+    b.N = n_side;
+    b.N_mesh = ic_mesh;
+    cn::config::GravityParams g;
+    g.solver = "PM";
+    g.deconvolve_cic = false;
+    cn::config::TimeParams t;
+    t.z_start = z_start;
+    t.z_final = 0.0;
+    t.delta_ln_a = std::log(1.0 + z_start) / 8.0;
+    cn::config::ICParams i;
+    i.mode = "generate";
+    i.lpt_order = 1;
+    i.seed = 1;
+    i.mesh_per_dimension = ic_mesh;
+    i.max_mode_per_axis = max_mode;
+    i.power_spectrum_file = table_path;
+    i.power_spectrum_redshift = table_z;
+    i.power_spectrum_fidelity = "precision_boltzmann";
+    i.amplitude_mode = "gaussian";
+    i.phase_pairing = "independent";
+    cn::config::OutputParams o;
+    o.snapshot_scale_factors = {1.0};
+    o.root_directory = "/tmp";
+    o.run_label = "oracle";
+    o.timestamped_run_directory = false;
+    return cn::config::SimulationParameters(cosmology_full(c), b, g, t, i, o);
+}
+
 }  // namespace
 
 extern "C" {
+
+// CosmologyModel growth factors at scale factors a[0..n).
+int ref_growth(double om, double ol, uint64_t n, const double* a, double* D1, double* f1, double* D2, double* f2) {
+    return guarded([&] {
+        const cn::cosmology::CosmologyModel model(cosmology(om, ol));
+        for (uint64_t i = 0; i < n; ++i) {
+            D1[i] = model.D1(a[i]);
+            f1[i] = model.f1(a[i]);
+            D2[i] = model.D2(a[i]);
+            f2[i] = model.f2(a[i]);
+        }
+    });
+}
+
+// math::CubicSpline through (x, y) at q[0..m).
+int ref_cubic_spline(uint64_t n, const double* x, const double* y, uint64_t m, const double* q, double* out) {
+    return guarded([&] {
+        cn::math::CubicSpline s(std::vector<double>(x, x + n), std::vector<double>(y, y + n));
+        for (uint64_t i = 0; i < m; ++i) out[i] = s.interpolate(q[i]);
+    });
+}
+
+// LinearPowerSpectrum::evaluate over a table file (c: h, Om, Ob, OL, sigma8, n_s).
+int ref_linear_power(const double* c, const char* table_path, double table_z, double z_start, double L,
+                     uint64_t n_side, uint64_t ic_mesh, uint64_t max_mode, uint64_t n, const double* k, double* out) {
+    return guarded([&] {
+        const auto cfg = ic_config(c, table_path, table_z, z_start, L, n_side, ic_mesh, max_mode);
+        const cn::ic::LinearPowerSpectrum pk(cfg);
+        for (uint64_t i = 0; i < n; ++i) out[i] = pk.evaluate(k[i]);
+    });
+}
+
+// RandomField's per-mode variance N^6/L^3 P(k_mag(m)), indexed by m^2 (3 K^2 + 1 entries).
+int ref_mode_variance(const double* c, const char* table_path, double table_z, double z_start, double L,
+                      uint64_t n_side, uint64_t ic_mesh, uint64_t max_mode, double* out) {
+    return guarded([&] {
+        const auto cfg = ic_config(c, table_path, table_z, z_start, L, n_side, ic_mesh, max_mode);
+        const cn::ic::LinearPowerSpectrum pk(cfg);
+        const double n_real = static_cast<double>(ic_mesh);
+        const std::array<double, 6> num{n_real, n_real, n_real, n_real, n_real, n_real};
+        const std::array<double, 3> den{L, L, L};
+        const auto norm = cn::math::ScaledPositiveProduct::from_quotient(num, den, "variance");
+        const double kf = 2.0 * std::numbers::pi / L;
+        const uint64_t K = max_mode;
+        out[0] = 0.0;
+        // every m^2 <= 3 K^2 is reached by some (mx, my, mz) with |m_axis| <= K
+        for (uint64_t m2 = 1; m2 <= 3 * K * K; ++m2) out[m2] = -1.0;
+        for (int64_t mx = 0; mx <= (int64_t)K; ++mx)
+            for (int64_t my = 0; my <= (int64_t)K; ++my)
+                for (int64_t mz = 0; mz <= (int64_t)K; ++mz) {
+                    const uint64_t m2 = (uint64_t)(mx * mx + my * my + mz * mz);
+                    if (m2 == 0 || out[m2] >= 0.0) continue;
+                    const double kmag = cn::ic::represented_mode_wavenumber(mx, my, mz, kf);
+                    out[m2] = norm.multiplied_by(pk.evaluate(kmag), "variance");
+                }
+    });
+}
 
 double ref_gravitational_constant() { return cn::cosmology::units::G; }
 

@@ -6,6 +6,9 @@ small seeded inputs; inputs and outputs are stored as .npz fixtures next to this
 script.  Run in a tree where the reference build exists:
 
     make -C oracle ref && python tests/golden/make_golden.py
+
+The IC fixtures also read the reference's example CAMB table
+(examples/quickstart_camb/linear_cb_z0.dat) as data.
 """
 from __future__ import annotations
 
@@ -117,7 +120,63 @@ def main():
         assert R.ref_kick_drift_one(p(pp), p(mm), p(acc), K, D, L) == 0
         cases.append(np.concatenate([pos, mom, acc, [K, D], pp, mm]))
     np.savez_compressed(os.path.join(HERE, "kick_drift.npz"), cases=np.array(cases), L=L)
+    ic_cases(R)
     print("golden vectors written to", HERE)
+
+
+REF_CAMB_TABLE = os.path.join(oracle.REFERENCE_TREE, "examples", "quickstart_camb", "linear_cb_z0.dat")
+
+
+def ic_cases(R):
+    """CosmologyModel growth, CubicSpline, LinearPowerSpectrum::evaluate and the
+    RandomField mode variance, from the reference's classes."""
+    import tempfile
+    from hyowon_amd import ic
+    rng = np.random.default_rng(77)
+    a = np.concatenate([np.geomspace(1e-7, 1.0, 120), rng.uniform(1e-3, 1.0, 60), [1 / 50, 1 / 20, 1.0]])
+    out = {"a": a}
+    for tag, (om, ol) in {"lcdm": (0.3, 0.7), "low": (0.25, 0.75), "eds": (1.0, 0.0)}.items():
+        g = [np.zeros(len(a)) for _ in range(4)]
+        assert R.ref_growth(om, ol, len(a), p(a), *[p(v) for v in g]) == 0
+        out.update({f"{tag}_{k}": v for k, v in zip(("D1", "f1", "D2", "f2"), g)})
+        out[f"{tag}_omega"] = np.array([om, ol])
+    np.savez_compressed(os.path.join(HERE, "ic_growth.npz"), **out)
+
+    x = np.sort(rng.uniform(-3.0, 5.0, 40))
+    y = rng.normal(0.0, 1.0, 40)
+    q = np.concatenate([x, rng.uniform(-4.0, 6.0, 300)])
+    xs = 1e-200 * np.arange(1.0, 41.0)  # knot spacing whose square underflows: normalised build
+    qs = np.concatenate([xs, rng.uniform(0.5e-200, 41e-200, 100)])
+    o, os_ = np.zeros(len(q)), np.zeros(len(qs))
+    assert R.ref_cubic_spline(40, p(x), p(y), len(q), p(q), p(o)) == 0
+    assert R.ref_cubic_spline(40, p(xs), p(y), len(qs), p(qs), p(os_)) == 0
+    np.savez_compressed(os.path.join(HERE, "ic_spline.npz"), x=x, y=y, q=q, out=o, xs=xs, qs=qs, out_s=os_)
+
+    def power_case(name, cosmo6, text, z_start, L, n_side, K, k):
+        with tempfile.NamedTemporaryFile("w", suffix=".dat", delete=False) as f:
+            f.write(text)
+            path = f.name
+        c6 = np.array(cosmo6, np.float64)
+        pk = np.zeros(len(k))
+        assert R.ref_linear_power(p(c6), path.encode(), 0.0, z_start, L, n_side, n_side, K, len(k), p(k), p(pk)) == 0
+        half = n_side // 2
+        var = np.zeros(3 * half * half + 1)
+        assert R.ref_mode_variance(p(c6), path.encode(), 0.0, z_start, L, n_side, n_side, K, p(var)) == 0
+        os.unlink(path)
+        np.savez_compressed(os.path.join(HERE, f"ic_power_{name}.npz"), cosmology=c6, table=np.array(text),
+                            z_start=z_start, L=L, n_side=n_side, K=K, k=k, power=pk, variance=var)
+
+    c = ic.Cosmology()
+    power_case("bbks", [c.h, c.omega_m, c.omega_b, c.omega_lambda, c.sigma8, c.n_s], c.table(), 49.0, 128.0, 64, 31,
+               np.concatenate([np.geomspace(1e-4, 1e2, 400), rng.uniform(1e-4, 1e2, 100)]))
+    # the reference's CAMB example table (data), rows rewritten with 17 digits
+    head, rows = [], []
+    for line in open(REF_CAMB_TABLE):
+        (head if line.lstrip().startswith("#") else rows).append(line.rstrip("\n"))
+    kp = np.array([[float(v) for v in r.split()] for r in rows if r.strip()])
+    text = "\n".join(head + [f"{kv:.17e} {pv:.17e}" for kv, pv in kp]) + "\n"
+    power_case("camb", [0.7, 0.3, 0.05, 0.7, 0.8, 1.0], text, 9.0, 64.0, 32, 15,
+               np.concatenate([np.geomspace(kp[0, 0], kp[-1, 0], 300), kp[::50, 0]]))
 
 
 if __name__ == "__main__":

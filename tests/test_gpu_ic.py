@@ -68,9 +68,7 @@ def test_philox_lpt_displacements_match_numpy(gpu):
     d = pos[0].cpu().numpy() - x_ref
     d -= L * np.round(d / L)
     assert np.abs(d).max() < 1e-9
-    cosmo = ic.Cosmology()
-    _, f = cosmo.growth(a)
-    vel = a * a * 100.0 * float(cosmo.E(a)) * f
+    vel = ic.momentum_factor(a)
     assert np.allclose(mom[0].cpu().numpy(), vel * psi.reshape(-1), rtol=1e-10, atol=1e-12)
 
 
@@ -91,9 +89,7 @@ def test_hip_lpt_matches_numpy(gpu, n):
     kzz = mz * 2 * np.pi / L
     k2 = k1[:, None, None] ** 2 + k1[None, :, None] ** 2 + kzz[None, None, :] ** 2
     k2[0, 0, 0] = 1.0
-    cosmo = ic.Cosmology()
-    _, f = cosmo.growth(a)
-    vel = a * a * 100.0 * float(cosmo.E(a)) * f
+    vel = ic.momentum_factor(a)
     q = np.arange(n) * (L / n)
     for axis in range(3):
         kd = [k1[:, None, None], k1[None, :, None], kzz[None, None, :]][axis]
@@ -118,3 +114,22 @@ def test_hip_lpt_sub_sampled_lattice(gpu):
     for axis in range(3):
         full = pos_full[axis].cpu().numpy().reshape(N, N, N)[::2, ::2, ::2].reshape(-1)
         assert np.array_equal(pos_sub[axis].cpu().numpy(), full)
+
+
+@pytest.mark.parametrize("n,n_side", [(16, 16), (32, 32), (48, 48), (64, 32), (128, 128)])
+def test_hip_lpt_matches_oracle(gpu, oracle_mod, n, n_side):
+    """hy_ic_lpt against the oracle's restatement of the reference's first-order
+    LPT (src/ic/lpt_displacement.cpp:202-384) on the same Philox modes, incl. a
+    lattice coarser than the IC mesh."""
+    from hyowon_amd import ic
+    L, a = 3.0 * n, 1.0 / 50.0
+    dk = ic.philox_modes(n, L, a, seed=5)
+    pf = ic.momentum_factor(a)
+    pos, mom = ic.lpt_from_delta_k_hip(dk, L, a, n_side=n_side)
+    rpos, rmom = oracle_mod.lpt_first_order(dk.cpu().numpy(), L, n_side, pf)
+    for ax in range(3):
+        d = pos[ax].cpu().numpy() - rpos[ax]
+        d -= L * np.round(d / L)
+        disp = np.abs(rmom[ax]).max() / abs(pf)
+        assert np.abs(d).max() <= 1e-12 * disp + 1e-13 * L, ax
+        assert np.abs(mom[ax].cpu().numpy() - rmom[ax]).max() <= 1e-12 * np.abs(rmom[ax]).max(), ax

@@ -97,3 +97,68 @@ def test_gaussian_modes_statistics_and_band_limit(oracle_mod):
     nz = inside.copy()
     nz[0, 0, 0] = False
     assert np.allclose(np.abs(F[nz]) ** 2, 3.0, rtol=1e-14)
+
+
+def _single_mode_spectrum(N, m, amp, theta):
+    """Half spectrum whose c2r (with 1/N^3) is amp cos(2 pi m.i / N + theta); m_z in (0, N/2)."""
+    dk = np.zeros((N, N, N // 2 + 1), np.complex128)
+    dk[m[0] % N, m[1] % N, m[2]] = 0.5 * amp * N ** 3 * np.exp(1j * theta)
+    return dk
+
+
+@pytest.mark.parametrize("N,n_side", [(16, 16), (16, 8), (12, 12)])
+def test_lpt_single_mode_closed_form(oracle_mod, N, n_side):
+    """delta = A cos(k.x + t)  =>  psi = -A (k/k^2) sin(k.x + t) (psi_k = i k/k^2
+    delta_k), sampled on the lattice; x = wrap(q + psi), p = p_factor psi."""
+    L, A, th, pf = 2.0 * N, 0.37, 0.6, 1.7
+    m = (3, -2, 1)
+    dk = _single_mode_spectrum(N, m, A, th)
+    pos, mom = oracle_mod.lpt_first_order(dk, L, n_side, pf)
+    s = N // n_side
+    idx = np.indices((n_side, n_side, n_side)).reshape(3, -1) * s
+    kv = np.array(m) * 2 * np.pi / L
+    k2 = (kv ** 2).sum()
+    phase = 2 * np.pi * (np.array(m)[:, None] * idx).sum(0) / N + th
+    for d in range(3):
+        psi = -A * kv[d] / k2 * np.sin(phase)
+        q = idx[d] // s * (L / n_side)
+        x = np.mod(q + psi, L)
+        dd = pos[d] - x
+        dd -= L * np.round(dd / L)
+        assert np.abs(dd).max() < 1e-13 * L
+        assert np.allclose(mom[d], pf * psi, rtol=0, atol=1e-13)
+
+
+def test_lpt_nyquist_derivative_is_zero(oracle_mod):
+    """A mode on the x Nyquist index has no x displacement (odd derivative
+    zeroed there) while y and z keep -A (k_d/k^2) sin with k^2 including k_x."""
+    N, L, A = 16, 16.0, 0.5
+    m = (8, 3, 2)
+    dk = _single_mode_spectrum(N, m, A, 0.0)
+    pos, mom = oracle_mod.lpt_first_order(dk, L, N, 1.0)
+    idx = np.indices((N, N, N)).reshape(3, -1)
+    kv = np.array(m) * 2 * np.pi / L
+    k2 = (kv ** 2).sum()
+    phase = 2 * np.pi * (np.array(m)[:, None] * idx).sum(0) / N
+    assert np.abs(mom[0]).max() < 1e-14
+    for d in (1, 2):
+        assert np.allclose(mom[d], -A * kv[d] / k2 * np.sin(phase), rtol=0, atol=1e-13)
+
+
+def test_lpt_discrete_divergence_matches_numpy(oracle_mod):
+    """For a random field, the spectral divergence of the oracle's psi (numpy's
+    own FFT) is -delta_k on every mode with no Nyquist index."""
+    N, L = 16, 20.0
+    rng = np.random.default_rng(4)
+    dk = np.fft.rfftn(rng.standard_normal((N, N, N)))
+    dk[0, 0, 0] = 0.0
+    _, mom = oracle_mod.lpt_first_order(dk, L, N, 1.0)
+    m = np.fft.fftfreq(N, 1.0 / N)
+    mz = np.arange(N // 2 + 1)
+    ks = [m[:, None, None], m[None, :, None], mz[None, None, :]]
+    div = np.zeros_like(dk)
+    for d in range(3):
+        div += 1j * (ks[d] * 2 * np.pi / L) * np.fft.rfftn(mom[d].reshape(N, N, N))
+    ok = (np.abs(ks[0]) != N // 2) & (np.abs(ks[1]) != N // 2) & (ks[2] != N // 2)
+    ok = np.broadcast_to(ok, dk.shape)
+    assert np.abs(div[ok] + dk[ok]).max() < 1e-10 * np.abs(dk).max()

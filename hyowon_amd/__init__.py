@@ -16,12 +16,12 @@ from __future__ import annotations
 
 from ._lib import (HY_ASYNC, HY_HAS_NEXT, HY_HAS_PREV, HY_HOST_POINTERS, HY_PREC_FP32_MIXED,
                    HY_PREC_FP64, HyError, LIB_PATH, lib)
-from .pm import (DriftKickIntegrator, LeapfrogIntegrator, PMSolver, ParticleState, TimeStep,
+from .pm import (DriftKickIntegrator, LeapfrogIntegrator, PMForceDiagnostics, PMSolver, ParticleState, TimeStep,
                  TimeStepper, geometric_mean, gravitational_constant, hubble, rho_crit0)
 
 __all__ = [
     "HY_ASYNC", "HY_HAS_NEXT", "HY_HAS_PREV", "HY_HOST_POINTERS", "HY_PREC_FP32_MIXED",
     "HY_PREC_FP64", "HyError", "LIB_PATH", "lib", "DriftKickIntegrator", "LeapfrogIntegrator",
-    "PMSolver", "ParticleState", "TimeStep", "TimeStepper", "geometric_mean",
+    "PMForceDiagnostics", "PMSolver", "ParticleState", "TimeStep", "TimeStepper", "geometric_mean",
     "gravitational_constant", "hubble", "rho_crit0",
 ]

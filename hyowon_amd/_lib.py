@@ -15,6 +15,7 @@ HY_HAS_PREV = 0x2
 HY_HAS_NEXT = 0x4
 HY_ASYNC = 0x8
 HY_REUSE_KEYS = 0x10
+HY_COLLECT_POTENTIAL = 0x20
 
 EXPORTS = [
     "hy_pm_create", "hy_pm_destroy", "hy_pm_get_info", "hy_last_error", "hy_abi_version",
@@ -26,7 +27,7 @@ EXPORTS = [
     "hy_ensemble_check", "hy_ensemble_reset", "hy_ensemble_destroy", "hy_pm_kdk_vjp", "hy_pm_density_vjp",
     "hy_ic_gaussian_modes", "hy_ic_last_error", "hy_pm_power_spectrum", "hy_pm_force_meshes",
     "hy_pm_interpolate", "hy_ic_lpt", "hy_xplane_order_static", "hy_static_last_error",
-    "hy_growth", "hy_linear_power", "hy_ic_mode_variance", "hy_cubic_spline",
+    "hy_growth", "hy_linear_power", "hy_ic_mode_variance", "hy_cubic_spline", "hy_pm_potential_energy",
 ]
 
 HY_FFT_PASSES = 5
@@ -157,6 +158,7 @@ def lib():
     fn("hy_pm_bucket_order", i, vp, u64, vp, vp, vp, vp, vp)
     fn("hy_pm_xplane_order", i, vp, u64, vp, vp, vp)
     fn("hy_pm_check", i, vp)
+    fn("hy_pm_potential_energy", i, vp, pd)
     fn("hy_hubble", d, d, d, d)
     fn("hy_drift_factor", i, d, d, d, d, pd)
     fn("hy_kick_factor", i, d, d, d, d, pd)

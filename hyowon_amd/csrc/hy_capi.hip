@@ -90,6 +90,10 @@ struct hy_pm_ctx {
     uint64_t device_bytes = 0;
     double green_num = 0.0;
     double last_input_mass = 0.0, last_deposited_mass = 0.0;
+    bool energy_valid = false;       // the last hy_pm_force collected the potential energy
+    double last_potential_energy = 0.0;
+    double* energy_partials = nullptr;  // potential-energy workgroup sums (allocated on first use)
+    uint64_t energy_cap = 0;            // workgroups the partials hold
     double mass_unit = 1.0;  // fixed-point deposit unit (uniform mass or max explicit mass)
     int plane_group = 0;     // kz planes per y/x/y^-1 launch group (0 = all planes at once)
     // Bucket keys/ranks/counts computed by the previous fused step for its output
@@ -592,6 +596,30 @@ void launch_gather(hy_pm_ctx* c, hy::GatherArgs<T, PT> a) {
     launch_check("k_gather_kdk");
 }
 
+// Potential-energy diagnostic of the potential in R (pure PM, single box):
+// workgroup partial sums, then 0.5 * their ordered sum into
+// energy_partials[energy_cap].
+void launch_potential_energy(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z,
+                             const double* masses, double uniform) {
+    const uint64_t blocks = blocks_for(n, hy::ENERGY_BLOCK);
+    if (blocks > c->energy_cap) {
+        const uint64_t cap = std::max<uint64_t>(blocks, blocks_for(c->cap, hy::ENERGY_BLOCK));
+        c->energy_partials = static_cast<double*>(c->alloc((cap + 1) * sizeof(double)));
+        c->energy_cap = cap;
+    }
+    if (c->prec == HY_PREC_FP64)
+        hipLaunchKernelGGL(hy::k_potential_partials<double>, dim3((uint32_t)blocks), dim3(hy::ENERGY_THREADS), 0,
+                           c->stream, c->geom, n, x, y, z, masses, uniform, static_cast<const double*>(c->R),
+                           c->energy_partials, c->err);
+    else
+        hipLaunchKernelGGL(hy::k_potential_partials<float>, dim3((uint32_t)blocks), dim3(hy::ENERGY_THREADS), 0,
+                           c->stream, c->geom, n, x, y, z, masses, uniform, static_cast<const float*>(c->R),
+                           c->energy_partials, c->err);
+    hipLaunchKernelGGL(hy::k_potential_total, dim3(1), dim3(64), 0, c->stream, c->energy_partials, blocks,
+                       c->energy_partials + c->energy_cap, c->err);
+    launch_check("k_potential_partials");
+}
+
 // Mass conservation (src/math/cic_mass_conservation.cpp:25-63).  The FP32
 // deposit sums cells in binary32, so its bound uses binary32 epsilon per cell
 // plus the binary64 bound of the reference.
@@ -947,6 +975,15 @@ int hy_pm_get_info(const hy_pm_ctx* c, hy_pm_info* info) {
     });
 }
 
+int hy_pm_potential_energy(const hy_pm_ctx* c, double* out) {
+    return guarded([&] {
+        if (!c || !out) fail(HY_EINVAL, "hy_pm_potential_energy: null argument");
+        if (!c->energy_valid)
+            fail(HY_ELOGIC, "the last hy_pm_force call did not collect the potential energy");
+        *out = c->last_potential_energy;
+    });
+}
+
 int hy_pm_check(hy_pm_ctx* c) {
     return guarded([&] {
         if (!c) fail(HY_EINVAL, "null context");
@@ -1159,6 +1196,13 @@ int hy_pm_force(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, cons
         }
         const uint64_t nper = set_boxes(c, n);
         if (masses && c->nbox > 1) fail(HY_EINVAL, "explicit masses take a single-box context");
+        const bool collect = (flags & HY_COLLECT_POTENTIAL) != 0;
+        c->energy_valid = false;
+        if (collect) {
+            require_single_box(c, "potential-energy collection");
+            if (c->method != HY_METHOD_PURE_PM)
+                fail(HY_EINVAL, "potential-energy collection takes a pure-PM context");
+        }
         const double M = input_mass(c, nper, masses, uniform, host);
         const double *dx_ = x, *dy_ = y, *dz_ = z, *dm = masses;
         if (host) {
@@ -1201,6 +1245,7 @@ int hy_pm_force(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, cons
             else interp(float{});
         } else {
             run_fft(c);
+            if (collect) launch_potential_energy(c, n, dx_, dy_, dz_, dm, uniform);
             if (c->prec == HY_PREC_FP64) {
                 hy::GatherArgs<double, double> a{};
                 fill(a);
@@ -1219,6 +1264,12 @@ int hy_pm_force(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, cons
         end_call(c);
         const auto st = fetch_status(c);
         raise_errors(st.err);
+        if (collect) {
+            hip_check(hipMemcpyAsync(&c->last_potential_energy, c->energy_partials + c->energy_cap, sizeof(double),
+                                     hipMemcpyDeviceToHost, c->stream), "energy copy");
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+            c->energy_valid = true;
+        }
         // commit: acc += g (only after every check passed)
         double *cx = ax, *cy = ay, *cz = az;
         if (host) {

@@ -1232,6 +1232,97 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs<T> a) {
     if (bad) atomicOr(a.err, ERR_FORCE_NONFINITE);
 }
 
+// Potential-energy diagnostic: sum_p m_p phi(x_p) with phi CIC-interpolated
+// (CICMassAssignment::interpolate_mass_weighted_sum_full_mesh,
+// src/mesh/mass_assignment_gather.inc:283-399; the caller halves it as
+// src/gravity/pm_solver.cpp:817-841 does).  Per particle, in binary64 as the
+// reference (its accumulation type is binary64): value = sum over corners
+// (x outer, z inner) of phi_c * ((wx wy) wz), then m * value, with the
+// reference's lost-product checks (ERR_INTERP_UNDERFLOW).  The summation is the
+// reference's deterministic blocked sum (include/cosmo_nbody/math/
+// deterministic_sum.hpp:48-104): workgroup b forms the terms of particles
+// [b*4096, (b+1)*4096) in LDS, then one lane adds them in index order with
+// Neumaier compensation; k_potential_total adds the block sums in order.
+constexpr int ENERGY_THREADS = 256;
+constexpr int ENERGY_BLOCK = 4096;
+
+__device__ __forceinline__ void neumaier_add(double v, double& sum, double& comp) {
+    const double t = sum + v;
+    if (fabs(sum) >= fabs(v)) comp += (sum - t) + v;
+    else comp += (v - t) + sum;
+    sum = t;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(ENERGY_THREADS) k_potential_partials(Geom g, uint64_t n, const double* __restrict__ x,
+                                                                      const double* __restrict__ y,
+                                                                      const double* __restrict__ z,
+                                                                      const double* __restrict__ masses,
+                                                                      double uniform_mass, const T* __restrict__ phi,
+                                                                      double* __restrict__ partials,
+                                                                      uint32_t* __restrict__ err) {
+    __shared__ double terms[ENERGY_BLOCK];
+    const uint64_t b0 = (uint64_t)blockIdx.x * ENERGY_BLOCK;
+    const uint64_t b1 = n < b0 + ENERGY_BLOCK ? n : b0 + ENERGY_BLOCK;
+    const int N = g.N;
+    const uint64_t NN = (uint64_t)N * (uint64_t)N;
+    bool lost = false, bad = false;
+    for (uint64_t p = b0 + threadIdx.x; p < b1; p += ENERGY_THREADS) {
+        int ix, iy, iz;
+        double fx, fy, fz;
+        double weighted = 0.0;
+        if (!cic_axes(x[p], y[p], z[p], g, ix, iy, iz, fx, fy, fz)) {
+            bad = true;
+        } else {
+            const double m = masses ? masses[p] : uniform_mass;
+            const double wx[2] = {1.0 - fx, fx}, wy[2] = {1.0 - fy, fy}, wz[2] = {1.0 - fz, fz};
+            const uint64_t xs[2] = {(uint64_t)ix, (uint64_t)wrap_cell(ix + 1, N)};
+            const uint64_t ys[2] = {(uint64_t)iy, (uint64_t)wrap_cell(iy + 1, N)};
+            const uint64_t zs[2] = {(uint64_t)iz, (uint64_t)wrap_cell(iz + 1, N)};
+            double value = 0.0;
+#pragma unroll
+            for (int ox = 0; ox < 2; ++ox)
+#pragma unroll
+                for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+                    for (int oz = 0; oz < 2; ++oz) {
+                        const double fv = (double)phi[xs[ox] * NN + ys[oy] * (uint64_t)N + zs[oz]];
+                        const double xy = wx[ox] * wy[oy];
+                        const double w = xy * wz[oz];
+                        if (wx[ox] != 0.0 && wy[oy] != 0.0 && wz[oz] != 0.0 && w == 0.0 && fv != 0.0 && m != 0.0)
+                            lost = true;
+                        const double c = fv * w;
+                        if (fv != 0.0 && w != 0.0 && c == 0.0 && m != 0.0) lost = true;
+                        value += c;
+                    }
+            weighted = m * value;
+            if (m != 0.0 && value != 0.0 && weighted == 0.0) lost = true;
+        }
+        terms[p - b0] = weighted;
+    }
+    if (bad) atomicOr(err, ERR_POSITION_NONFINITE);
+    if (lost) atomicOr(err, ERR_INTERP_UNDERFLOW);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0, c = 0.0;
+        const uint32_t count = (uint32_t)(b1 - b0);
+        for (uint32_t i = 0; i < count; ++i) neumaier_add(terms[i], s, c);
+        partials[blockIdx.x] = s + c;
+    }
+}
+
+// 0.5 * (Neumaier sum of the workgroup partials, in order) -> out[0]
+__global__ void k_potential_total(const double* __restrict__ partials, uint64_t count, double* __restrict__ out,
+                                  uint32_t* __restrict__ err) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0, c = 0.0;
+    for (uint64_t i = 0; i < count; ++i) neumaier_add(partials[i], s, c);
+    const double total = s + c;
+    const double half = 0.5 * total;
+    if (!isfinite(total) || (total != 0.0 && half == 0.0)) atomicOr(err, ERR_INTERP_UNDERFLOW);
+    out[0] = half;
+}
+
 // 1LPT displacement spectrum of one axis (src/ic/lpt_displacement.cpp:306-340):
 // psi_k = (-Im d, Re d) * k_d / k^2 with k_d = 0 on the Nyquist index of the
 // axis and psi_k = 0 where k^2 = 0, times 1/N^3 (the inverse transform's

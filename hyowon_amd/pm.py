@@ -13,7 +13,7 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from ._lib import (FFT_PASS_NAMES, HY_ASYNC, HY_FFT_PASSES, HY_HAS_NEXT, HY_HAS_PREV,
+from ._lib import (FFT_PASS_NAMES, HY_ASYNC, HY_COLLECT_POTENTIAL, HY_FFT_PASSES, HY_HAS_NEXT, HY_HAS_PREV,
                    HY_HOST_POINTERS, HY_METHOD_PURE_PM, HY_METHOD_TREEPM_LONG_RANGE,
                    HY_PREC_FP32_MIXED, HY_PREC_FP64, HY_REUSE_KEYS, check,
                    hy_fft_tuning, hy_pk_bin, hy_pk_options, hy_pm_desc, hy_pm_info, lib)
@@ -140,6 +140,14 @@ def _tuning_dict(t) -> dict:
 
 
 # --------------------------------------------------------------------------
+@dataclass
+class PMForceDiagnostics:
+    """gravity::PMForceDiagnostics (include/cosmo_nbody/gravity/pm_solver.hpp:30-35):
+    sum_i 0.5 m_i phi_i before the cosmological 1/a conversion."""
+    potential_energy_comoving: float
+    cic_self_energy_comoving: Optional[float] = None
+
+
 class PMSolver:
     """gravity::PMSolver (include/cosmo_nbody/gravity/pm_solver.hpp:53-70),
     pure PM (discrete-Laplacian Green function, centered-gradient gather).
@@ -238,14 +246,24 @@ class PMSolver:
         return torch.float64 if self.precision == HY_PREC_FP64 else torch.float32
 
     # -- reference-shaped force entry
-    def compute_forces(self, x, y, z, masses, uniform_mass, acc_x, acc_y, acc_z) -> None:
-        """Adds the PM acceleration into acc_* (PMSolver::compute_forces)."""
+    def compute_forces(self, x, y, z, masses, uniform_mass, acc_x, acc_y, acc_z,
+                       collect_potential_energy: bool = False) -> Optional["PMForceDiagnostics"]:
+        """Adds the PM acceleration into acc_* (PMSolver::compute_forces); with
+        collect_potential_energy returns PMForceDiagnostics (0.5 sum m phi),
+        else None."""
         n = len(x)
         host = _is_host(x)
         flags = HY_HOST_POINTERS if host else 0
+        if collect_potential_energy:
+            flags |= HY_COLLECT_POTENTIAL
         um = float(uniform_mass) if uniform_mass is not None else 0.0
         check(lib().hy_pm_force(self._h, n, _ptr(x), _ptr(y), _ptr(z), _ptr(masses), um,
                                 _ptr(acc_x), _ptr(acc_y), _ptr(acc_z), flags))
+        if not collect_potential_energy:
+            return None
+        e = C.c_double()
+        check(lib().hy_pm_potential_energy(self._h, C.byref(e)))
+        return PMForceDiagnostics(e.value)
 
     # -- stages (device tensors)
     def density(self, x, y, z, masses=None, uniform_mass: float = 1.0):

@@ -66,9 +66,46 @@ private:
     bool deconvolve_cic_;
 };
 
+// mesh::MeshGeometry (include/cosmo_nbody/mesh/mesh_geometry.hpp:18-72): the
+// periodic box and its cells per axis.
+class MeshGeometry {
+public:
+    MeshGeometry(double box_size, std::size_t grid_size) : L_(box_size), N_(grid_size) {
+        if (!(box_size > 0.0) || box_size == std::numeric_limits<double>::infinity())
+            throw std::invalid_argument("MeshGeometry box size must be finite and positive");
+        if (grid_size == 0) throw std::invalid_argument("MeshGeometry grid size must be positive");
+    }
+    double box_size() const noexcept { return L_; }
+    std::size_t grid_size() const noexcept { return N_; }
+    double cell_size() const noexcept { return L_ / static_cast<double>(N_); }
+
+private:
+    double L_;
+    std::size_t N_;
+};
+
 }  // namespace mesh
 
+namespace config {
+
+// config::MemoryPolicyParams (include/cosmo_nbody/config/memory_policy.hpp):
+// scratch placement.  The GPU workspace lives in HBM whatever the mode says.
+enum class ScratchMode { Memory, Disk };
+struct MemoryPolicyParams {
+    ScratchMode ic_scratch_mode{ScratchMode::Memory};
+    ScratchMode evolution_scratch_mode{ScratchMode::Memory};
+    std::string scratch_directory{};
+};
+
+}  // namespace config
+
 namespace gravity {
+
+// gravity::PMForceDiagnostics (include/cosmo_nbody/gravity/pm_solver.hpp:30-35).
+struct PMForceDiagnostics {
+    double potential_energy_comoving{0.0};
+    std::optional<double> cic_self_energy_comoving;
+};
 
 // gravity::PMSolver: pure PM (discrete-Laplacian Green function,
 // centered-gradient CIC gather) or the TreePM long-range force.
@@ -81,6 +118,64 @@ public:
                    device, stream) {}
     PMSolver(double box_size, std::size_t mesh_size, mesh::PMForceMethod method, std::size_t max_particles,
              Precision precision = Precision::FP32Mixed, int device = 0, void* stream = nullptr) {
+        desc_ = describe(box_size, mesh_size, method, precision, device, stream);
+        create(max_particles);
+    }
+    // The reference's constructor (include/cosmo_nbody/gravity/pm_solver.hpp:53-57).
+    // As the reference's ensure_workspace (src/gravity/pm_solver.cpp:388-492),
+    // the device workspace is allocated on the first force call, for that
+    // call's particle count, and grown when a later call needs more.  The
+    // geometry is checked here.  mpi_global_reduce selects the reference's
+    // multi-rank force, which this single-GPU build does not provide.  The
+    // precision defaults to binary64, the reference's arithmetic.
+    PMSolver(const mesh::MeshGeometry& geom, mesh::PMForceMethod method, bool mpi_global_reduce,
+             config::MemoryPolicyParams /*memory_policy: the workspace is device memory*/,
+             Precision precision = Precision::FP64, int device = 0, void* stream = nullptr) {
+        if (mpi_global_reduce)
+            throw std::invalid_argument("PMSolver MPI-global reduction is not available on this single-GPU solver");
+        desc_ = describe(geom.box_size(), geom.grid_size(), method, precision, device, stream);
+        create(1);  // validates the geometry now; resized on the first call
+        capacity_ = 0;
+    }
+    ~PMSolver() {
+        if (ctx_) hy_pm_destroy(ctx_);
+    }
+    PMSolver(const PMSolver&) = delete;
+    PMSolver& operator=(const PMSolver&) = delete;
+    PMSolver(PMSolver&& o) noexcept : desc_(o.desc_), ctx_(o.ctx_), capacity_(o.capacity_) { o.ctx_ = nullptr; }
+
+    // Adds the PM acceleration into acc_* (reference semantics: transactional,
+    // explicit masses when `masses` is non-empty, else the uniform mass).  With
+    // collect_potential_energy the result carries 0.5 sum m phi
+    // (src/gravity/pm_solver.cpp:817-841); otherwise it is empty.
+    std::optional<PMForceDiagnostics> compute_forces(std::span<const double> pos_x, std::span<const double> pos_y,
+                                                     std::span<const double> pos_z, std::span<const double> masses,
+                                                     std::optional<double> uniform_mass, std::span<double> acc_x,
+                                                     std::span<double> acc_y, std::span<double> acc_z,
+                                                     bool collect_potential_energy = false) {
+        const std::size_t n = pos_x.size();
+        if (pos_y.size() != n || pos_z.size() != n || acc_x.size() != n || acc_y.size() != n ||
+            acc_z.size() != n)
+            throw std::invalid_argument("PMSolver position and acceleration component sizes must match");
+        if (!uniform_mass && masses.size() != n)
+            throw std::invalid_argument("PMSolver explicit mass count must match particle count");
+        if (n > capacity_) create(n);
+        const uint32_t flags = HY_HOST_POINTERS | (collect_potential_energy ? HY_COLLECT_POTENTIAL : 0u);
+        throw_status(hy_pm_force(ctx_, n, pos_x.data(), pos_y.data(), pos_z.data(),
+                                 uniform_mass ? nullptr : masses.data(),
+                                 uniform_mass.value_or(0.0), acc_x.data(), acc_y.data(),
+                                 acc_z.data(), flags));
+        if (!collect_potential_energy) return std::nullopt;
+        PMForceDiagnostics d;
+        throw_status(hy_pm_potential_energy(ctx_, &d.potential_energy_comoving));
+        return d;
+    }
+
+    hy_pm_ctx* handle() const noexcept { return ctx_; }
+
+private:
+    static hy_pm_desc describe(double box_size, std::size_t mesh_size, const mesh::PMForceMethod& method,
+                               Precision precision, int device, void* stream) {
         hy_pm_desc d{};
         d.box_size = box_size;
         d.mesh_size = mesh_size;
@@ -88,40 +183,24 @@ public:
         d.precision = static_cast<int>(precision);
         d.device = device;
         d.stream = stream;
-        d.max_particles = max_particles;
         d.method = method.kind();
         d.split_scale = method.split_scale();
-        throw_status(hy_pm_create(&d, &ctx_));
+        return d;
     }
-    ~PMSolver() {
+    // (re)creates the context for `particles` (the old one is kept on failure)
+    void create(std::size_t particles) {
+        hy_pm_desc d = desc_;
+        d.max_particles = particles;
+        hy_pm_ctx* c = nullptr;
+        throw_status(hy_pm_create(&d, &c));
         if (ctx_) hy_pm_destroy(ctx_);
-    }
-    PMSolver(const PMSolver&) = delete;
-    PMSolver& operator=(const PMSolver&) = delete;
-    PMSolver(PMSolver&& o) noexcept : ctx_(o.ctx_) { o.ctx_ = nullptr; }
-
-    // Adds the PM acceleration into acc_* (reference semantics: transactional,
-    // explicit masses when `masses` is non-empty, else the uniform mass).
-    void compute_forces(std::span<const double> pos_x, std::span<const double> pos_y,
-                        std::span<const double> pos_z, std::span<const double> masses,
-                        std::optional<double> uniform_mass, std::span<double> acc_x,
-                        std::span<double> acc_y, std::span<double> acc_z) {
-        const std::size_t n = pos_x.size();
-        if (pos_y.size() != n || pos_z.size() != n || acc_x.size() != n || acc_y.size() != n ||
-            acc_z.size() != n)
-            throw std::invalid_argument("PMSolver position and acceleration component sizes must match");
-        if (!uniform_mass && masses.size() != n)
-            throw std::invalid_argument("PMSolver explicit mass count must match particle count");
-        throw_status(hy_pm_force(ctx_, n, pos_x.data(), pos_y.data(), pos_z.data(),
-                                 uniform_mass ? nullptr : masses.data(),
-                                 uniform_mass.value_or(0.0), acc_x.data(), acc_y.data(),
-                                 acc_z.data(), HY_HOST_POINTERS));
+        ctx_ = c;
+        capacity_ = particles;
     }
 
-    hy_pm_ctx* handle() const noexcept { return ctx_; }
-
-private:
+    hy_pm_desc desc_{};
     hy_pm_ctx* ctx_ = nullptr;
+    std::size_t capacity_ = 0;
 };
 
 }  // namespace gravity

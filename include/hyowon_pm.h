@@ -64,6 +64,8 @@ typedef enum {
 #define HY_ASYNC          0x8u  /* do not synchronise; errors are reported by hy_pm_check */
 #define HY_REUSE_KEYS     0x10u /* hy_pm_kdk: x_in/y_in/z_in are the previous hy_pm_kdk call's
                                    outputs on this context; reuse the bucket keys it made */
+#define HY_COLLECT_POTENTIAL 0x20u /* hy_pm_force: also evaluate 0.5 sum m phi(x) (pure PM,
+                                      single box); read it with hy_pm_potential_energy */
 
 typedef struct {
     double box_size;        /* L (Mpc/h) */
@@ -133,6 +135,14 @@ int hy_pm_kdk(hy_pm_ctx* ctx, uint64_t n, const double* x_in, const double* y_in
               void* pz_out, const double* masses, double uniform_mass, double k_prev,
               double k_next, double d_next, double* acc_x, double* acc_y, double* acc_z,
               uint32_t flags);
+
+/* PMForceDiagnostics::potential_energy_comoving of the last hy_pm_force call
+ * made with HY_COLLECT_POTENTIAL (include/cosmo_nbody/gravity/pm_solver.hpp:30-35;
+ * src/gravity/pm_solver.cpp:817-841 over src/mesh/mass_assignment_gather.inc:
+ * 283-399): 0.5 sum_i m_i phi(x_i), phi CIC-interpolated from the force's
+ * potential, summed in binary64 in a fixed order.  logic_error if that call did
+ * not collect it. */
+int hy_pm_potential_energy(const hy_pm_ctx* ctx, double* potential_energy_comoving);
 
 /* Stage entry points (device pointers; mesh arrays use the context precision,
  * row-major (ix*N+iy)*N+iz as include/cosmo_nbody/mesh/mesh_geometry.hpp:74-79). */

@@ -120,6 +120,7 @@ def lib():
         L.fn("or_interpolate_add", C.c_int, _u64, _pd, _pd, _pd, _pd, _d, _u64, _pd)
         L.fn("or_pm_force_treepm", C.c_int, _u64, _pd, _pd, _pd, _pd, _d, _d, _u64, _d, _pd, _pd,
              _pd, _pd)
+        L.fn("or_potential_energy", C.c_int, _u64, _pd, _pd, _pd, _pd, _pd, _d, _d, _u64, _pd)
         L.fn("or_lpt_first_order", C.c_int, _u64, _pd, _u64, _d, _d, _pd, _pd, _pd, _pd, _pd, _pd)
         L.fn("or_set_threads", None, C.c_int)
         L.fn("or_get_threads", C.c_int)
@@ -172,6 +173,7 @@ def ref():
         L.fn("ref_gravitational_constant", _d)
         L.fn("ref_rho_crit0", _d)
         L.fn("ref_philox4x32_10", None, _pu32, _pu32, _pu32)
+        L.fn("ref_potential_energy", C.c_int, _u64, _pd, _pd, _pd, _pd, _pd, _d, _d, _u64, _pd)
         L.fn("ref_growth", C.c_int, _d, _d, _u64, _pd, _pd, _pd, _pd, _pd)
         L.fn("ref_cubic_spline", C.c_int, _u64, _pd, _pd, _u64, _pd, _pd)
         L.fn("ref_linear_power", C.c_int, _pd, C.c_char_p, _d, _d, _d, _u64, _u64, _u64, _u64, _pd, _pd)
@@ -424,3 +426,14 @@ def lpt_first_order(delta_k, L, n_side, p_factor):
     _check(lib().or_lpt_first_order(N, dk.ctypes.data_as(_pd), n_side, L, p_factor, *[_ptr(o) for o in out]),
            "or_lpt_first_order")
     return np.stack(out[:3]), np.stack(out[3:])
+
+
+def potential_energy(phi, x, y, z, L, masses=None, uniform_mass=1.0):
+    """0.5 sum_p m_p phi(x_p) (PMForceDiagnostics::potential_energy_comoving)."""
+    N = phi.shape[0]
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    m = _f64(masses) if masses is not None else None
+    out = _d(0)
+    _check(lib().or_potential_energy(len(x), _ptr(_f64(phi).reshape(-1)), _ptr(x), _ptr(y), _ptr(z), _ptr(m),
+                                     uniform_mass, L, N, C.byref(out)), "or_potential_energy")
+    return out.value

@@ -914,6 +914,85 @@ int or_interpolate_add(uint64_t n, const double* field, const double* x, const d
     return OR_OK;
 }
 
+/* Potential-energy diagnostic sum_p m_p phi(x_p) (core::Accum is binary64):
+ * CICMassAssignment::interpolate_mass_weighted_sum_full_mesh
+ * (src/mesh/mass_assignment_gather.inc:283-399) over the fixed-block Neumaier sum
+ * of include/cosmo_nbody/math/deterministic_sum.hpp:48-104 (blocks of 4096);
+ * masses NULL selects the uniform mass.  *out = 0.5 * sum (pm_solver.cpp:817-841). */
+static void neumaier(double v, double* sum, double* comp) {
+    const double t = *sum + v;
+    if (fabs(*sum) >= fabs(v)) *comp += (*sum - t) + v;
+    else *comp += (v - t) + *sum;
+    *sum = t;
+}
+
+int or_potential_energy(uint64_t n, const double* field, const double* x, const double* y, const double* z,
+                        const double* masses, double uniform_mass, double L, uint64_t N, double* out) {
+    const double dx = L / (double)N;
+    for (uint64_t p = 0; p < n; ++p)
+        if (!isfinite(x[p]) || !isfinite(y[p]) || !isfinite(z[p])) return OR_EINVAL;
+    const uint64_t block = 4096;
+    const uint64_t nb = n ? 1 + (n - 1) / block : 0;
+    double* part = (double*)calloc(nb ? nb : 1, sizeof(double));
+    if (!part) return OR_ERUNTIME;
+    int lost = 0, bad = 0;
+#pragma omp parallel for schedule(static) num_threads(OR_NT) reduction(|: lost, bad)
+    for (int64_t bi = 0; bi < (int64_t)nb; ++bi) {
+        const uint64_t b0 = (uint64_t)bi * block, b1 = b0 + block < n ? b0 + block : n;
+        double sum = 0.0, comp = 0.0;
+        for (uint64_t p = b0; p < b1; ++p) {
+            int64_t ix, iy, iz;
+            double fx, fy, fz;
+            if (or_cic_coordinate(x[p], L, dx, N, &ix, &fx) != OR_OK
+                || or_cic_coordinate(y[p], L, dx, N, &iy, &fy) != OR_OK
+                || or_cic_coordinate(z[p], L, dx, N, &iz, &fz) != OR_OK) {
+                bad = 1;
+                continue;
+            }
+            const double wx[2] = {1.0 - fx, fx}, wy[2] = {1.0 - fy, fy}, wz[2] = {1.0 - fz, fz};
+            const double m = masses ? masses[p] : uniform_mass;
+            double v = 0.0;
+            int lost_p = 0;
+            for (int ox = 0; ox < 2; ++ox) {
+                const uint64_t gx = wrap_index(ix + ox, N);
+                for (int oy = 0; oy < 2; ++oy) {
+                    const uint64_t gy = wrap_index(iy + oy, N);
+                    for (int oz = 0; oz < 2; ++oz) {
+                        const uint64_t gz = wrap_index(iz + oz, N);
+                        const double f = field[(gx * N + gy) * N + gz];
+                        const double xy = wx[ox] * wy[oy];
+                        const double w = xy * wz[oz];
+                        if (wx[ox] != 0.0 && wy[oy] != 0.0 && wz[oz] != 0.0 && w == 0.0 && f != 0.0 && m != 0.0)
+                            lost_p = 1;
+                        const double c = f * w;
+                        if (f != 0.0 && w != 0.0 && c == 0.0 && m != 0.0) lost_p = 1;
+                        v += c;
+                    }
+                }
+            }
+            const double weighted = m * v;
+            if (m != 0.0 && v != 0.0 && weighted == 0.0) lost_p = 1;
+            if (lost_p) {
+                lost = 1;
+                continue;
+            }
+            neumaier(weighted, &sum, &comp);
+        }
+        part[bi] = sum + comp;
+    }
+    double total = 0.0, comp = 0.0;
+    for (uint64_t b = 0; b < nb; ++b) neumaier(part[b], &total, &comp);
+    free(part);
+    if (bad) return OR_ELOGIC;
+    if (lost) return OR_EUNDERFLOW;
+    const double sum = total + comp;
+    if (!isfinite(sum)) return OR_EUNDERFLOW;
+    const double half = 0.5 * sum;
+    if (sum != 0.0 && half == 0.0) return OR_EUNDERFLOW;
+    *out = half;
+    return OR_OK;
+}
+
 /* The long-range PM force of TreePM (PMForceMethod::treepm_long_range):
  * deposit, density source, r2c, continuum Green x exp(-k^2 r_s^2) / W^2, then
  * per axis -i k_axis phi (k_axis = 0 on the Nyquist index), c2r with 1/N^3 and

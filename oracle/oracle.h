@@ -240,6 +240,11 @@ int or_power_spectrum(uint64_t n, const double* x, const double* y, const double
  * divergence against numpy's FFT (tests/test_oracle_ic_field.py). */
 int or_lpt_first_order(uint64_t N, const double* delta_k, uint64_t n_side, double L, double p_factor,
                        double* x, double* y, double* z, double* px, double* py, double* pz);
+/* or_potential_energy: 0.5 sum_p m_p phi(x_p), the reference's binary64
+ * blocked Neumaier sum; bit-exact against the reference build
+ * (tests/test_oracle_reference.py). */
+int or_potential_energy(uint64_t n, const double* field, const double* x, const double* y, const double* z,
+                        const double* masses, double uniform_mass, double L, uint64_t N, double* out);
 int or_pk_edges(double minimum, double maximum, int bins, double* edges);
 int or_pk_bin_index(const double* edges, int bins, double v);
 long double or_pk_window_axis(double k, double dx);

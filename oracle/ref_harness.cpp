@@ -351,6 +351,22 @@ int ref_interpolate_add(uint64_t n, const double* mesh, const double* x, const d
     });
 }
 
+// 0.5 * CICMassAssignment::interpolate_mass_weighted_sum_full_mesh, as PMSolver reports it.
+int ref_potential_energy(uint64_t n, const double* mesh, const double* x, const double* y, const double* z,
+                         const double* masses, double uniform_mass, double L, uint64_t N, double* out) {
+    return guarded([&] {
+        const cn::mesh::MeshGeometry geom(L, N);
+        cn::mesh::CICMassAssignment cic(geom);
+        cn::mesh::RealField field(const_cast<double*>(mesh), geom.real_size(), cn::mesh::external_mesh_storage);
+        std::optional<double> um;
+        if (!masses) um = uniform_mass;
+        const double s = cic.interpolate_mass_weighted_sum_full_mesh(
+            field, std::span<const double>(x, n), std::span<const double>(y, n), std::span<const double>(z, n),
+            masses ? std::span<const double>(masses, n) : std::span<const double>(), um);
+        *out = 0.5 * s;
+    });
+}
+
 int ref_gather(uint64_t n, const double* phi, const double* x, const double* y, const double* z,
                double L, uint64_t N, double* ax, double* ay, double* az) {
     return guarded([&] {

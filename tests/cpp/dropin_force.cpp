@@ -7,7 +7,9 @@
 //
 // usage: dropin_force <in.bin> <out.bin> <L> <N> <precision 0|1> <uniform_mass>
 //   in.bin : uint64 n, then x[n], y[n], z[n] (binary64)
-//   out.bin: ax[n], ay[n], az[n], uint64 steps, then steps x (K1, D, K2)
+//   out.bin: ax[n], ay[n], az[n], uint64 steps, then steps x (K1, D, K2), then the
+//            forces of the reference-shaped solver (two calls, halved) and its
+//            collected potential energy
 #include "hyowon/pm_solver.hpp"
 
 #include <cmath>
@@ -48,6 +50,30 @@ int main(int argc, char** argv) {
             factors.push_back(dk.drift_factor(s->a_begin, s->a_end));
             factors.push_back(dk.kick_factor(s->a_half, s->a_end));
         }
+        // the reference-shaped constructor (geometry, method, MPI flag, memory
+        // policy): workspace allocated on the first call, potential energy collected
+        hyowon::gravity::PMSolver shaped(hyowon::mesh::MeshGeometry(L, N),
+                                         hyowon::mesh::PMForceMethod::pure_pm(false), false,
+                                         hyowon::config::MemoryPolicyParams{}, prec);
+        std::vector<double> bx(n, 0.0), by(n, 0.0), bz(n, 0.0);
+        const auto diag = shaped.compute_forces(x, y, z, {}, m, bx, by, bz, true);
+        if (!diag) throw std::logic_error("potential energy was not collected");
+        if (shaped.compute_forces(x, y, z, {}, m, bx, by, bz)) throw std::logic_error("unrequested diagnostics");
+        for (std::size_t i = 0; i < n; ++i) {
+            bx[i] *= 0.5;
+            by[i] *= 0.5;
+            bz[i] *= 0.5;
+        }
+        bool mpi_threw = false;
+        try {
+            hyowon::gravity::PMSolver mpi(hyowon::mesh::MeshGeometry(L, N),
+                                          hyowon::mesh::PMForceMethod::pure_pm(false), true,
+                                          hyowon::config::MemoryPolicyParams{});
+        } catch (const std::invalid_argument&) {
+            mpi_threw = true;
+        }
+        if (!mpi_threw) throw std::logic_error("an MPI-global solver was accepted");
+
         // an invalid mesh size must surface as the reference's exception class
         bool threw = false;
         try {
@@ -65,6 +91,10 @@ int main(int argc, char** argv) {
         const uint64_t steps = factors.size() / 3;
         std::fwrite(&steps, 8, 1, o);
         std::fwrite(factors.data(), 8, factors.size(), o);
+        std::fwrite(bx.data(), 8, n, o);
+        std::fwrite(by.data(), 8, n, o);
+        std::fwrite(bz.data(), 8, n, o);
+        std::fwrite(&diag->potential_energy_comoving, 8, 1, o);
         std::fclose(o);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "dropin_force: %s\n", e.what());

@@ -34,10 +34,17 @@ def test_cpp_dropin_force_and_schedule(gpu, oracle_mod, tmp_path, prec, tol):
     raw = np.fromfile(out, dtype=np.float64)
     acc = raw[: 3 * n].reshape(3, n)
     steps = int(raw[3 * n:3 * n + 1].view(np.uint64)[0])
-    fac = raw[3 * n + 1:].reshape(steps, 3)
-    ref = oracle_mod.pm_force(pos[0], pos[1], pos[2], L, N, uniform_mass=m)
+    fac = raw[3 * n + 1:3 * n + 1 + 3 * steps].reshape(steps, 3)
+    rest = raw[3 * n + 1 + 3 * steps:]
+    acc2 = rest[: 3 * n].reshape(3, n)
+    energy = rest[3 * n]
+    ref, phi = oracle_mod.pm_force(pos[0], pos[1], pos[2], L, N, uniform_mass=m, return_phi=True)
     err = math.sqrt(((acc - ref) ** 2).sum() / (ref ** 2).sum())
     assert err < tol, err
+    err2 = math.sqrt(((acc2 - ref) ** 2).sum() / (ref ** 2).sum())
+    assert err2 < tol, err2
+    e_ref = oracle_mod.potential_energy(phi, pos[0], pos[1], pos[2], L, uniform_mass=m)
+    assert abs(energy - e_ref) <= (1e-12 if prec == 0 else 1e-5) * abs(e_ref), (energy, e_ref)
     g = np.load(os.path.join(ROOT, "tests", "golden", "time_integrals.npz"), allow_pickle=False)
     b = g["boundaries_c0"]
     assert steps == len(b) - 1 == 32

@@ -2,6 +2,7 @@
 C ABI (libhyowon_pm).  Integer outputs (bucket / x-plane permutations) and the
 binary64 deposit and gather must be bit-exact; FFT-based outputs agree to
 binary64 roundoff (FP64) or to the stated binary32 tolerances (FP32)."""
+import ctypes as C
 import math
 
 import numpy as np
@@ -387,3 +388,55 @@ def test_lattice_block_schedule_bit_identical(gpu, prec, monkeypatch):
     plain = run()
     for u, v in zip(tiled, plain):
         assert np.array_equal(u, v)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+@pytest.mark.parametrize("explicit", [False, True])
+def test_potential_energy_diagnostic(gpu, oracle_mod, prec, tol, explicit):
+    """compute_forces(..., collect_potential_energy=True) returns
+    PMForceDiagnostics with 0.5 sum m phi (src/gravity/pm_solver.cpp:817-841)
+    equal to the oracle's (bit-exact against the reference's blocked sum) on
+    the oracle's potential, to binary64 FFT roundoff; forces are unchanged."""
+    import torch
+    import hyowon_amd as H
+    from hyowon_amd._lib import HyLogicError
+    N, L, n = 48, 48.0, 30000
+    pos = clumpy_particles(n, L, 21, n_clumps=30, sigma=0.5)
+    masses = np.random.default_rng(3).uniform(0.2, 2.0, n) if explicit else None
+    m = 1.3
+    ref, phi = oracle_mod.pm_force(pos[0], pos[1], pos[2], L, N, masses=masses, uniform_mass=m, return_phi=True)
+    e_ref = oracle_mod.potential_energy(phi, pos[0], pos[1], pos[2], L, masses=masses, uniform_mass=m)
+    from hyowon_amd._lib import check, lib
+    s = _solver(L, N, prec, n)
+    assert s.compute_forces(*(_dev(p, gpu) for p in pos), None if masses is None else _dev(masses, gpu), m,
+                            *(torch.zeros(n, dtype=torch.float64, device=gpu) for _ in range(3))) is None
+    with pytest.raises(HyLogicError):  # that call did not collect it
+        check(lib().hy_pm_potential_energy(s.handle, C.byref(C.c_double())))
+    acc = [torch.zeros(n, dtype=torch.float64, device=gpu) for _ in range(3)]
+    d = s.compute_forces(*(_dev(p, gpu) for p in pos), None if masses is None else _dev(masses, gpu), m, *acc,
+                         collect_potential_energy=True)
+    got = np.stack([a.cpu().numpy() for a in acc])
+    assert math.sqrt(((got - ref) ** 2).sum() / (ref ** 2).sum()) < (1e-10 if prec == "fp64" else 1e-4)
+    print(f"{prec}: potential energy {d.potential_energy_comoving:.17g} vs oracle {e_ref:.17g}")
+    assert abs(d.potential_energy_comoving - e_ref) <= tol * abs(e_ref)
+    s.close()
+
+
+def test_potential_energy_bit_exact_on_a_given_potential(gpu, oracle_mod):
+    """With the same binary64 potential on both sides (the GPU's own, copied
+    out), the diagnostic's per-particle arithmetic and blocked Neumaier order
+    reproduce the oracle bit for bit."""
+    import torch
+    import hyowon_amd as H
+    N, L, n = 32, 40.0, 70000
+    pos = uniform_particles(n, L, 31)
+    m = 0.8
+    s = _solver(L, N, "fp64", n)
+    x, y, z = (_dev(p, gpu) for p in pos)
+    src = s.density(x, y, z, None, m)
+    phi = s.potential(src).cpu().numpy()
+    acc = [torch.zeros(n, dtype=torch.float64, device=gpu) for _ in range(3)]
+    d = s.compute_forces(x, y, z, None, m, *acc, collect_potential_energy=True)
+    e_ref = oracle_mod.potential_energy(phi, pos[0], pos[1], pos[2], L, uniform_mass=m)
+    assert d.potential_energy_comoving == e_ref
+    s.close()

@@ -103,8 +103,10 @@ struct hy_pm_ctx {
     const double* keys_y = nullptr;
     const double* keys_z = nullptr;
     uint64_t keys_n = 0;
-    hy::DepRecord* rec = nullptr;
+    hy::DepRecord* rec = nullptr;       // binary32 contexts: deposit records in bucket order
     float* rec_mass = nullptr;
+    hy::DepRecord64* rec64 = nullptr;   // binary64 contexts
+    double* rec_mass64 = nullptr;
 
     // mesh
     void* R = nullptr;
@@ -307,9 +309,14 @@ void bucketize(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const
                        c->block_sums, c->grand_total, offsets);
     launch_check("k_scan");
     if (mode == 0 && c->prec == HY_PREC_FP32_MIXED) {
-        hipLaunchKernelGGL(hy::k_scatter_records, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, x, y,
-                           z, masses, 1.0 / c->mass_unit, c->geom, c->keys, c->ranks, offsets,
+        hipLaunchKernelGGL((hy::k_scatter_records<hy::DepRecord, float>), dim3(blocks_for(n, 256)), dim3(256), 0,
+                           c->stream, n, x, y, z, masses, 1.0 / c->mass_unit, c->geom, c->keys, c->ranks, offsets,
                            order, c->rec, c->rec_mass, c->err);
+        launch_check("k_scatter_records");
+    } else if (mode == 0) {
+        hipLaunchKernelGGL((hy::k_scatter_records<hy::DepRecord64, double>), dim3(blocks_for(n, 256)), dim3(256), 0,
+                           c->stream, n, x, y, z, masses, 1.0, c->geom, c->keys, c->ranks, offsets, order,
+                           c->rec64, c->rec_mass64, c->err);
         launch_check("k_scatter_records");
     } else if (carried) {
         hipLaunchKernelGGL(hy::k_scatter_checked, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, x, y,
@@ -355,6 +362,8 @@ DepositPlan plan_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double
     P.uniform_mass = uniform;
     P.offsets = c->offsets;
     P.order = c->order;
+    P.rec64 = c->rec64;
+    P.rec_mass64 = c->rec_mass64;
     P.n = n;
     P.mean_mass = mean;
     P.inv_volume = inv_volume;
@@ -637,7 +646,7 @@ void run_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, con
                  const double* masses, double uniform, double Min, bool sync, bool carried = false) {
     mark(c, 0);
     // the binary32 deposit reads records only: no particle order is written
-    uint32_t* order = c->prec == HY_PREC_FP64 ? c->order : nullptr;
+    uint32_t* order = nullptr;
     bucketize(c, n, x, y, z, 0, c->counts, c->offsets, c->nball, order, carried, masses);
     mark(c, HY_STAGE_BUCKET + 1);
     auto plan = plan_deposit(c, n, x, y, z, masses, uniform, Min);
@@ -937,8 +946,13 @@ int hy_pm_create(const hy_pm_desc* desc, hy_pm_ctx** out) {
         c->scratch = (uint64_t*)c->alloc(c->scratch_cap * 8);
         c->scratch_cursor = (unsigned long long*)c->alloc(16);
         c->err = (uint32_t*)c->alloc(16);
-        c->rec = (hy::DepRecord*)c->alloc(cap * sizeof(hy::DepRecord));
-        c->rec_mass = (float*)c->alloc(cap * sizeof(float));
+        if (c->prec == HY_PREC_FP64) {
+            c->rec64 = (hy::DepRecord64*)c->alloc(cap * sizeof(hy::DepRecord64));
+            c->rec_mass64 = (double*)c->alloc(cap * sizeof(double));
+        } else {
+            c->rec = (hy::DepRecord*)c->alloc(cap * sizeof(hy::DepRecord));
+            c->rec_mass = (float*)c->alloc(cap * sizeof(float));
+        }
         c->gx = c->alloc(cap * rb);
         c->gy = c->alloc(cap * rb);
         c->gz = c->alloc(cap * rb);
@@ -1539,11 +1553,13 @@ void deposit_values(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, 
     auto plan = plan_deposit(c, n, x, y, z, q, uniform, Min);
     plan.P.raw = 1;
     c->mass_unit = unit;
-    if (c->prec == HY_PREC_FP32_MIXED) {
-        hipLaunchKernelGGL(hy::k_scatter_values, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, q,
+    if (c->prec == HY_PREC_FP32_MIXED)
+        hipLaunchKernelGGL(hy::k_scatter_values<float>, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, q,
                            1.0 / unit, c->keys, c->ranks, c->offsets, c->rec_mass);
-        launch_check("k_scatter_values");
-    }
+    else
+        hipLaunchKernelGGL(hy::k_scatter_values<double>, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, q,
+                           1.0, c->keys, c->ranks, c->offsets, c->rec_mass64);
+    launch_check("k_scatter_values");
     launch_deposit(c, plan.P, false);
 }
 

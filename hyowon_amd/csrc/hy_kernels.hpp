@@ -202,15 +202,27 @@ struct DepRecord {
     float fx, fy, fz;
 };
 
+// Binary64 deposit record: tile-local base cell, the particle index (the
+// reference's per-cell summation order is by index) and the binary64
+// fractions.
+struct DepRecord64 {
+    uint32_t cell;
+    uint32_t index;
+    double fx, fy, fz;
+};
+
 // Scatter into bucket order, writing the particle index and its deposit
 // record.  The bucket is recomputed from the current position and must equal
 // the stored key (keys carried over from the previous step are checked here).
+// R = DepRecord (binary32 fractions, masses / mass unit) or DepRecord64
+// (binary64 fractions and particle index, masses as given).
+template <typename R, typename MT>
 __global__ void k_scatter_records(uint64_t n, const double* __restrict__ x, const double* __restrict__ y,
                                   const double* __restrict__ z, const double* __restrict__ masses,
                                   double inv_mass_unit, Geom g, const uint32_t* __restrict__ keys,
                                   const uint32_t* __restrict__ ranks, const uint32_t* __restrict__ offsets,
-                                  uint32_t* __restrict__ order, DepRecord* __restrict__ rec,
-                                  float* __restrict__ rec_mass, uint32_t* __restrict__ err) {
+                                  uint32_t* __restrict__ order, R* __restrict__ rec,
+                                  MT* __restrict__ rec_mass, uint32_t* __restrict__ err) {
     const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const uint32_t k = keys[p];
@@ -231,13 +243,22 @@ __global__ void k_scatter_records(uint64_t n, const double* __restrict__ x, cons
     if (order) order[dst] = (uint32_t)p;
     const int lx = g.T == 16 ? bx & 15 : bx % g.T, ly = g.T == 16 ? by & 15 : by % g.T,
               lz = g.T == 16 ? bz & 15 : bz % g.T;
-    DepRecord r;
+    R r;
     r.cell = (uint32_t)((lx * g.T + ly) * g.T + lz);
-    r.fx = (float)fx;
-    r.fy = (float)fy;
-    r.fz = (float)fz;
-    rec[dst] = r;
-    if (masses) rec_mass[dst] = (float)(masses[p] * inv_mass_unit);
+    if constexpr (sizeof(R) == sizeof(DepRecord64)) {
+        r.index = (uint32_t)p;
+        r.fx = fx;
+        r.fy = fy;
+        r.fz = fz;
+        rec[dst] = r;
+        if (masses) rec_mass[dst] = masses[p];
+    } else {
+        r.fx = (float)fx;
+        r.fy = (float)fy;
+        r.fz = (float)fz;
+        rec[dst] = r;
+        if (masses) rec_mass[dst] = (float)(masses[p] * inv_mass_unit);
+    }
 }
 
 // ===========================================================================
@@ -324,7 +345,7 @@ k_canonicalize(uint64_t nb, const uint32_t* __restrict__ offsets, uint32_t* __re
 // ===========================================================================
 // Deterministic CIC deposit + density source
 // ===========================================================================
-constexpr int DEP_THREADS = 256;
+constexpr int DEP_THREADS = 512;
 constexpr int DEP_MAX_T = 16;
 constexpr int DEP_EXT = (DEP_MAX_T + 1) * (DEP_MAX_T + 1) * (DEP_MAX_T + 1);  // 4913
 constexpr int DEP_NRANGES = 20;
@@ -345,6 +366,8 @@ struct DepParams {
     double uniform_mass;
     const uint32_t* offsets;  // bucket offsets (nb+1)
     const uint32_t* order;    // bucketed particle indices
+    const DepRecord64* rec64; // binary64 deposit records in bucket order
+    const double* rec_mass64; // binary64 masses in bucket order (explicit masses, or signed adjoint values)
     uint64_t n;               // particle count (bound of every order entry)
     double mean_mass;         // M / N_cells
     double inv_volume;        // 1 / dx^3
@@ -530,17 +553,49 @@ __device__ __forceinline__ void load_source_ranges(const uint32_t* __restrict__ 
     __syncthreads();
 }
 
+// Exclusive scan of a[0..count) in LDS, in place, by all threads of the
+// workgroup (each thread owns a contiguous chunk): a shuffle scan inside each
+// wavefront, then the wavefront totals (part: one word per wavefront).
+__device__ __forceinline__ void wg_exclusive_scan_lds(uint32_t* a, int count, uint32_t* part) {
+    const int nt = (int)blockDim.x;
+    const int per = (count + nt - 1) / nt;
+    const int b = (int)threadIdx.x * per;
+    const int e = b + per < count ? b + per : count;
+    uint32_t s = 0;
+    for (int i = b; i < e; ++i) s += a[i];
+    const int lane = (int)lane_id(), wave = (int)threadIdx.x / 64;
+    uint32_t incl = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) part[wave] = incl;
+    __syncthreads();
+    uint32_t run = incl - s;
+    for (int w = 0; w < wave; ++w) run += part[w];
+    for (int i = b; i < e; ++i) {
+        const uint32_t v = a[i];
+        a[i] = run;
+        run += v;
+    }
+    __syncthreads();
+}
+
 template <typename T, int TE = 0>
 __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restrict__ mesh) {
     using Cfg = DepCfg<T>;
     constexpr int CAP = Cfg::CAP;
+    constexpr int PER = CAP / DEP_THREADS;  // records per thread on the LDS path
+    static_assert(CAP % DEP_THREADS == 0, "LDS capacity must be a multiple of the workgroup size");
     __shared__ uint64_t sh_keys[CAP];
     __shared__ T sh_fx[CAP], sh_fy[CAP], sh_fz[CAP], sh_m[CAP];
     __shared__ uint32_t sh_start[DEP_EXT + 1];
     __shared__ uint32_t sh_rs[DEP_NRANGES], sh_rc[DEP_NRANGES + 1];
     __shared__ uint8_t sh_rd[DEP_NRANGES];
     __shared__ uint64_t sh_gbase;
-    __shared__ double sh_red[DEP_THREADS];
+    __shared__ double sh_red[DEP_THREADS / 64];
+    __shared__ uint32_t sh_part[DEP_THREADS / 64];
 
     const Geom& g = P.g;
     const int nT = g.nT;
@@ -569,46 +624,84 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restr
             return;
         }
         gkeys = P.scratch + sh_gbase;
+    } else {
+        for (int i = threadIdx.x; i <= ncell; i += blockDim.x) sh_start[i] = 0u;
+        __syncthreads();
     }
 
-    // Load records: CIC coordinate per particle, ext cell relative to this tile.
-    for (uint32_t v = threadIdx.x; v < M; v += blockDim.x) {
+    // Load the records (read contiguously per source range): ext cell relative
+    // to this tile, particle index, binary64 fractions.  A neighbour record
+    // sits on the last cell of its tile along each shifted axis.  On the LDS
+    // path each thread keeps its keys in registers and counts them per cell.
+    uint64_t mykey[PER];
+    uint32_t myrank[PER];
+    auto load = [&](uint32_t v) -> uint64_t {
         int r = 0;
         while (sh_rc[r + 1] <= v) ++r;
-        uint32_t pidx = P.order[sh_rs[r] + (v - sh_rc[r])];
-        bool routed = pidx < P.n;  // an order slot left unwritten by a failed scatter
-        if (!routed) pidx = 0;
+        const uint32_t slot = sh_rs[r] + (v - sh_rc[r]);
+        const DepRecord64 rc = P.rec64[slot];
         const int d = sh_rd[r];
-        int bx = 0, by = 0, bz = 0;
-        double fx = 0.0, fy = 0.0, fz = 0.0;
-        const double px = P.x[pidx], py = P.y[pidx], pz = P.z[pidx];
-        const bool ok = routed && cic_axes(px, py, pz, g, bx, by, bz, fx, fy, fz);
-        int ex = (d & 4) ? 0 : bx - tx * g.T + 1;
-        int ey = (d & 2) ? 0 : by - ty * g.T + 1;
-        int ez = (d & 1) ? 0 : bz - tz * g.T + 1;
-        if (!ok || ex < 0 || ex > g.T || ey < 0 || ey > g.T || ez < 0 || ez > g.T) {
+        const int TL = g.T;
+        const uint32_t pidx = rc.index < P.n ? rc.index : 0u;  // guards a slot a failed scatter left unwritten
+        const int cz = (int)(rc.cell % (uint32_t)TL), cy = (int)((rc.cell / (uint32_t)TL) % (uint32_t)TL),
+                  cx = (int)(rc.cell / (uint32_t)(TL * TL));
+        int ex = (d & 4) ? 0 : cx + 1;
+        int ey = (d & 2) ? 0 : cy + 1;
+        int ez = (d & 1) ? 0 : cz + 1;
+        if (rc.index >= P.n || rc.cell >= (uint32_t)(TL * TL * TL) || ((d & 4) && cx != TL - 1) ||
+            ((d & 2) && cy != TL - 1) || ((d & 1) && cz != TL - 1)) {
             atomicOr(P.err, ERR_COORDINATE);
-            ex = ex < 0 ? 0 : (ex > g.T ? g.T : ex);
-            ey = ey < 0 ? 0 : (ey > g.T ? g.T : ey);
-            ez = ez < 0 ? 0 : (ez > g.T ? g.T : ez);
+            ex = ex > TL ? TL : ex;
+            ey = ey > TL ? TL : ey;
+            ez = ez > TL ? TL : ez;
         }
         const uint32_t cell = (uint32_t)((ex * Te + ey) * Te + ez);
-        const uint64_t key = ((uint64_t)cell << 51) | ((uint64_t)pidx << 19) | (use_lds ? v : 0u);
         if (use_lds) {
-            sh_keys[v] = key;
-            sh_fx[v] = (T)fx; sh_fy[v] = (T)fy; sh_fz[v] = (T)fz;
-            if (P.masses) sh_m[v] = (T)P.masses[pidx];
-        } else {
-            gkeys[v] = key;
+            sh_fx[v] = (T)rc.fx; sh_fy[v] = (T)rc.fy; sh_fz[v] = (T)rc.fz;
+            if (P.masses) sh_m[v] = (T)P.rec_mass64[slot];
         }
+        return ((uint64_t)cell << 51) | ((uint64_t)pidx << 19) | (use_lds ? v : 0u);
+    };
+    if (use_lds) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint32_t v = threadIdx.x + (uint32_t)j * DEP_THREADS;
+            mykey[j] = 0;
+            myrank[j] = 0;
+            if (v < M) {
+                mykey[j] = load(v);
+                myrank[j] = atomicAdd(&sh_start[key_cell(mykey[j])], 1u);
+            }
+        }
+    } else {
+        for (uint32_t v = threadIdx.x; v < M; v += blockDim.x) gkeys[v] = load(v);
     }
     __threadfence_block();
     __syncthreads();
 
     double mass_acc = 0.0;
     if (use_lds) {
-        wg_sort_u64(sh_keys, M);
-        build_cell_starts(sh_keys, M, sh_start, ncell);
+        // counting sort by cell, then each cell's few records by particle index
+        wg_exclusive_scan_lds(sh_start, ncell + 1, sh_part);
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint32_t v = threadIdx.x + (uint32_t)j * DEP_THREADS;
+            if (v < M) sh_keys[sh_start[key_cell(mykey[j])] + myrank[j]] = mykey[j];
+        }
+        __syncthreads();
+        for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+            const uint32_t b = sh_start[c], e = sh_start[c + 1];
+            for (uint32_t i = b + 1; i < e; ++i) {
+                const uint64_t k = sh_keys[i];
+                uint32_t j = i;
+                while (j > b && sh_keys[j - 1] > k) {
+                    sh_keys[j] = sh_keys[j - 1];
+                    --j;
+                }
+                sh_keys[j] = k;
+            }
+        }
+        __syncthreads();
         deposit_cells<T, true, TE>(P, mesh, tx, ty, tz, sh_keys, M, sh_start, sh_fx, sh_fy, sh_fz,
                                sh_m, mass_acc);
     } else {
@@ -618,14 +711,16 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restr
                                 sh_m, mass_acc);
     }
 
-    // Deterministic per-tile mass: fixed-order tree over threads.
-    sh_red[threadIdx.x] = mass_acc;
+    // Deterministic per-tile mass: a fixed shuffle tree per wavefront, then
+    // the wavefront sums in order.
+    for (int off = 32; off > 0; off >>= 1) mass_acc += __shfl_down(mass_acc, off);
+    if (lane_id() == 0) sh_red[threadIdx.x / 64] = mass_acc;
     __syncthreads();
-    for (int s = DEP_THREADS / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh_red[threadIdx.x] += sh_red[threadIdx.x + s];
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int w = 0; w < DEP_THREADS / 64; ++w) m += sh_red[w];
+        P.tile_mass[gtile] = m;
     }
-    if (threadIdx.x == 0) P.tile_mass[gtile] = sh_red[0];
 }
 
 template __global__ void k_deposit<double, 0>(DepParams, double*);
@@ -1446,14 +1541,16 @@ __global__ void k_absmax(uint64_t n, const double* __restrict__ q, double* __res
 
 // Per-particle values q, scaled by inv_unit, into bucket order for the
 // binary32 record deposit (keys/ranks/offsets of the same positions).
+template <typename MT>
 __global__ void k_scatter_values(uint64_t n, const double* __restrict__ q, double inv_unit,
                                  const uint32_t* __restrict__ keys, const uint32_t* __restrict__ ranks,
-                                 const uint32_t* __restrict__ offsets, float* __restrict__ out) {
+                                 const uint32_t* __restrict__ offsets, MT* __restrict__ out) {
     const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const uint32_t k = keys[p];
     if (k == 0xffffffffu) return;
-    out[offsets[k] + ranks[p]] = (float)(q[p] * inv_unit);
+    if constexpr (sizeof(MT) == sizeof(double)) out[offsets[k] + ranks[p]] = q[p];
+    else out[offsets[k] + ranks[p]] = (MT)(q[p] * inv_unit);
 }
 
 // out = sum_a D_a A_a (periodic centred differences), one box.

@@ -28,6 +28,7 @@ EXPORTS = [
     "hy_ic_gaussian_modes", "hy_ic_last_error", "hy_pm_power_spectrum", "hy_pm_force_meshes",
     "hy_pm_interpolate", "hy_ic_lpt", "hy_xplane_order_static", "hy_static_last_error",
     "hy_growth", "hy_linear_power", "hy_ic_mode_variance", "hy_cubic_spline", "hy_pm_potential_energy",
+    "hy_snapshot_write", "hy_snapshot_read_header", "hy_snapshot_read", "hy_snapshot_last_error",
 ]
 
 HY_FFT_PASSES = 5
@@ -113,6 +114,26 @@ class hy_cosmology(C.Structure):
                 ("omega_lambda", C.c_double), ("sigma8", C.c_double), ("n_s", C.c_double)]
 
 
+class hy_snapshot_meta(C.Structure):
+    _fields_ = [("box_size", C.c_double), ("omega_m", C.c_double), ("omega_lambda", C.c_double),
+                ("h", C.c_double), ("omega_b", C.c_double), ("sigma8", C.c_double), ("n_s", C.c_double),
+                ("seed", C.c_uint64), ("particles_per_dim", C.c_uint32), ("pm_mesh_per_dim", C.c_uint32),
+                ("ic_mesh_per_dim", C.c_uint32), ("lpt_order", C.c_int32), ("softening", C.c_double),
+                ("z_start", C.c_double), ("z_final", C.c_double), ("delta_ln_a", C.c_double),
+                ("power_spectrum_redshift", C.c_double), ("deconvolve_cic", C.c_int),
+                ("ic_amplitude_mode", C.c_char_p), ("ic_phase_pairing", C.c_char_p),
+                ("power_spectrum_fidelity", C.c_char_p)]
+
+
+class hy_snapshot_header(C.Structure):
+    _fields_ = [("a", C.c_double), ("redshift", C.c_double), ("box_size", C.c_double), ("omega_m", C.c_double),
+                ("omega_lambda", C.c_double), ("h", C.c_double), ("uniform_mass", C.c_double),
+                ("particles", C.c_uint64), ("explicit_masses", C.c_int)]
+
+
+HY_SNAPSHOT_REPLACE = 0x1
+
+
 class hy_pk_options(C.Structure):
     _fields_ = [("num_bins", C.c_int), ("subtract_shot_noise", C.c_int), ("interlaced", C.c_int),
                 ("max_k_fraction_nyquist", C.c_double)]
@@ -193,6 +214,12 @@ def lib():
     fn("hy_linear_power", i, pc, C.c_char_p, d, d, u64, pd, pd)
     fn("hy_ic_mode_variance", i, pc, C.c_char_p, d, d, u64, d, u64, pd)
     fn("hy_cubic_spline", i, u64, pd, pd, u64, pd, pd)
+    fn("hy_snapshot_write", i, C.c_char_p, C.POINTER(hy_snapshot_meta), d, u64, pd, pd, pd, pd, pd, pd,
+       C.POINTER(C.c_uint64), pd, d, u32)
+    fn("hy_snapshot_read_header", i, C.c_char_p, C.POINTER(hy_snapshot_header))
+    fn("hy_snapshot_read", i, C.c_char_p, C.POINTER(hy_snapshot_header), u64, pd, pd, pd, pd, pd, pd,
+       C.POINTER(C.c_uint64), pd)
+    fn("hy_snapshot_last_error", C.c_char_p)
     _lib = L
     return L
 

@@ -68,7 +68,7 @@ def build(verbose: bool = False) -> str:
             raise RuntimeError(f"hipcc failed:\n{msg}")
         tmp_lib = LIB + ".tmp"
         link = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp_lib, *objs,
-                "-lquadmath"]
+                "-lquadmath", "-ldl"]
         if verbose:
             print(" ".join(link), flush=True)
         subprocess.run(link, check=True)

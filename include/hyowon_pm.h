@@ -410,6 +410,50 @@ int hy_ic_mode_variance(const hy_cosmology* c, const char* table, double table_r
 /* math::CubicSpline (include/cosmo_nbody/math/cubic_spline.hpp): natural spline
  * through (x, y) (n >= 3, x increasing), evaluated at q[0..m). */
 int hy_cubic_spline(uint64_t n, const double* x, const double* y, uint64_t m, const double* q, double* out);
+/* Native HYOWON snapshot files (HDF5, the reference's single-file schema:
+ * include/cosmo_nbody/io/output_schema.hpp:22-140).  Host memory, binary64.
+ * The HDF5 C library is loaded at run time (libhdf5.so, or the path in the
+ * environment variable HY_HDF5_LIBRARY); without it these calls return
+ * HY_ERUNTIME.  Errors: hy_snapshot_last_error(). */
+typedef struct {
+    double box_size;               /* Header.BoxSize (Mpc/h) */
+    double omega_m, omega_lambda, h, omega_b, sigma8, n_s;
+    uint64_t seed;                 /* Parameters.Seed */
+    uint32_t particles_per_dim, pm_mesh_per_dim, ic_mesh_per_dim;
+    int32_t lpt_order;
+    double softening;              /* Parameters.SofteningComoving (0 for pure PM) */
+    double z_start, z_final, delta_ln_a, power_spectrum_redshift;
+    int deconvolve_cic;
+    const char* ic_amplitude_mode;       /* "gaussian" when NULL */
+    const char* ic_phase_pairing;        /* "independent" when NULL */
+    const char* power_spectrum_fidelity; /* "precision_boltzmann" when NULL */
+} hy_snapshot_meta;
+typedef struct {
+    double a, redshift, box_size, omega_m, omega_lambda, h;
+    double uniform_mass;           /* MassTable[1]; 0 when the file holds explicit masses */
+    uint64_t particles;            /* NumPart_ThisFile[1] */
+    int explicit_masses;           /* PartType1/Masses present */
+} hy_snapshot_header;
+#define HY_SNAPSHOT_REPLACE 0x1u   /* hy_snapshot_write: replace an existing file (else it must not exist) */
+/* SnapshotIO::write_snapshot_to_path (include/cosmo_nbody/io/snapshot_io.hpp:97-110;
+ * src/io/snapshot_io_write_impl.inc): /Header, /Parameters, /Config (identity,
+ * physics text, run metadata, semantic bindings), /PartType1 Coordinates (n x 3),
+ * Velocities = p / a (n x 3, km/s), ParticleIDs (ids or 0..n-1), Masses when
+ * masses != NULL (else MassTable[1] = uniform_mass).  Positions must lie in
+ * [0, L). */
+int hy_snapshot_write(const char* path, const hy_snapshot_meta* meta, double a, uint64_t n, const double* x,
+                      const double* y, const double* z, const double* px, const double* py, const double* pz,
+                      const uint64_t* ids, const double* masses, double uniform_mass, uint32_t flags);
+/* The header checks of SnapshotIO::read_snapshot (src/io/snapshot_io_read_impl.inc:33-190):
+ * Redshift consistent with Time, one file, PartType1 only, native units, the
+ * Config semantic bindings, non-empty provenance. */
+int hy_snapshot_read_header(const char* path, hy_snapshot_header* out);
+/* Reads the particles (capacity >= header.particles) with p = a v; expect
+ * (optional) is the admitted box and cosmology, compared as the reference does.
+ * ids and masses may be NULL (masses is required when the file holds them). */
+int hy_snapshot_read(const char* path, const hy_snapshot_header* expect, uint64_t capacity, double* x, double* y,
+                     double* z, double* px, double* py, double* pz, uint64_t* ids, double* masses);
+const char* hy_snapshot_last_error(void);
 /* G in (Mpc/h)(km/s)^2/(1e10 Msun/h) and rho_crit0 (include/cosmo_nbody/cosmology/units.hpp:26-30). */
 double hy_gravitational_constant(void);
 double hy_rho_crit0(void);

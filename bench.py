@@ -566,13 +566,14 @@ def run_adjoint(args):
     """Config-4 workload: 128^3 particles from a 128^3 white-noise field (seed
     7), 256^3 mesh, L = 256 Mpc/h, 16 KDK steps z = 19 -> 0; loss = mean
     squared difference between the final CIC density contrast and that of the
-    seed-8 field run through the same forward; hand-written adjoint with
+    seed-8 field run through the binary32 forward; hand-written adjoint with
     checkpoints every 4 refreshes.  Reports forward / backward times and the
-    gradient checks: central differences (eps = 1e-3 as specified, and 1e-4)
-    on 64 random noise components in binary64, and FP32 vs binary64 gradients.
-    The loss is piecewise smooth (CIC weights kink at cell faces): the
-    central-difference deviation from the adjoint shrinks with eps as fewer
-    particles cross a face inside the differencing interval."""
+    gradient checks: central differences (eps = 1e-3 as specified, 1e-4 and
+    1e-5) on 64 random noise components in binary64, and the binary32
+    gradient against the binary64 reference gradient of the same loss.  The
+    loss is piecewise smooth (CIC weights kink where a particle crosses a mesh
+    plane): the central-difference deviation from the adjoint shrinks with eps
+    as fewer particles cross a plane inside the differencing interval."""
     import numpy as np
     import torch
     import hyowon_amd as H
@@ -586,15 +587,15 @@ def run_adjoint(args):
     noise = ic.white_noise(n_side, 7, dev).to(torch.float64)
     out = {}
     grads = {}
-    for prec in ("fp32", "fp64"):
+    m = ic.particle_mass(n_side, L)
+    s = H.PMSolver(L, N, precision="fp32", max_particles=n_side ** 3)
+    with torch.no_grad():  # target: the seed-8 field through the binary32 forward
+        pos, mom = ic.lpt_from_noise(ic.white_noise(n_side, 8, dev).to(torch.float64), L, a0)
+        st = pm_run((pos[0], pos[1], pos[2], *(t.to(torch.float32) for t in mom)), s, m, K1, D, K2, 4)
+        target = cic_density(st[0], st[1], st[2], s, m).to(torch.float64)
+    s.close()
+    for prec in ("fp32", "fp64"):  # the binary32 gradient and the binary64 reference gradient of one loss
         s = H.PMSolver(L, N, precision=prec, max_particles=n_side ** 3)
-        pdt = torch.float64 if prec == "fp64" else torch.float32
-        m = ic.particle_mass(n_side, L)
-        with torch.no_grad():  # target: the seed-8 field through the same forward
-            pos, mom = ic.lpt_from_noise(ic.white_noise(n_side, 8, dev).to(torch.float64), L, a0)
-            st = pm_run((pos[0], pos[1], pos[2], mom[0].to(pdt), mom[1].to(pdt), mom[2].to(pdt)), s, m,
-                        K1, D, K2, 4)
-            target = cic_density(st[0], st[1], st[2], s, m).to(torch.float64)
         warm = noise.clone().requires_grad_(True)  # first launches load code objects
         density_loss(warm, target, s, L, a0, K1, D, K2, checkpoint_every=4, precision=prec).backward()
         nz = noise.clone().requires_grad_(True)

@@ -152,3 +152,54 @@ def test_fp32_adjoint_agrees_with_fp64_reference(gpu):
     rel = np.linalg.norm(g32 - g64) / np.linalg.norm(g64)
     print(f"FP32 vs FP64 gradient: relative L2 difference {rel:.3e}")
     assert rel < 1e-3, rel
+
+
+def test_adjoint_gradient_matches_oracle_finite_differences(gpu, oracle_mod):
+    """An FP64 reference gradient independent of the adjoint code: central
+    differences of the oracle's binary64 forward (its KDK step loop and its
+    CIC deposit, src/time/leapfrog_integrator.cpp:143-279 and
+    src/mesh/mass_assignment_deposit.inc:15-334 restated) on the reduced run,
+    against the GPU adjoint gradient.  The oracle steps one boundary interval
+    per call (the same arithmetic as one call over all of them), so the CIC
+    base index floor(x / dx) of every particle is seen at every step: per
+    component the largest eps of a ladder whose +-eps runs keep every base
+    index (no kink of the piecewise-linear CIC weights inside the interval)
+    is used."""
+    from hyowon_amd import ic
+    import hyowon_amd as H
+    _, grad, (s, noise, target, L, a0, K1, D, K2) = _loss_grad("fp64", gpu)
+    n_side, N = noise.shape[0], s.mesh_size
+    steps = len(K1)
+    b = H.TimeStepper(a0, 1.0, math.log(1.0 / a0) / steps, [1.0]).boundaries
+    m = ic.particle_mass(n_side, L)
+    tgt = target.cpu().numpy()
+    mean = m * n_side ** 3 / N ** 3
+
+    def oracle_run(v):
+        pos, mom = ic.lpt_from_noise(v, L, a0)
+        st = [t.numpy() for t in (*pos, *mom)]
+        corners = [np.floor(np.stack(st[:3]) * (N / L))]
+        for k in range(len(b) - 1):
+            st, _ = oracle_mod.run(*st, L, N, [b[k], b[k + 1]], m)
+            corners.append(np.floor(np.stack(st[:3]) * (N / L)))
+        mc = oracle_mod.deposit(st[0], st[1], st[2], L, N, uniform_mass=m)
+        return float(((mc / mean - 1.0 - tgt) ** 2).mean()), np.stack(corners)
+
+    flat = noise.cpu().reshape(-1)
+    idx = np.random.default_rng(13).choice(flat.numel(), size=12, replace=False)
+    errs = []
+    for i in idx:
+        for eps in (1e-4, 3e-5, 1e-5, 3e-6):
+            lp, lm = flat.clone(), flat.clone()
+            lp[i] += eps
+            lm[i] -= eps
+            (Lp, cp), (Lm, cm) = oracle_run(lp.view_as(noise)), oracle_run(lm.view_as(noise))
+            if np.array_equal(cp, cm):
+                break
+        else:
+            pytest.fail(f"component {i}: every eps of the ladder crosses a CIC kink")
+        fd = (Lp - Lm) / (2 * eps)
+        g = grad.reshape(-1)[i]
+        errs.append(abs(fd - g) / max(abs(fd), 1e-12))
+        print(f"component {i}: GPU adjoint {g:.8e}  oracle central difference {fd:.8e} (eps {eps:g})")
+    assert np.median(errs) < 1e-6 and max(errs) < 1e-5, errs

@@ -250,3 +250,223 @@ def test_ensemble192_graph_steps_match_single_boxes_and_oracle(gpu, oracle_mod):
         for got_t, ref_t in zip((st.x, st.y, st.z, st.px, st.py, st.pz), (a.x, a.y, a.z, a.px, a.py, a.pz)):
             assert torch.equal(got_t[j * n:(j + 1) * n], ref_t), (ci, j)
     run.close()
+
+
+# ---------------------------------------------------------------------------
+# Differentiable run: 128^3 particles, 256^3 mesh, 16 steps, checkpoints every 4
+# ---------------------------------------------------------------------------
+NS4, N4, L4, STEPS4 = 128, 256, 256.0, 16
+A4 = 1.0 / 20.0
+LADDER = (1e-3, 1e-4, 3e-5, 1e-5, 3e-6, 1e-6, 3e-7)
+
+
+@pytest.fixture(scope="module")
+def adjoint_run(gpu):
+    """Seed-7 noise, the target (the seed-8 field through the binary32
+    forward), and the binary32 adjoint gradient with its binary64 reference
+    gradient of the same loss (checkpoints every 4 refreshes)."""
+    import torch
+    import hyowon_amd as H
+    from hyowon_amd import ic
+    from hyowon_amd.adjoint import cic_density, density_loss, pm_run
+    from hyowon_amd.ensemble import schedule
+    K1, D, K2 = schedule(A4, 1.0, STEPS4)
+    noise = ic.white_noise(NS4, 7, gpu).to(torch.float64)
+    m = ic.particle_mass(NS4, L4)
+    out = dict(K1=K1, D=D, K2=K2, noise=noise, m=m)
+    solvers = {prec: H.PMSolver(L4, N4, precision=prec, max_particles=NS4 ** 3) for prec in ("fp32", "fp64")}
+    with torch.no_grad():
+        pos, mom = ic.lpt_from_noise(ic.white_noise(NS4, 8, gpu).to(torch.float64), L4, A4)
+        st = pm_run((pos[0], pos[1], pos[2], *(t.to(torch.float32) for t in mom)), solvers["fp32"], m,
+                    K1, D, K2, 4)
+        out["target"] = cic_density(st[0], st[1], st[2], solvers["fp32"], m).to(torch.float64)
+    for prec, s in solvers.items():
+        nz = noise.clone().requires_grad_(True)
+        density_loss(nz, out["target"], s, L4, A4, K1, D, K2, checkpoint_every=4, precision=prec).backward()
+        out[prec] = dict(solver=s, grad=nz.grad.detach().reshape(-1).clone())
+    out["solver2"] = H.PMSolver(L4, N4, precision="fp64", max_particles=NS4 ** 3)
+    out["hybrid"] = _adjoint_along(out, solvers["fp64"], solvers["fp32"])
+    yield out
+    for s in solvers.values():
+        s.close()
+    out["solver2"].close()
+
+
+def _adjoint_along(r, traj_solver, adj_solver):
+    """dL/dnoise from the adjoint steps of `adj_solver` applied along the
+    trajectory of `traj_solver` (every refresh input kept): with the binary64
+    trajectory and the binary32 adjoint it isolates the adjoint's own
+    binary32 arithmetic from the binary32 forward's trajectory."""
+    import torch
+    from hyowon_amd import ic
+    from hyowon_amd._lib import check, lib
+    from hyowon_amd.adjoint import _calls, _clone, _fused, cic_density
+    from hyowon_amd.pm import ParticleState, _ptr
+    nz = r["noise"].clone().requires_grad_(True)
+    pos, mom = ic.lpt_from_noise(nz, L4, A4)
+    pdt = traj_solver._torch_mesh_dtype()  # the momentum type density_loss gives that precision
+    st = ParticleState(*(t.detach().contiguous() for t in pos), *(t.detach().to(pdt).contiguous() for t in mom))
+    tmp = st.empty_like()
+    calls = _calls(r["K1"], r["D"], r["K2"])
+    states = []
+    with torch.no_grad():
+        for call in calls:
+            states.append(_clone(st))
+            _fused(traj_solver, st, tmp, r["m"], call)
+            st, tmp = tmp, st
+        n = st.x.numel()
+        delta = cic_density(st.x, st.y, st.z, traj_solver, r["m"]).to(torch.float64)
+        u = (2.0 * (delta - r["target"]) / delta.numel() / (r["m"] * n / N4 ** 3))
+        u = u.to(adj_solver._torch_mesh_dtype()).contiguous()
+        lam = [torch.zeros(n, dtype=torch.float64, device=st.x.device) for _ in range(6)]
+        check(lib().hy_pm_density_vjp(adj_solver.handle, n, _ptr(st.x), _ptr(st.y), _ptr(st.z), float(r["m"]),
+                                      _ptr(u), 1.0, *(_ptr(t) for t in lam[:3])))
+        for c in reversed(range(len(calls))):
+            xin = states[c]
+            kp, kn, dn, flags = calls[c]
+            check(lib().hy_pm_kdk_vjp(adj_solver.handle, n, _ptr(xin.x), _ptr(xin.y), _ptr(xin.z), float(r["m"]),
+                                      *(_ptr(t) for t in lam), float(kp), float(kn), float(dn), flags))
+    return torch.autograd.grad(list(pos) + list(mom), nz, grad_outputs=lam)[0].reshape(-1)
+
+
+def _corners(st, inv_dx):
+    """The CIC base index floor(x / dx) per axis (mesh points at i dx,
+    include/cosmo_nbody/mesh/cic_coordinate.hpp:95-121): the branch of the
+    piecewise-linear CIC weights a particle sits on."""
+    import torch
+    return torch.stack([torch.floor(t * inv_dx).to(torch.int32) for t in (st.x, st.y, st.z)])
+
+
+def _pair_losses(r, vp, vm):
+    """Binary64 losses of the noise fields vp and vm, run side by side, and the
+    number of (particle, refresh) pairs whose CIC corner differs between the two
+    runs: a kink of the loss crossed between them."""
+    import torch
+    from hyowon_amd import ic
+    from hyowon_amd.adjoint import _calls, _fused, cic_density
+    from hyowon_amd.pm import ParticleState
+    solvers = (r["fp64"]["solver"], r["solver2"])
+    sts = []
+    for v in (vp, vm):
+        pos, mom = ic.lpt_from_noise(v.view_as(r["noise"]), L4, A4)
+        sts.append(ParticleState(*(t.contiguous() for t in (*pos, *mom))))
+    tmps = [st.empty_like() for st in sts]
+    inv_dx = N4 / L4
+    crossings = 0
+    for call in _calls(r["K1"], r["D"], r["K2"]):  # every refresh's input positions (the last = final)
+        crossings += int((_corners(sts[0], inv_dx) != _corners(sts[1], inv_dx)).any(0).sum())
+        for j in range(2):
+            _fused(solvers[j], sts[j], tmps[j], r["m"], call)
+            sts[j], tmps[j] = tmps[j], sts[j]
+    losses = []
+    for st in sts:
+        d = cic_density(st.x, st.y, st.z, solvers[0], r["m"]).to(torch.float64)
+        losses.append(float(((d - r["target"]) ** 2).mean()))
+    return losses[0], losses[1], crossings
+
+
+def _central(r, i, eps):
+    flat = r["noise"].reshape(-1)
+    vp, vm = flat.clone(), flat.clone()
+    vp[i] += eps
+    vm[i] -= eps
+    lp, lm, crossings = _pair_losses(r, vp, vm)
+    return (lp - lm) / (2 * eps), crossings
+
+
+def test_adjoint128_mesh256_matches_kink_free_central_differences(gpu, adjoint_run):
+    """64 random noise components (seed 2024).  The loss is continuous and
+    piecewise smooth in the noise: every particle's CIC weights kink where it
+    crosses a mesh plane.  At eps = 1e-3 each +-eps interval holds of order
+    a hundred such crossings, so the central difference there is a secant
+    across kinks; per component the test takes the largest eps <= 1e-4 of a
+    ladder whose interval crosses none (base indices equal at every refresh
+    of both runs) and requires the binary64 and binary32 adjoints to match
+    that central difference: the binary64 adjoint to 1e-3 per component.
+    A component whose every interval crosses a kink (a particle within ~1e-7
+    cells of a mesh plane at some step) is reported and left out; at most 4
+    of the 64 may be.  The binary32 gradient is split in two: the adjoint's
+    binary32 arithmetic applied along the binary64 trajectory must give the
+    binary64 reference gradient to 1e-5 (L2 over all components); the full
+    binary32 gradient also differentiates the binary32 trajectory, which lies
+    ~5e-6 cells (RMS) from the binary64 one and puts some particles on other
+    CIC branches, and is held to 2e-3 against the reference gradient over all
+    components and to 3e-3 against the central differences."""
+    import torch
+    r = adjoint_run
+    idx = np.random.default_rng(2024).choice(r["noise"].numel(), size=64, replace=False)
+    g64 = r["fp64"]["grad"][torch.as_tensor(idx, device=gpu)].cpu().numpy()
+    g32 = r["fp32"]["grad"][torch.as_tensor(idx, device=gpu)].cpu().numpy()
+    fd = np.zeros(64)
+    fd3, cross3, used = np.zeros(64), np.zeros(64, int), np.zeros(64)
+    with torch.no_grad():
+        for k, i in enumerate(idx):
+            fd3[k], cross3[k] = _central(r, int(i), 1e-3)
+            for eps in LADDER[1:]:
+                fd[k], c = _central(r, int(i), eps)
+                if c == 0:
+                    used[k] = eps
+                    break
+            else:
+                print(f"component {i}: every interval down to eps = {LADDER[-1]:g} crosses a kink ({c} at the last)")
+            if (k + 1) % 16 == 0:
+                print(f"central differences: {k + 1}/64 components", flush=True)
+    ok = used > 0
+    assert ok.sum() >= 60, f"only {ok.sum()} of 64 components have a kink-free interval"
+    g64k, g32k, fdk = g64[ok], g32[ok], fd[ok]
+    rel64 = np.abs(g64k - fdk) / np.abs(fdk)
+    rel32 = np.abs(g32k - fdk) / np.abs(fdk)
+    l2_64 = np.linalg.norm(g64k - fdk) / np.linalg.norm(fdk)
+    l2_32 = np.linalg.norm(g32k - fdk) / np.linalg.norm(fdk)
+    l2_3 = np.linalg.norm(g64 - fd3) / np.linalg.norm(fd3)
+    eps_used = {e: int((used == e).sum()) for e in LADDER[1:] if (used == e).any()}
+    print(f"eps = 1e-3: {cross3.min()}..{cross3.max()} kink crossings per component, binary64 adjoint vs "
+          f"central difference L2 {l2_3:.3e}")
+    print(f"kink-free eps per component {eps_used} ({ok.sum()} of 64): binary64 adjoint L2 {l2_64:.3e} "
+          f"(max {rel64.max():.3e}), binary32 adjoint L2 {l2_32:.3e} (max {rel32.max():.3e})")
+    ga, gb, gh = r["fp32"]["grad"], r["fp64"]["grad"], r["hybrid"]
+    l2_all = float(torch.linalg.norm(ga - gb) / torch.linalg.norm(gb))
+    l2_hyb = float(torch.linalg.norm(gh - gb) / torch.linalg.norm(gb))
+    print(f"over all {gb.numel()} components: binary32 adjoint along the binary64 trajectory vs binary64 "
+          f"L2 {l2_hyb:.3e}; binary32 gradient vs binary64 L2 {l2_all:.3e}")
+    assert (cross3 > 0).all()
+    assert rel64.max() <= 1e-3 and l2_64 <= 1e-4
+    assert l2_hyb <= 1e-5
+    assert l2_all <= 2e-3 and l2_32 <= 3e-3
+
+
+def test_adjoint128_mesh256_eps1e3_deviation_is_the_kinks(gpu, adjoint_run):
+    """At eps = 1e-3 the central difference is the mean of dL/dnoise_i over
+    [-eps, eps] (the loss is continuous), so it must equal the adjoint averaged
+    over that interval (midpoint rule on 16 nodes), not the adjoint at 0:
+    averaging the adjoint over the path removes most of the deviation the
+    kinks make (over 4 components, on average at least half of it)."""
+    import torch
+    from hyowon_amd.adjoint import density_loss
+    r = adjoint_run
+    s = r["fp64"]["solver"]
+    idx = np.random.default_rng(2024).choice(r["noise"].numel(), size=64, replace=False)[:4]
+    eps, M = 1e-3, 16
+    flat = r["noise"].reshape(-1)
+    devs = []
+    for i in idx:
+        i = int(i)
+        with torch.no_grad():
+            fd, crossings = _central(r, i, eps)
+        g0 = float(r["fp64"]["grad"][i])
+        acc = 0.0
+        for k in range(M):
+            v = flat.clone()
+            v[i] += -eps + (k + 0.5) * (2 * eps / M)
+            nz = v.view_as(r["noise"]).clone().requires_grad_(True)
+            density_loss(nz, r["target"], s, L4, A4, r["K1"], r["D"], r["K2"], checkpoint_every=4,
+                         precision="fp64").backward()
+            acc += float(nz.grad.reshape(-1)[i])
+        mean = acc / M
+        dev0, devm = abs(fd - g0) / abs(fd), abs(fd - mean) / abs(fd)
+        print(f"component {i}: {crossings} crossings, central difference vs adjoint at 0 {dev0:.3e}, "
+              f"vs adjoint averaged over the interval {devm:.3e}")
+        assert devm <= 1e-3
+        devs.append((dev0, devm))
+    d0, dm = np.mean(devs, axis=0)
+    assert dm <= 0.5 * d0, (d0, dm)

@@ -20,6 +20,7 @@
 #include <limits>
 #include <numbers>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -103,8 +104,23 @@ struct hy_pm_ctx {
     const double* keys_y = nullptr;
     const double* keys_z = nullptr;
     uint64_t keys_n = 0;
-    hy::DepRecord* rec = nullptr;       // binary32 contexts: deposit records in bucket order
+    hy::DepRecord* rec = nullptr;       // binary32 contexts: deposit records by bucket
     float* rec_mass = nullptr;
+    // Binary32 uniform-mass HY_ASYNC steps hand the next step the deposit
+    // records of their drifted positions in a capacity layout (see
+    // k_cap_reduce); rec_ready marks them valid for exactly those positions.
+    bool rec_layout = false;            // the buffers below exist (binary32 contexts)
+    bool rec_ready = false;
+    const double* rec_x = nullptr;
+    const double* rec_y = nullptr;
+    const double* rec_z = nullptr;
+    uint64_t rec_n = 0;
+    uint32_t* cap_cur = nullptr;        // nball+1: layout the carried records were written in
+    uint32_t* cap_next = nullptr;       // nball+1: layout the next step's records go to
+    uint32_t* ovf_start = nullptr;      // nball+1: overflow prefix of the carried layout
+    uint32_t* ovf_count = nullptr;      // overflow records the last step appended
+    hy::DepRecord* ovf_rec = nullptr;   // overflow records (aliases the binary64 fallback scratch)
+    uint32_t* cap_block_sums = nullptr; // 3 per scan block
     hy::DepRecord64* rec64 = nullptr;   // binary64 contexts
     double* rec_mass64 = nullptr;
 
@@ -289,25 +305,51 @@ double input_mass(hy_pm_ctx* c, uint64_t n, const double* masses, double uniform
     return M;
 }
 
+// Three-channel scan of the deposit bucket counts (k_cap_reduce): dense
+// offsets (or null), the overflow prefix of the carried layout cur (or null),
+// the next step's capacity layout capn (or null), and the grand total.
+void cap_scan(hy_pm_ctx* c, const uint32_t* cur, uint32_t* dense, uint32_t* capn) {
+    hy::CapScan s{};
+    s.counts = c->counts;
+    s.nb = c->nball;
+    s.cur = cur;
+    s.dense = dense;
+    s.ovf_start = c->ovf_start;
+    s.capn = capn;
+    s.block_sums = c->cap_block_sums;
+    s.grand_total = c->grand_total;
+    const uint64_t nblk = blocks_for(c->nball, hy::SCAN_TILE);
+    hipLaunchKernelGGL(hy::k_cap_reduce, dim3(nblk), dim3(hy::SCAN_THREADS), 0, c->stream, s);
+    hipLaunchKernelGGL(hy::k_cap_blocks, dim3(1), dim3(hy::SCAN_THREADS), 0, c->stream, s, (uint32_t)nblk);
+    hipLaunchKernelGGL(hy::k_cap_apply, dim3(nblk), dim3(hy::SCAN_THREADS), 0, c->stream, s);
+    launch_check("k_cap_scan");
+}
+
 // Bucket count/rank + scan + scatter.  mode 0: deposit buckets, 1: x-planes.
 // carried: keys, ranks and counts were produced by the previous fused step.
+// capn (binary32 deposit buckets): also lay out the capacities of the next
+// step's carried records from these counts.
 void bucketize(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z, int mode,
                uint32_t* counts, uint32_t* offsets, uint64_t nb, uint32_t* order, bool carried = false,
-               const double* masses = nullptr) {
+               const double* masses = nullptr, uint32_t* capn = nullptr) {
     if (!carried) {
         fill_u32(c, counts, nb);
         hipLaunchKernelGGL(hy::k_bucket_key, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, x, y, z,
                            c->geom, mode, c->keys, c->ranks, counts, c->err);
         launch_check("k_bucket_key");
     }
-    const uint64_t nblk = blocks_for(nb, hy::SCAN_TILE);
-    hipLaunchKernelGGL(hy::k_scan_reduce, dim3(nblk), dim3(hy::SCAN_THREADS), 0, c->stream, counts, nb,
-                       c->block_sums);
-    hipLaunchKernelGGL(hy::k_scan_blocks, dim3(1), dim3(hy::SCAN_THREADS), 0, c->stream, c->block_sums,
-                       (uint32_t)nblk, c->grand_total);
-    hipLaunchKernelGGL(hy::k_scan_apply, dim3(nblk), dim3(hy::SCAN_THREADS), 0, c->stream, counts, nb,
-                       c->block_sums, c->grand_total, offsets);
-    launch_check("k_scan");
+    if (mode == 0 && c->prec == HY_PREC_FP32_MIXED && c->rec_layout) {
+        cap_scan(c, nullptr, offsets, capn);
+    } else {
+        const uint64_t nblk = blocks_for(nb, hy::SCAN_TILE);
+        hipLaunchKernelGGL(hy::k_scan_reduce, dim3(nblk), dim3(hy::SCAN_THREADS), 0, c->stream, counts, nb,
+                           c->block_sums);
+        hipLaunchKernelGGL(hy::k_scan_blocks, dim3(1), dim3(hy::SCAN_THREADS), 0, c->stream, c->block_sums,
+                           (uint32_t)nblk, c->grand_total);
+        hipLaunchKernelGGL(hy::k_scan_apply, dim3(nblk), dim3(hy::SCAN_THREADS), 0, c->stream, counts, nb,
+                           c->block_sums, c->grand_total, offsets);
+        launch_check("k_scan");
+    }
     if (mode == 0 && c->prec == HY_PREC_FP32_MIXED) {
         hipLaunchKernelGGL((hy::k_scatter_records<hy::DepRecord, float>), dim3(blocks_for(n, 256)), dim3(256), 0,
                            c->stream, n, x, y, z, masses, 1.0 / c->mass_unit, c->geom, c->keys, c->ranks, offsets,
@@ -407,14 +449,18 @@ long double mass_tol_factor(const hy_pm_ctx* c, uint64_t n) {
 
 // status_check: also evaluate the routing identity and the mass bound on the
 // device (error bits reported by the synchronous check or hy_pm_check).
-void launch_deposit(hy_pm_ctx* c, const hy::DepParams& P, bool status_check = true) {
+// Binary32: the records lie in the layout start (dense offsets unless
+// carried in a capacity layout, whose overflow prefix is ovf).
+void launch_deposit(hy_pm_ctx* c, const hy::DepParams& P, bool status_check = true,
+                    const uint32_t* start = nullptr, const uint32_t* ovf = nullptr) {
     if (c->prec == HY_PREC_FP64)
         hipLaunchKernelGGL((c->geom.T == 16 ? hy::k_deposit<double, 16> : hy::k_deposit<double, 0>),
                            dim3(c->ntiles * c->nbox), dim3(hy::DEP_THREADS), 0, c->stream, P,
                            static_cast<double*>(c->R));
     else {
         hy::DepFxParams Q{P, c->mass_unit, std::ldexp(c->mass_unit, -32), c->rec,
-                          P.masses ? c->rec_mass : nullptr};
+                          P.masses ? c->rec_mass : nullptr, start ? start : c->offsets, c->counts, ovf,
+                          c->nball};
         hipLaunchKernelGGL(c->geom.T == 16 ? hy::k_deposit_fixed<16> : hy::k_deposit_fixed<0>,
                            dim3(c->ntiles * c->nbox), dim3(hy::DEPFX_THREADS), 0, c->stream, Q,
                            static_cast<float*>(c->R));
@@ -598,10 +644,12 @@ void launch_gather(hy_pm_ctx* c, hy::GatherArgs<T, PT> a) {
     a.lat = c->nbox == 1 ? lattice_tiling(a.n, 256) : hy::LatticeTiling{0, 0, 0, 0, -1, -1, -1, -1};
     a.dfac = c->dfac;
     const bool o32 = c->nbox == 1 && (uint64_t)c->N * c->N * c->N * sizeof(T) <= (1ull << 32);
-    if (o32)
-        hipLaunchKernelGGL((hy::k_gather_kdk<T, PT, true>), dim3(blocks_for(a.n, 256)), dim3(256), 0, c->stream, a);
-    else
-        hipLaunchKernelGGL((hy::k_gather_kdk<T, PT, false>), dim3(blocks_for(a.n, 256)), dim3(256), 0, c->stream, a);
+    const dim3 grid((uint32_t)blocks_for(a.n, 256)), block(256);
+    void (*kern)(hy::GatherArgs<T, PT>) = o32 ? hy::k_gather_kdk<T, PT, true> : hy::k_gather_kdk<T, PT, false>;
+    if constexpr (std::is_same_v<T, float>) {
+        if (a.rec_out) kern = o32 ? hy::k_gather_kdk<T, PT, true, true> : hy::k_gather_kdk<T, PT, false, true>;
+    }
+    hipLaunchKernelGGL(kern, grid, block, 0, c->stream, a);
     launch_check("k_gather_kdk");
 }
 
@@ -642,15 +690,37 @@ void check_mass(hy_pm_ctx* c, double input, double deposited, uint64_t n) {
 }
 
 // Deposit stage: bucketize, deposit (+ retry with a larger fallback scratch).
+// carried_rec (binary32, asynchronous): deposit the records the previous
+// fused step wrote into its capacity layout for these positions.  rec_out:
+// also lay out the capacities of the records this step's gather writes.
 void run_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z,
-                 const double* masses, double uniform, double Min, bool sync, bool carried = false) {
+                 const double* masses, double uniform, double Min, bool sync, bool carried = false,
+                 bool carried_rec = false, bool rec_out = false) {
     mark(c, 0);
     // the binary32 deposit reads records only: no particle order is written
     uint32_t* order = nullptr;
-    bucketize(c, n, x, y, z, 0, c->counts, c->offsets, c->nball, order, carried, masses);
+    uint32_t* capn = rec_out ? c->cap_next : nullptr;
+    const uint32_t* start = nullptr;
+    const uint32_t* ovf = nullptr;
+    if (carried_rec) {
+        // the layout the records were written in, then the overflow prefix and
+        // the next layout from the final counts; overflow records are placed
+        // after the capacity slots
+        hipLaunchKernelGGL(hy::k_copy_u32, dim3(512), dim3(256), 0, c->stream, c->cap_next, c->cap_cur,
+                           c->nball + 1);
+        launch_check("k_copy_u32");
+        cap_scan(c, c->cap_cur, nullptr, capn);
+        hipLaunchKernelGGL(hy::k_place_overflow, dim3(1024), dim3(256), 0, c->stream, c->ovf_count, c->keys,
+                           c->ranks, c->ovf_rec, c->cap_cur, c->ovf_start, c->nball, c->rec);
+        launch_check("k_place_overflow");
+        start = c->cap_cur;
+        ovf = c->ovf_start;
+    } else {
+        bucketize(c, n, x, y, z, 0, c->counts, c->offsets, c->nball, order, carried, masses, capn);
+    }
     mark(c, HY_STAGE_BUCKET + 1);
     auto plan = plan_deposit(c, n, x, y, z, masses, uniform, Min);
-    launch_deposit(c, plan.P);
+    launch_deposit(c, plan.P, true, start, ovf);
     mark(c, HY_STAGE_DEPOSIT + 1);
     c->last_input_mass = Min;
     if (!sync) return;
@@ -779,7 +849,7 @@ void autotune(hy_pm_ctx* c, const hy::FftKernels<T>& K, uint32_t reps, hy_fft_tu
     (void)hipEventDestroy(e1);
     hip_check(hipMemsetAsync(c->err, 0, sizeof(uint32_t), c->stream), "memset err");
     hip_check(hipStreamSynchronize(c->stream), "sync");
-    c->keys_ready = false;
+    c->keys_ready = c->rec_ready = false;
     fill_tuning(c, K, t);
 }
 
@@ -950,8 +1020,23 @@ int hy_pm_create(const hy_pm_desc* desc, hy_pm_ctx** out) {
             c->rec64 = (hy::DepRecord64*)c->alloc(cap * sizeof(hy::DepRecord64));
             c->rec_mass64 = (double*)c->alloc(cap * sizeof(double));
         } else {
-            c->rec = (hy::DepRecord*)c->alloc(cap * sizeof(hy::DepRecord));
+            // capacity layout of carried records: the capacities (count + count/8
+            // + slack per bucket) and room for every record to overflow
+            const uint64_t slots = 2 * cap + cap / 8 + (uint64_t)hy::CAP_SLACK * c->nball + 64;
+            c->rec_layout = slots <= 0xffffffffull;
+            c->rec = (hy::DepRecord*)c->alloc((c->rec_layout ? slots : cap) * sizeof(hy::DepRecord));
             c->rec_mass = (float*)c->alloc(cap * sizeof(float));
+            if (c->rec_layout) {
+                c->cap_cur = (uint32_t*)c->alloc((c->nball + 1) * 4);
+                c->cap_next = (uint32_t*)c->alloc((c->nball + 1) * 4);
+                c->ovf_start = (uint32_t*)c->alloc((c->nball + 1) * 4);
+                c->ovf_count = (uint32_t*)c->alloc(16);
+                c->cap_block_sums = (uint32_t*)c->alloc((3 * blocks_for(c->nball, hy::SCAN_TILE) + 3) * 4);
+                // the binary32 deposit never uses the binary64 fallback scratch
+                static_assert(sizeof(hy::DepRecord) == 2 * sizeof(uint64_t), "overflow records in the scratch");
+                if (c->scratch_cap < 2 * cap) fail(HY_ELOGIC, "deposit scratch smaller than the overflow list");
+                c->ovf_rec = reinterpret_cast<hy::DepRecord*>(c->scratch);
+            }
         }
         c->gx = c->alloc(cap * rb);
         c->gy = c->alloc(cap * rb);
@@ -1005,7 +1090,7 @@ int hy_pm_check(hy_pm_ctx* c) {
         const auto st = fetch_status(c);
         if (st.err) {
             // report once; the context stays usable
-            c->keys_ready = false;
+            c->keys_ready = c->rec_ready = false;
             reset_status(c);
             hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
         }
@@ -1019,7 +1104,7 @@ int hy_pm_density(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, co
         if (!c || !x || !y || !z || !mesh_out) fail(HY_EINVAL, "hy_pm_density: null argument");
         require_single_box(c, "hy_pm_density");
         set_device(c);
-        c->keys_ready = false;
+        c->keys_ready = c->rec_ready = false;
         require_particles(c, n);
         set_boxes(c, n);
         const double M = input_mass(c, n, masses, uniform, false);
@@ -1191,7 +1276,7 @@ int hy_pm_force(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, cons
         if (n == 0) fail(HY_ERUNTIME, "PMSolver exact input mass identity must be finite and positive");
         if (!x || !y || !z || !ax || !ay || !az) fail(HY_EINVAL, "hy_pm_force: null argument");
         set_device(c);
-        c->keys_ready = false;
+        c->keys_ready = c->rec_ready = false;
         require_particles(c, n);
         const bool host = (flags & HY_HOST_POINTERS) != 0;
         if (host) {
@@ -1334,15 +1419,22 @@ int hy_pm_kdk(hy_pm_ctx* c, uint64_t n, const double* x_in, const double* y_in, 
         if (sync) reset_status(c);
         else fill_u32(c, c->scratch_cursor, 2);
         if (masses) hipLaunchKernelGGL(hy::k_check_mass, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, masses, c->err);
-        // keys of the previous fused step are valid for exactly its output positions
-        const bool carried = (flags & HY_REUSE_KEYS) && c->keys_ready && c->keys_n == n && c->keys_x == x_in &&
-                             c->keys_y == y_in && c->keys_z == z_in;
-        c->keys_ready = false;
-        run_deposit(c, n, x_in, y_in, z_in, masses, uniform, M, sync, carried);
+        // keys (or records) of the previous fused step are valid for exactly its
+        // output positions
+        const bool reuse = (flags & HY_REUSE_KEYS) != 0;
+        const bool carried = reuse && c->keys_ready && c->keys_n == n && c->keys_x == x_in && c->keys_y == y_in &&
+                             c->keys_z == z_in;
+        const bool carried_rec = reuse && !sync && c->rec_ready && c->rec_n == n && c->rec_x == x_in &&
+                                 c->rec_y == y_in && c->rec_z == z_in;
+        // binary32 uniform-mass asynchronous steps carry deposit records, others keys
+        const bool rec_out = has_next && !sync && !masses && c->rec_layout;
+        c->keys_ready = c->rec_ready = false;
+        run_deposit(c, n, x_in, y_in, z_in, masses, uniform, M, sync, carried, carried_rec, rec_out);
         // counts are consumed by the scan above; the gather below accumulates the
         // next step's counts into the zeroed buffer
         const bool make_keys = has_next;
         if (make_keys) fill_u32(c, c->counts, c->nball);
+        if (rec_out) fill_u32(c, c->ovf_count, 1);
         run_fft(c);
         auto fill = [&](auto& a) {
             a.g = c->geom; a.n = n;
@@ -1370,6 +1462,12 @@ int hy_pm_kdk(hy_pm_ctx* c, uint64_t n, const double* x_in, const double* y_in, 
         } else {
             hy::GatherArgs<float, float> a{};
             fill(a);
+            if (rec_out) {
+                a.rec_start = c->cap_next;
+                a.rec_out = c->rec;
+                a.ovf_count = c->ovf_count;
+                a.ovf_rec = c->ovf_rec;
+            }
             a.phi = static_cast<const float*>(c->R);
             a.px_in = (const float*)px_in; a.py_in = (const float*)py_in; a.pz_in = (const float*)pz_in;
             a.px_out = (float*)px_out; a.py_out = (float*)py_out; a.pz_out = (float*)pz_out;
@@ -1381,7 +1479,13 @@ int hy_pm_kdk(hy_pm_ctx* c, uint64_t n, const double* x_in, const double* y_in, 
             const auto st = fetch_status(c);
             raise_errors(st.err);
         }
-        if (make_keys) {
+        if (rec_out) {
+            c->rec_ready = true;
+            c->rec_x = x_out ? x_out : x_in;
+            c->rec_y = y_out ? y_out : y_in;
+            c->rec_z = z_out ? z_out : z_in;
+            c->rec_n = n;
+        } else if (make_keys) {
             c->keys_ready = true;
             c->keys_x = x_out ? x_out : x_in;
             c->keys_y = y_out ? y_out : y_in;
@@ -1397,7 +1501,7 @@ int hy_pm_bucket_order(hy_pm_ctx* c, uint64_t n, const double* x, const double* 
         if (!c || !x || !y || !z || !order || !offsets) fail(HY_EINVAL, "hy_pm_bucket_order: null argument");
         require_single_box(c, "hy_pm_bucket_order");
         set_device(c);
-        c->keys_ready = false;
+        c->keys_ready = c->rec_ready = false;
         require_particles(c, n);
         set_boxes(c, n);
         reset_status(c);
@@ -1416,7 +1520,7 @@ int hy_pm_xplane_order(hy_pm_ctx* c, uint64_t n, const double* x, uint32_t* orde
         if (!c || !x || !order || !offsets) fail(HY_EINVAL, "hy_pm_xplane_order: null argument");
         require_single_box(c, "hy_pm_xplane_order");
         set_device(c);
-        c->keys_ready = false;
+        c->keys_ready = c->rec_ready = false;
         require_particles(c, n);
         set_boxes(c, n);
         reset_status(c);
@@ -1580,7 +1684,7 @@ int hy_pm_kdk_vjp(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, co
         set_device(c);
         require_particles(c, n);
         set_boxes(c, n);
-        c->keys_ready = false;
+        c->keys_ready = c->rec_ready = false;
         const double M = input_mass(c, n, nullptr, uniform, false);
         ensure_adjoint_workspace(c);
         reset_status(c);
@@ -1740,7 +1844,7 @@ int hy_pm_power_spectrum(hy_pm_ctx* c, uint64_t n, const double* x, const double
         if (!x || !y || !z) fail(HY_EINVAL, "hy_pm_power_spectrum: null argument");
         require_single_box(c, "hy_pm_power_spectrum");
         set_device(c);
-        c->keys_ready = false;
+        c->keys_ready = c->rec_ready = false;
         require_particles(c, n);
         set_boxes(c, n);
         const bool host = (flags & HY_HOST_POINTERS) != 0;
@@ -1952,23 +2056,36 @@ int hy_ensemble_step(hy_ensemble* e, double k_prev, double k_next, double d_next
         if ((has_prev && !std::isfinite(k_prev)) || (has_next && (!std::isfinite(k_next) || !std::isfinite(d_next))))
             fail(HY_EOVERFLOW, "Leapfrog drift/kick factors must be finite");
         hip_check(hipSetDevice(e->device), "hipSetDevice");
-        if (e->fresh) {
-            // positions were replaced outside the graphs: one eager step with
-            // fresh bucket keys, after which the recorded steps' reused keys
-            // are again those of the members' current positions
-            const uint32_t kflags = (has_prev ? HY_HAS_PREV : 0u) | (has_next ? HY_HAS_NEXT : 0u) | HY_ASYNC;
+        // A recorded step replays the routing its capture saw: it must start
+        // from what a replay leaves, the deposit records (binary32, uniform
+        // mass) or bucket keys of the members' current positions.  Until
+        // every member holds them (and for a first step, which has nothing to
+        // carry) the step runs eagerly.
+        bool steady = has_prev && !e->fresh;
+        for (const hy_ensemble_member& m : e->members) {
+            const hy_pm_ctx* c = m.ctx;
+            const bool recs = c->rec_ready && c->rec_n == m.n && c->rec_x == m.x && c->rec_y == m.y && c->rec_z == m.z;
+            const bool keys = c->keys_ready && c->keys_n == m.n && c->keys_x == m.x && c->keys_y == m.y &&
+                              c->keys_z == m.z;
+            steady = steady && (c->rec_layout ? recs : keys);
+        }
+        if (!steady) {
+            // positions replaced outside the graphs (fresh: route them anew),
+            // or the carried routing is not yet the replays' own
+            const uint32_t kflags = (has_prev ? HY_HAS_PREV : 0u) | (has_next ? HY_HAS_NEXT : 0u) | HY_ASYNC |
+                                    (e->fresh ? 0u : HY_REUSE_KEYS);
             for (const hy_ensemble_member& m : e->members) {
                 const int rc = hy_pm_kdk(m.ctx, m.n, m.x, m.y, m.z, m.px, m.py, m.pz, m.x, m.y, m.z, m.px, m.py,
                                          m.pz, nullptr, m.uniform_mass, k_prev, k_next, d_next, nullptr, nullptr,
                                          nullptr, kflags);
-                if (rc != HY_OK) fail(rc, "ensemble re-keying step: " + g_last_error);
+                if (rc != HY_OK) fail(rc, "ensemble eager step: " + g_last_error);
             }
             e->fresh = false;
             return;
         }
+        const int idx = (has_prev ? 1 : 0) | (has_next ? 2 : 0);
         hipLaunchKernelGGL(hy::k_set_factors, dim3(1), dim3(64), 0, e->stream, e->dfac, k_prev, k_next, d_next);
         launch_check("k_set_factors");
-        const int idx = (has_prev ? 1 : 0) | (has_next ? 2 : 0);
         if (!e->exec[idx]) {
             const uint32_t kflags = (has_prev ? HY_HAS_PREV : 0u) | (has_next ? HY_HAS_NEXT : 0u) | HY_ASYNC |
                                     HY_REUSE_KEYS;
@@ -2014,7 +2131,7 @@ int hy_ensemble_reset(hy_ensemble* e) {
         hip_check(hipSetDevice(e->device), "hipSetDevice");
         hip_check(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
         for (const hy_ensemble_member& m : e->members) {
-            m.ctx->keys_ready = false;
+            m.ctx->keys_ready = m.ctx->rec_ready = false;
             reset_status(m.ctx);  // errors of the replaced run are discarded
         }
         hip_check(hipStreamSynchronize(e->stream), "hipStreamSynchronize");

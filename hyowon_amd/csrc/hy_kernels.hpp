@@ -40,6 +40,13 @@ __device__ __forceinline__ uint32_t bucket_of(int bx, int by, int bz, const Geom
     return (((uint32_t)tx * g.nT + ty) * g.nT + tz) * 8u + cls;
 }
 
+// Cell of a deposit record inside its tile: (lx*T + ly)*T + lz.
+__device__ __forceinline__ uint32_t record_cell(int bx, int by, int bz, const Geom& g) {
+    const int T = g.T;
+    const int lx = T == 16 ? bx & 15 : bx % T, ly = T == 16 ? by & 15 : by % T, lz = T == 16 ? bz & 15 : bz % T;
+    return (uint32_t)((lx * T + ly) * T + lz);
+}
+
 // Wave-aggregated integer atomic: lanes with equal keys reserve consecutive
 // slots.  Groups are found with ballots first; then every group leader issues
 // its atomic in the same instruction, so a wave pays one atomic round trip.
@@ -172,6 +179,154 @@ __global__ void k_scatter(uint64_t n, const uint32_t* __restrict__ keys,
     }
 }
 
+// Deposit record in bucket order: tile-local base cell (lx*T+ly)*T+lz and the
+// three CIC fractions (computed in binary64, stored in binary32).
+struct DepRecord {
+    uint32_t cell;
+    float fx, fy, fz;
+};
+
+// ---------------------------------------------------------------------------
+// Capacity layout of carried deposit records (binary32, uniform mass).
+//
+// A fused step that drifts the particles writes each particle's deposit
+// record for the next step straight into a slot of its bucket: bucket b owns
+// cap(c_b) = c_b + c_b/8 + CAP_SLACK slots, c_b its count at this step (the
+// particles move little per step), and a record whose rank within its bucket
+// reaches the capacity goes to an overflow list instead.  Before the next
+// deposit one three-channel scan over the final counts gives
+//   ch 0: the exclusive prefix of the counts (dense offsets; its total is the
+//         routing identity's grand total),
+//   ch 1: the prefix of the overflow counts max(0, c_b - cap_b) of the layout
+//         the records were written in (overflow records are placed after the
+//         capacity slots, bucket by bucket),
+//   ch 2: the prefix of cap(c_b), the layout the next step's records go to.
+// The binary32 deposit sums integers, so the order of a bucket's records
+// (the ranks come from atomics) leaves every cell sum unchanged.
+// ---------------------------------------------------------------------------
+constexpr uint32_t CAP_SLACK = 4;
+
+__device__ __forceinline__ uint32_t capacity_of(uint32_t count) { return count + (count >> 3) + CAP_SLACK; }
+
+struct CapScan {
+    const uint32_t* counts;  // records per bucket
+    uint64_t nb;             // buckets over all boxes
+    const uint32_t* cur;     // nb+1 prefix of the layout the records were written in (null: no overflow)
+    uint32_t* dense;         // out: nb+1 prefix of the counts (or null)
+    uint32_t* ovf_start;     // out: nb+1 prefix of the overflow counts (when cur)
+    uint32_t* capn;          // out: nb+1 prefix of the next layout's capacities (or null)
+    uint32_t* block_sums;    // 3 per scan block
+    uint32_t* grand_total;   // out: total count
+};
+
+__device__ __forceinline__ void cap_channels(const CapScan& s, uint64_t b, uint32_t v[3]) {
+    const uint32_t c = s.counts[b];
+    v[0] = c;
+    if (s.cur) {
+        const uint32_t cap = s.cur[b + 1] - s.cur[b];
+        v[1] = c > cap ? c - cap : 0u;
+    } else {
+        v[1] = 0u;
+    }
+    v[2] = capacity_of(c);
+}
+
+__global__ void k_cap_reduce(CapScan s) {
+    __shared__ uint32_t sh[SCAN_THREADS];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    uint32_t acc[3] = {0u, 0u, 0u};
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        const uint64_t j = base + (uint64_t)i * SCAN_THREADS + threadIdx.x;
+        if (j < s.nb) {
+            uint32_t v[3];
+            cap_channels(s, j, v);
+            for (int ch = 0; ch < 3; ++ch) acc[ch] += v[ch];
+        }
+    }
+    for (int ch = 0; ch < 3; ++ch) {
+        uint32_t total;
+        (void)block_exclusive_scan(acc[ch], sh, total);
+        if (threadIdx.x == 0) s.block_sums[3 * blockIdx.x + ch] = total;
+    }
+}
+
+// One workgroup: exclusive scan of each channel's block sums; the totals go
+// to index nb of each output and the count total to grand_total.
+__global__ void k_cap_blocks(CapScan s, uint32_t nblocks) {
+    __shared__ uint32_t sh[SCAN_THREADS];
+    for (int ch = 0; ch < 3; ++ch) {
+        uint32_t carry = 0;
+        for (uint32_t base = 0; base < nblocks; base += SCAN_THREADS) {
+            const uint32_t j = base + threadIdx.x;
+            const uint32_t v = j < nblocks ? s.block_sums[3 * j + ch] : 0u;
+            uint32_t total;
+            const uint32_t ex = block_exclusive_scan(v, sh, total);
+            if (j < nblocks) s.block_sums[3 * j + ch] = ex + carry;
+            carry += total;
+        }
+        if (threadIdx.x == 0) {
+            if (ch == 0) {
+                *s.grand_total = carry;
+                if (s.dense) s.dense[s.nb] = carry;
+            } else if (ch == 1 && s.cur) {
+                s.ovf_start[s.nb] = carry;
+            } else if (ch == 2 && s.capn) {
+                s.capn[s.nb] = carry;
+            }
+        }
+    }
+}
+
+__global__ void k_cap_apply(CapScan s) {
+    __shared__ uint32_t sh[SCAN_THREADS];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS][3];
+    uint32_t acc[3] = {0u, 0u, 0u};
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        if (base + i < s.nb) {
+            cap_channels(s, base + i, v[i]);
+        } else {
+            v[i][0] = v[i][1] = v[i][2] = 0u;
+        }
+        for (int ch = 0; ch < 3; ++ch) acc[ch] += v[i][ch];
+    }
+    uint32_t run[3];
+    for (int ch = 0; ch < 3; ++ch) {
+        uint32_t total;
+        run[ch] = block_exclusive_scan(acc[ch], sh, total) + s.block_sums[3 * blockIdx.x + ch];
+    }
+    uint32_t* out[3] = {s.dense, s.cur ? s.ovf_start : nullptr, s.capn};
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        if (base + i < s.nb) {
+            for (int ch = 0; ch < 3; ++ch)
+                if (out[ch]) out[ch][base + i] = run[ch];
+        }
+        for (int ch = 0; ch < 3; ++ch) run[ch] += v[i][ch];
+    }
+}
+
+__global__ void k_copy_u32(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// Overflow records of the carried layout cur: entry i holds its bucket, its
+// rank and its record; it goes to cur[nb] + ovf_start[b] + (rank - cap_b), so each
+// bucket's overflow records follow the capacity slots contiguously.  Grid
+// stride up to the device count the gather left.
+__global__ void k_place_overflow(const uint32_t* __restrict__ ovf_count, const uint32_t* __restrict__ ovf_key,
+                                 const uint32_t* __restrict__ ovf_rank, const DepRecord* __restrict__ ovf_rec,
+                                 const uint32_t* __restrict__ cur, const uint32_t* __restrict__ ovf_start,
+                                 uint64_t nb, DepRecord* __restrict__ rec) {
+    const uint32_t total = *ovf_count;
+    const uint32_t base = cur[nb];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const uint32_t b = ovf_key[i];
+        const uint32_t cap = cur[b + 1] - cur[b];
+        rec[base + ovf_start[b] + (ovf_rank[i] - cap)] = ovf_rec[i];
+    }
+}
+
 // Scatter with keys made by an earlier call: each key is checked against the
 // bucket of the current position first (ERR_STALE_KEYS on a mismatch).
 __global__ void k_scatter_checked(uint64_t n, const double* __restrict__ x, const double* __restrict__ y,
@@ -194,13 +349,6 @@ __global__ void k_scatter_checked(uint64_t n, const double* __restrict__ x, cons
     }
     order[offsets[k] + ranks[p]] = (uint32_t)p;
 }
-
-// Deposit record in bucket order: tile-local base cell (lx*T+ly)*T+lz and the
-// three CIC fractions (computed in binary64, stored in binary32).
-struct DepRecord {
-    uint32_t cell;
-    float fx, fy, fz;
-};
 
 // Binary64 deposit record: tile-local base cell, the particle index (the
 // reference's per-cell summation order is by index) and the binary64
@@ -241,10 +389,8 @@ __global__ void k_scatter_records(uint64_t n, const double* __restrict__ x, cons
     }
     const uint32_t dst = offsets[k] + ranks[p];
     if (order) order[dst] = (uint32_t)p;
-    const int lx = g.T == 16 ? bx & 15 : bx % g.T, ly = g.T == 16 ? by & 15 : by % g.T,
-              lz = g.T == 16 ? bz & 15 : bz % g.T;
     R r;
-    r.cell = (uint32_t)((lx * g.T + ly) * g.T + lz);
+    r.cell = record_cell(bx, by, bz, g);
     if constexpr (sizeof(R) == sizeof(DepRecord64)) {
         r.index = (uint32_t)p;
         r.fx = fx;
@@ -744,9 +890,90 @@ struct DepFxParams {
     DepParams base;
     double mass_unit;               // uniform mass, or max explicit mass
     double fixed_to_mass;           // mass_unit * 2^-32
-    const DepRecord* rec;           // deposit records in bucket order
-    const float* rec_mass;          // explicit masses / mass_unit in bucket order (or null)
+    const DepRecord* rec;           // deposit records by bucket
+    const float* rec_mass;          // explicit masses / mass_unit, as rec (or null)
+    const uint32_t* start;          // nb+1: first slot of each bucket (dense offsets or a capacity layout)
+    const uint32_t* counts;         // records per bucket
+    const uint32_t* ovf_start;      // nb+1 prefix of the overflow records (or null: none)
+    uint64_t nb;                    // buckets over all boxes (start[nb] = first overflow slot)
 };
+
+// The 27 buckets that feed a tile: its own eight classes, then the 19
+// neighbour (d, c) slots 1..19 of range_slot.  Each gives a main range (its
+// slots, at most its capacity) and an overflow range.
+constexpr int DEPFX_BUCKETS = 27;
+constexpr int DEPFX_NRANGES = 2 * DEPFX_BUCKETS;
+static_assert(DEPFX_NRANGES <= 64, "one lane per range");
+
+// The non-empty ranges in order: rs their first slot, rc the prefix of their
+// counts (rc[DEPFX_NRANGES] = total), rd the neighbour offset d, *nranges
+// their number; one global round trip, the first wave.
+__device__ __forceinline__ void load_fx_ranges(const DepFxParams& Q, uint32_t box_base, int tx, int ty, int tz,
+                                               int nT, uint32_t* rs, uint32_t* rc, uint8_t* rd,
+                                               uint32_t* nranges) {
+    const int t = threadIdx.x;
+    if (t < 64) {
+        uint32_t cnt = 0, rs_lane = 0;
+        uint8_t rd_lane = 0;
+        if (t < DEPFX_NRANGES) {
+            const int j = t % DEPFX_BUCKETS;
+            const bool ovf = t >= DEPFX_BUCKETS;
+            int d = 0, cls = j;
+            if (j >= 8) {
+                const RangeSlot sl = range_slot(j - 7);
+                d = sl.d;
+                cls = sl.c;
+            }
+            const int sx = (tx - ((d >> 2) & 1) + nT) % nT, sy = (ty - ((d >> 1) & 1) + nT) % nT,
+                      sz = (tz - (d & 1) + nT) % nT;
+            const uint32_t b = box_base + (((uint32_t)sx * nT + sy) * nT + sz) * 8u + (uint32_t)cls;
+            uint32_t first = 0;
+            if (!ovf) {
+                const uint32_t a = Q.start[b], e = Q.start[b + 1], c = Q.counts[b];
+                first = a;
+                cnt = c < e - a ? c : e - a;
+            } else if (Q.ovf_start) {
+                const uint32_t a = Q.ovf_start[b], e = Q.ovf_start[b + 1];
+                first = Q.start[Q.nb] + a;
+                cnt = e - a;
+            }
+            rs_lane = first;
+            rd_lane = (uint8_t)d;
+        }
+        // the non-empty ranges, compacted in order: rc[0..R] is their count
+        // prefix and rc[R..DEPFX_NRANGES] repeats the total
+        const uint64_t live = __ballot(cnt != 0u);
+        const uint32_t at = (uint32_t)__popcll(live & ((1ull << t) - 1ull));
+        const uint32_t R = (uint32_t)__popcll(live);
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o);
+            if (t >= o) incl += y;
+        }
+        if (cnt != 0u) {
+            rs[at] = rs_lane;
+            rd[at] = rd_lane;
+            rc[at + 1] = incl;
+        }
+        const uint32_t total = __shfl(incl, 63);
+        if (t == 0) rc[0] = 0;
+        if ((uint32_t)t > R && t <= DEPFX_NRANGES) rc[t] = total;
+        if (t == 0) *nranges = R;
+    }
+    __syncthreads();
+}
+
+// The range of record v: the last r < R with rc[r] <= v (binary search).
+__device__ __forceinline__ int fx_range_of(const uint32_t* rc, uint32_t R, uint32_t v) {
+    int lo = 0, hi = (int)R - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (rc[mid] <= v) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
 
 // TE: compile-time tile edge (16, the edge of every mesh divisible by 16:
 // cell index arithmetic becomes shifts), or 0 to read it from the geometry.
@@ -754,9 +981,9 @@ template <int TE>
 __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, float* __restrict__ mesh) {
     const DepParams& P = Q.base;
     __shared__ unsigned long long acc[DEPFX_MAX_CELLS];
-    __shared__ uint32_t sh_rs[DEP_NRANGES], sh_rc[DEP_NRANGES + 1];
-    __shared__ uint8_t sh_rd[DEP_NRANGES];
-    __shared__ uint8_t sh_rsrc[DEP_NRANGES];
+    __shared__ uint32_t sh_rs[DEPFX_NRANGES], sh_rc[DEPFX_NRANGES + 1];
+    __shared__ uint8_t sh_rd[DEPFX_NRANGES];
+    __shared__ uint32_t sh_nr;
     __shared__ double sh_red[DEPFX_THREADS / 64];
 
     const Geom& g = P.g;
@@ -771,10 +998,10 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
     mesh += (uint64_t)box * g.N * g.N * g.N;
 
     for (int c = threadIdx.x; c < cells; c += blockDim.x) acc[c] = 0ull;
-    load_source_ranges(P.offsets + (uint64_t)box * buckets_per_box(g), tx, ty, tz, nT, sh_rs, sh_rc, sh_rd,
-                       sh_rsrc);
-    const uint32_t M = sh_rc[DEP_NRANGES];
-    int r = 0;
+    load_fx_ranges(Q, box * buckets_per_box(g), tx, ty, tz, nT, sh_rs, sh_rc, sh_rd, &sh_nr);
+    const uint32_t M = sh_rc[DEPFX_NRANGES];
+    const uint32_t nr = sh_nr;
+    int r = threadIdx.x < M ? fx_range_of(sh_rc, nr, threadIdx.x) : 0;
     bool bad = false;
     // records in pairs: both loads are issued before the first atomic, so a
     // thread with two records waits for one memory round trip, not two
@@ -1013,8 +1240,12 @@ struct GatherArgs {
     T* g_z;
     LatticeTiling lat;          // block schedule over the Lagrangian lattice (or identity)
     uint32_t* next_counts;      // optional: bucket counts/ranks of the drifted positions
-    uint32_t* next_keys;
-    uint32_t* next_ranks;
+    uint32_t* next_keys;        // keys and ranks per particle, or with rec_out the overflow
+    uint32_t* next_ranks;       // list's bucket and rank per entry
+    const uint32_t* rec_start;  // (binary32) the drifted positions' deposit records are written into
+    DepRecord* rec_out;         // the capacity layout rec_start (nb+1 prefix); those past a bucket's
+    uint32_t* ovf_count;        // capacity are appended to the overflow list (ovf_rec, next_keys,
+    DepRecord* ovf_rec;         // next_ranks)
     const double* dfac;         // optional: k_prev, k_next, d_next read from device memory
                                 // (graph replays take new factors without re-capture)
     double k_prev;              // K2 of the previous step
@@ -1027,8 +1258,9 @@ struct GatherArgs {
 
 // O32: one box whose mesh spans at most 2^32 bytes; the stencil then uses
 // 32-bit byte offsets from the mesh base (one scalar base, no per-lane 64-bit
-// address arithmetic).
-template <typename T, typename PT, bool O32 = false>
+// address arithmetic).  REC: the drifted positions' deposit records are
+// written into the capacity layout (binary32); otherwise keys and ranks.
+template <typename T, typename PT, bool O32 = false, bool REC = false>
 __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
     // each XCD walks a contiguous particle range (spatially compact for
     // lattice-ordered particles), so potential lines are reused in its L2
@@ -1039,6 +1271,7 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
     bool live = p < a.n;
     uint32_t next_key = 0;
     bool key_ok = false;
+    DepRecord next_rec{0u, 0.0f, 0.0f, 0.0f};
     if (live) {
         int ix = 0, iy = 0, iz = 0;
         double fx = 0.0, fy = 0.0, fz = 0.0;
@@ -1217,10 +1450,18 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
                             if (a.next_counts) {
                                 // bucket of the drifted position for the next deposit
                                 int bx, by, bz;
-                                double f;
-                                key_ok = cic_axis(nx, g, bx, f) && cic_axis(ny, g, by, f)
-                                         && cic_axis(nz, g, bz, f);
-                                if (key_ok) next_key = box_of(p, g) * buckets_per_box(g) + bucket_of(bx, by, bz, g);
+                                double fxn, fyn, fzn;
+                                key_ok = cic_axis(nx, g, bx, fxn) && cic_axis(ny, g, by, fyn)
+                                         && cic_axis(nz, g, bz, fzn);
+                                if (key_ok) {
+                                    next_key = box_of(p, g) * buckets_per_box(g) + bucket_of(bx, by, bz, g);
+                                    if constexpr (REC) {  // the record k_scatter_records would write
+                                        next_rec.cell = record_cell(bx, by, bz, g);
+                                        next_rec.fx = (float)fxn;
+                                        next_rec.fy = (float)fyn;
+                                        next_rec.fz = (float)fzn;
+                                    }
+                                }
                             }
                         }
                     }
@@ -1230,10 +1471,45 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
         if (fail) atomicOr(a.err, fail);
     }
     if (a.next_counts) {  // uniform across the grid: every lane reaches the ballots
-        const uint32_t rank = aggregated_rank(a.next_counts, next_key, live && key_ok);
-        if (live) {
-            __builtin_nontemporal_store(key_ok ? next_key : 0xffffffffu, &a.next_keys[p]);
-            __builtin_nontemporal_store(rank, &a.next_ranks[p]);
+        const bool have = live && key_ok;
+        if constexpr (REC) {
+            // the bucket's capacity slots are loaded ahead of the rank atomic, so
+            // the two round trips overlap
+            uint32_t s0 = 0, s1 = 0;
+            if (have) {
+                s0 = a.rec_start[next_key];
+                s1 = a.rec_start[next_key + 1];
+            }
+            const uint32_t rank = aggregated_rank(a.next_counts, next_key, have);
+            uint32_t slot = 0;
+            bool over = false;
+            if (have) {
+                if (rank < s1 - s0) slot = s0 + rank;
+                else over = true;
+            }
+            const uint64_t om = __ballot(over);
+            uint32_t obase = 0;
+            if (om) {  // one atomic per wave for its overflow records
+                const int leader = __ffsll((unsigned long long)om) - 1;
+                if ((int)lane_id() == leader) obase = atomicAdd(a.ovf_count, (uint32_t)__popcll(om));
+                obase = __shfl(obase, leader);
+            }
+            if (have) {
+                if (!over) {
+                    a.rec_out[slot] = next_rec;
+                } else {
+                    const uint32_t o = obase + (uint32_t)__popcll(om & ((1ull << lane_id()) - 1ull));
+                    a.ovf_rec[o] = next_rec;
+                    a.next_keys[o] = next_key;
+                    a.next_ranks[o] = rank;
+                }
+            }
+        } else {
+            const uint32_t rank = aggregated_rank(a.next_counts, next_key, have);
+            if (live) {
+                __builtin_nontemporal_store(key_ok ? next_key : 0xffffffffu, &a.next_keys[p]);
+                __builtin_nontemporal_store(rank, &a.next_ranks[p]);
+            }
         }
     }
 }

@@ -128,7 +128,15 @@ int hy_pm_force(hy_pm_ctx* ctx, uint64_t n, const double* x, const double* y, co
  * drifted positions; HY_REUSE_KEYS on the next call skips recomputing them.
  * Reused keys are checked against the positions: synchronous calls fall back
  * to fresh keys when they differ, HY_ASYNC calls report logic_error through
- * hy_pm_check (outputs of that call are then undefined). */
+ * hy_pm_check (outputs of that call are then undefined).
+ * HY_ASYNC calls with HY_HAS_NEXT on a binary32 context with uniform mass
+ * carry more: the deposit records of the drifted positions, written by the
+ * gather into per-bucket capacity slots (overflow placed before the next
+ * deposit), and the next HY_ASYNC | HY_REUSE_KEYS call deposits them without
+ * re-reading the positions.  Results are bit-identical to routing afresh; the
+ * caller must pass that call's output positions unmodified, since a change
+ * made to them in place is not detected on this path (synchronous calls route
+ * from the positions). */
 int hy_pm_kdk(hy_pm_ctx* ctx, uint64_t n, const double* x_in, const double* y_in,
               const double* z_in, const void* px_in, const void* py_in, const void* pz_in,
               double* x_out, double* y_out, double* z_out, void* px_out, void* py_out,
@@ -277,13 +285,17 @@ int hy_pm_density_vjp(hy_pm_ctx* ctx, uint64_t n, const double* x, const double*
  * from device memory, instantiates it and launches it; later steps only write
  * the three factors and relaunch the graph.  A run starts with one eager
  * hy_pm_kdk per member with HY_HAS_NEXT only (force at the initial positions,
- * K1, drift) before the first graph step, so that the recorded steps reuse the
- * bucket keys the previous step made.  The recorded steps assume the members'
- * positions change only through the ensemble: after positions are written by
- * anything else (new initial conditions in the same buffers), call
- * hy_ensemble_reset, whose next hy_ensemble_step runs eagerly with fresh keys
- * (it also discards errors not yet reported); without it the reused keys are
- * stale and hy_ensemble_check raises logic_error.  Calls do not synchronise;
+ * K1, drift) before the first graph step.  A step is recorded only when every
+ * member holds what a recorded step leaves for its successor (the deposit
+ * records of binary32 members, the bucket keys of binary64 ones, made for the
+ * members' current positions); until then, and for steps without HY_HAS_PREV,
+ * the step runs eagerly.  The recorded steps assume the members' positions
+ * change only through the ensemble: after positions are written by anything
+ * else (new initial conditions in the same buffers), call hy_ensemble_reset,
+ * whose next hy_ensemble_step runs eagerly and routes the particles afresh (it
+ * also discards errors not yet reported); without it binary64 members' reused
+ * keys are stale (hy_ensemble_check raises logic_error) and binary32 members
+ * deposit stale records.  Calls do not synchronise;
  * hy_ensemble_check checks every member, clears their error words and returns
  * the first kernel-raised error. */
 typedef struct hy_ensemble hy_ensemble;

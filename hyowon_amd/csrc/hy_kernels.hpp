@@ -577,6 +577,15 @@ __device__ __forceinline__ T contrast_of(T sum, const DepParams& P) {
     return (T)v;
 }
 
+// Binary32 deposit: (m - M/N_c) / dx^3 in binary64, rounded to binary32.  It
+// differs from the reference's fma-centred form (taken by contrast_of when
+// M/N_c is not exact) by about one binary64 ulp, far below the binary32
+// rounding of the cell, and avoids that form's 64-bit mantissa arithmetic.
+__device__ __forceinline__ float contrast_fp32(double m, const DepParams& P) {
+    if (P.raw) return (float)m;
+    return (float)((m - P.mean_mass) * P.inv_volume);
+}
+
 template <typename T, bool LDS_PAYLOAD, int TE, typename KP, typename FP>
 __device__ __forceinline__ void deposit_cells(const DepParams& P, T* __restrict__ mesh, int tx,
                                               int ty, int tz, KP keys, uint32_t M,
@@ -1079,7 +1088,7 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
             for (int j = 0; j < 4; ++j) {
                 const long long q = (long long)acc[c + j];
                 mass_acc += (double)q;
-                o[j] = contrast_of((double)q * Q.fixed_to_mass, P);
+                o[j] = contrast_fp32((double)q * Q.fixed_to_mass, P);
             }
             const uint64_t gx = (uint64_t)(tx * T + lx), gy = (uint64_t)(ty * T + ly), gz = (uint64_t)(tz * T + lz);
             *reinterpret_cast<F4*>(&mesh[(gx * g.N + gy) * g.N + gz]) = F4{o[0], o[1], o[2], o[3]};
@@ -1091,7 +1100,7 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
             const double mcell = (double)q * Q.fixed_to_mass;
             const int lz = c % T, ly = (c / T) % T, lx = c / (T * T);
             const uint64_t gx = (uint64_t)(tx * T + lx), gy = (uint64_t)(ty * T + ly), gz = (uint64_t)(tz * T + lz);
-            mesh[(gx * g.N + gy) * g.N + gz] = contrast_of((double)mcell, P);
+            mesh[(gx * g.N + gy) * g.N + gz] = contrast_fp32(mcell, P);
         }
     }
     // deterministic per-tile mass in units of the fixed-point quantum

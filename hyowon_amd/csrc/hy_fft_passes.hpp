@@ -224,31 +224,73 @@ __global__ void __launch_bounds__((RowsCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) 
     const uint64_t q0 = (uint64_t)a.kz0 * N + (uint64_t)b * ROWS;  // row index q = kz*N + ky
     const int tid = threadIdx.x;
     C* rows = a.B + q0 * N;
-    staged_copy<ROWS * N, NT, C>(
-        tid, [&](int l) { return rows[l]; }, [&](int l, C v) { lds[(l / N) * RS + lds_slot(l % N)] = v; });
-    __syncthreads();
-    fft_rows<C, N, ROWS, RS, -1>(lds, a.tw, 1, tid);
     constexpr int PER = ROWS * N / NT;
     static_assert(ROWS * N % NT == 0, "x pass covers whole rows");
+    if constexpr (!SG && N % NT == 0) {
+        // each thread owns the columns kx = tid + j NT of all ROWS rows: the
+        // rows' (ky, kz) table values are uniform over the workgroup and its
+        // few kx values are loaded once, all ahead of the row fetch
+        constexpr int PX = N / NT;
+        T lxv[PX], wxv[PX], lyv[ROWS], lzv[ROWS], wyv[ROWS], wzv[ROWS];
+        int kyr[ROWS], kzr[ROWS];
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-        const int l = tid + i * NT;
-        const int r = l / N, kx = l % N;
-        const uint64_t q = q0 + r;
-        const int kz = (int)(q / N), ky = (int)(q % N);
-        T f;
-        if constexpr (SG) {
-            f = green_factor_treepm(a, kx, ky, kz);
-        } else if (kx == 0 && ky == 0 && kz == 0) {
-            f = (T)0;
-        } else {
-            f = green_factor(a, kx, ky, kz);
+        for (int j = 0; j < PX; ++j) {
+            lxv[j] = a.green_lap[tid + j * NT];
+            wxv[j] = a.green_win[tid + j * NT];
         }
-        C v = lds[r * RS + lds_slot(kx)];
-        v.x *= f;
-        v.y *= f;
-        if (SG && a.deriv == 1) v = mul_mi(v, a.kd[kx]);
-        lds[r * RS + lds_slot(kx)] = v;
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+            const uint64_t q = q0 + r;
+            kzr[r] = (int)(q / N);
+            kyr[r] = (int)(q % N);
+            lyv[r] = a.green_lap[kyr[r]];
+            lzv[r] = a.green_lap[kzr[r]];
+            wyv[r] = a.green_win[kyr[r]];
+            wzv[r] = a.green_win[kzr[r]];
+        }
+        staged_copy<ROWS * N, NT, C>(
+            tid, [&](int l) { return rows[l]; }, [&](int l, C v) { lds[(l / N) * RS + lds_slot(l % N)] = v; });
+        __syncthreads();
+        fft_rows<C, N, ROWS, RS, -1>(lds, a.tw, 1, tid);
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+#pragma unroll
+            for (int j = 0; j < PX; ++j) {
+                const int kx = tid + j * NT;
+                const T f = (kx == 0 && kyr[r] == 0 && kzr[r] == 0)
+                                ? (T)0
+                                : green_axes<T, 0>(a, lxv[j], lyv[r], lzv[r], wxv[j], wyv[r], wzv[r]);
+                C v = lds[r * RS + lds_slot(kx)];
+                v.x *= f;
+                v.y *= f;
+                lds[r * RS + lds_slot(kx)] = v;
+            }
+        }
+    } else {
+        staged_copy<ROWS * N, NT, C>(
+            tid, [&](int l) { return rows[l]; }, [&](int l, C v) { lds[(l / N) * RS + lds_slot(l % N)] = v; });
+        __syncthreads();
+        fft_rows<C, N, ROWS, RS, -1>(lds, a.tw, 1, tid);
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int l = tid + i * NT;
+            const int r = l / N, kx = l % N;
+            const uint64_t q = q0 + r;
+            const int kz = (int)(q / N), ky = (int)(q % N);
+            T f;
+            if constexpr (SG) {
+                f = green_factor_treepm(a, kx, ky, kz);
+            } else if (kx == 0 && ky == 0 && kz == 0) {
+                f = (T)0;
+            } else {
+                f = green_factor(a, kx, ky, kz);
+            }
+            C v = lds[r * RS + lds_slot(kx)];
+            v.x *= f;
+            v.y *= f;
+            if (SG && a.deriv == 1) v = mul_mi(v, a.kd[kx]);
+            lds[r * RS + lds_slot(kx)] = v;
+        }
     }
     __syncthreads();
     fft_rows<C, N, ROWS, RS, +1>(lds, a.tw, 1, tid);

@@ -128,6 +128,8 @@ def run_gpu(args):
             plan = solver.tuning()
         else:
             plan = solver.autotune(reps=3) if args.autotune else solver.tuning()
+        # gather schedule (lattice rows per column tile) for these particles
+        plan["gather"] = solver.autotune_gather(st.x, st.y, st.z, m, reps=2) if args.autotune else None
         real = 4 if prec == "fp32" else 8
         steps = args.steps if prec == args.precisions[0] else max(2, args.fp64_steps)
         warm = args.warmup if prec == args.precisions[0] else 1

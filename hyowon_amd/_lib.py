@@ -29,9 +29,13 @@ EXPORTS = [
     "hy_pm_interpolate", "hy_ic_lpt", "hy_xplane_order_static", "hy_static_last_error",
     "hy_growth", "hy_linear_power", "hy_ic_mode_variance", "hy_cubic_spline", "hy_pm_potential_energy",
     "hy_snapshot_write", "hy_snapshot_read_header", "hy_snapshot_read", "hy_snapshot_last_error",
+    "hy_pm_autotune_gather", "hy_pm_set_gather_rows",
 ]
 
 HY_FFT_PASSES = 5
+HY_GATHER_ROWS_DEFAULT = 0xFFFFFFFF
+HY_GATHER_TUNE_CANDIDATES = 5
+GATHER_TUNE_ROWS = [0, 16, 64, 256, 1024]
 HY_FFT_MAX_VARIANTS = 4
 FFT_PASS_NAMES = ["z_r2c", "y_fwd", "x_green", "y_bwd", "z_c2r"]
 
@@ -193,6 +197,8 @@ def lib():
     fn("hy_pm_get_tuning", i, vp, C.POINTER(hy_fft_tuning))
     fn("hy_pm_set_tuning", i, vp, C.POINTER(hy_fft_tuning))
     fn("hy_pm_autotune", i, vp, u32, C.POINTER(hy_fft_tuning))
+    fn("hy_pm_autotune_gather", i, vp, u64, vp, vp, vp, d, u32, C.POINTER(u32), C.POINTER(C.c_float))
+    fn("hy_pm_set_gather_rows", i, vp, u32)
     fn("hy_ensemble_create", i, C.POINTER(hy_ensemble_member), u32, C.POINTER(vp))
     fn("hy_ensemble_step", i, vp, d, d, d, u32)
     fn("hy_ensemble_check", i, vp)

@@ -97,6 +97,7 @@ struct hy_pm_ctx {
     uint64_t energy_cap = 0;            // workgroups the partials hold
     double mass_unit = 1.0;  // fixed-point deposit unit (uniform mass or max explicit mass)
     int plane_group = 0;     // kz planes per y/x/y^-1 launch group (0 = all planes at once)
+    uint32_t gather_rows = HY_GATHER_ROWS_DEFAULT;  // lattice rows per gather column tile (0: particle order)
     // Bucket keys/ranks/counts computed by the previous fused step for its output
     // positions; reused when the next step's input is that output.
     bool keys_ready = false;
@@ -614,13 +615,15 @@ void require_pure_pm(const hy_pm_ctx* c, const char* what) {
 // Lattice block schedule for n = S^3 particles when a 256-particle block
 // tiles z-rows exactly (S a multiple of 256); about 64K particles (the
 // blocks resident on one XCD) per column tile.  Identity otherwise.
-hy::LatticeTiling lattice_tiling(uint64_t n, uint32_t block) {
+// rows_pref: lattice rows per column tile (HY_GATHER_ROWS_DEFAULT: 256 / ZC,
+// about 256 blocks per tile; 0: particle order).
+hy::LatticeTiling lattice_tiling(uint64_t n, uint32_t block, uint32_t rows_pref = HY_GATHER_ROWS_DEFAULT) {
     hy::LatticeTiling t{0, 0, 0, 0, -1, -1, -1, -1};
-    if (std::getenv("HY_NO_LATTICE_SCHEDULE")) return t;
+    if (rows_pref == 0) return t;
     const uint64_t S = (uint64_t)std::llround(std::cbrt((double)n));
     if (S == 0 || S * S * S != n || S % block != 0 || S > 2048) return t;
     const uint32_t ZC = (uint32_t)(S / block);
-    uint32_t rows = std::max<uint32_t>(1, 256 / ZC);  // TX*TY rows per tile
+    uint32_t rows = rows_pref == HY_GATHER_ROWS_DEFAULT ? std::max<uint32_t>(1, 256 / ZC) : rows_pref;  // TX*TY
     uint32_t TX = 1, TY = 1;
     while (TX * TY < rows) {
         if (TY <= TX && S % (TY * 2) == 0) TY *= 2;
@@ -641,7 +644,7 @@ hy::LatticeTiling lattice_tiling(uint64_t n, uint32_t block) {
 
 template <typename T, typename PT>
 void launch_gather(hy_pm_ctx* c, hy::GatherArgs<T, PT> a) {
-    a.lat = c->nbox == 1 ? lattice_tiling(a.n, 256) : hy::LatticeTiling{0, 0, 0, 0, -1, -1, -1, -1};
+    a.lat = c->nbox == 1 ? lattice_tiling(a.n, 256, c->gather_rows) : hy::LatticeTiling{0, 0, 0, 0, -1, -1, -1, -1};
     a.dfac = c->dfac;
     const bool o32 = c->nbox == 1 && (uint64_t)c->N * c->N * c->N * sizeof(T) <= (1ull << 32);
     const dim3 grid((uint32_t)blocks_for(a.n, 256)), block(256);
@@ -1612,6 +1615,85 @@ int hy_pm_autotune(hy_pm_ctx* c, uint32_t reps, hy_fft_tuning* out) {
         if (c->prec == HY_PREC_FP64) autotune(c, c->fft64, reps, &t);
         else autotune(c, c->fft32, reps, &t);
         if (out) *out = t;
+    });
+}
+
+int hy_pm_set_gather_rows(hy_pm_ctx* c, uint32_t rows) {
+    return guarded([&] {
+        if (!c) fail(HY_EINVAL, "null context");
+        if (rows != HY_GATHER_ROWS_DEFAULT && rows != 0 && (rows & (rows - 1)) != 0)
+            fail(HY_EINVAL, "gather rows per tile must be 0, a power of two or HY_GATHER_ROWS_DEFAULT");
+        c->gather_rows = rows;
+    });
+}
+
+int hy_pm_autotune_gather(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z,
+                          double uniform, uint32_t reps, uint32_t* rows_out, float* ms_out) {
+    return guarded([&] {
+        if (!c || !x || !y || !z) fail(HY_EINVAL, "hy_pm_autotune_gather: null argument");
+        if (reps == 0 || reps > 1000) fail(HY_EINVAL, "autotune repetitions must be 1..1000");
+        require_single_box(c, "hy_pm_autotune_gather");
+        require_pure_pm(c, "hy_pm_autotune_gather");
+        set_device(c);
+        require_particles(c, n);
+        set_boxes(c, n);
+        c->keys_ready = c->rec_ready = false;
+        const double M = input_mass(c, n, nullptr, uniform, false);
+        reset_status(c);
+        run_deposit(c, n, x, y, z, nullptr, uniform, M, true);  // the potential of these positions
+        run_fft(c);
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        auto gather = [&]() {
+            auto fill = [&](auto& a) {
+                a.g = c->geom; a.n = n;
+                a.x_in = x; a.y_in = y; a.z_in = z;
+                a.err = c->err;
+            };
+            if (c->prec == HY_PREC_FP64) {
+                hy::GatherArgs<double, double> a{};
+                fill(a);
+                a.phi = static_cast<const double*>(c->R);
+                a.g_x = (double*)c->gx; a.g_y = (double*)c->gy; a.g_z = (double*)c->gz;
+                launch_gather(c, a);
+            } else {
+                hy::GatherArgs<float, float> a{};
+                fill(a);
+                a.phi = static_cast<const float*>(c->R);
+                a.g_x = (float*)c->gx; a.g_y = (float*)c->gy; a.g_z = (float*)c->gz;
+                launch_gather(c, a);
+            }
+        };
+        static const uint32_t cand[HY_GATHER_TUNE_CANDIDATES] = {0, 16, 64, 256, 1024};
+        float best_ms = 0.0f;
+        uint32_t best = 0;
+        std::string err;
+        for (int i = 0; i < HY_GATHER_TUNE_CANDIDATES; ++i) {
+            c->gather_rows = cand[i];
+            const hy::LatticeTiling t = lattice_tiling(n, 256, cand[i]);
+            float ms = 0.0f;
+            if (cand[i] == 0 || t.S != 0) {
+                gather();  // warm
+                hip_check(hipEventRecord(e0, c->stream), "hipEventRecord");
+                for (uint32_t r = 0; r < reps; ++r) gather();
+                hip_check(hipEventRecord(e1, c->stream), "hipEventRecord");
+                hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                ms /= (float)reps;
+                if (best_ms == 0.0f || ms < best_ms) {
+                    best_ms = ms;
+                    best = cand[i];
+                }
+            }
+            if (ms_out) ms_out[i] = ms;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        c->gather_rows = best;
+        if (rows_out) *rows_out = best;
+        const auto st = fetch_status(c);
+        raise_errors(st.err);
     });
 }
 

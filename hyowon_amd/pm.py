@@ -237,6 +237,22 @@ class PMSolver:
         check(lib().hy_pm_autotune(self._h, int(reps), C.byref(t)))
         return _tuning_dict(t)
 
+    def autotune_gather(self, x, y, z, uniform_mass: float, reps: int = 3) -> dict:
+        """Times the gather of the potential of these positions under each
+        lattice column-tile size (lattice rows per tile; 0 = particle order)
+        and keeps the fastest (results are unchanged)."""
+        from ._lib import GATHER_TUNE_ROWS, HY_GATHER_TUNE_CANDIDATES
+        rows = C.c_uint32(0)
+        ms = (C.c_float * HY_GATHER_TUNE_CANDIDATES)()
+        check(lib().hy_pm_autotune_gather(self._h, len(x), _ptr(x), _ptr(y), _ptr(z), float(uniform_mass),
+                                          int(reps), C.byref(rows), ms))
+        return {"rows": int(rows.value), "ms": {r: float(v) for r, v in zip(GATHER_TUNE_ROWS, ms)}}
+
+    def set_gather_rows(self, rows: Optional[int]) -> None:
+        """Lattice rows per gather column tile (0 = particle order, None = default)."""
+        from ._lib import HY_GATHER_ROWS_DEFAULT
+        check(lib().hy_pm_set_gather_rows(self._h, HY_GATHER_ROWS_DEFAULT if rows is None else int(rows)))
+
     @property
     def mesh_dtype(self):
         return np.float64 if self.precision == HY_PREC_FP64 else np.float32

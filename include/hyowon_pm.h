@@ -254,6 +254,25 @@ int hy_pm_get_tuning(hy_pm_ctx* ctx, hy_fft_tuning* out);
 int hy_pm_set_tuning(hy_pm_ctx* ctx, const hy_fft_tuning* in);
 int hy_pm_autotune(hy_pm_ctx* ctx, uint32_t reps, hy_fft_tuning* out);
 
+/* Gather schedule.  The fused gather walks particle arrays held in Lagrangian
+ * lattice order (n = S^3, S a multiple of 256) in column tiles of TX x TY
+ * lattice rows, so the blocks resident on one XCD share that XCD's L2.
+ * hy_pm_autotune_gather deposits and solves for the given positions (single
+ * box, pure PM, uniform mass), times the gather under each candidate number
+ * of rows per tile (0 = particle order, 16, 64, 256, 1024; ms_out[5] receives
+ * the times, 0 where the lattice does not admit one) and keeps the fastest in
+ * the context; hy_pm_set_gather_rows sets it (0, a power of two, or
+ * HY_GATHER_ROWS_DEFAULT = 256 / (S / 256)).  Every schedule gives identical
+ * results.  No reference counterpart (the reference gathers on CPU threads);
+ * the FFT launch shapes are tuned by hy_pm_autotune, as FFTW_MEASURE plans do
+ * for the reference (src/mesh/fft_backend.cpp:967-1003). */
+#define HY_GATHER_ROWS_DEFAULT 0xffffffffu
+#define HY_GATHER_TUNE_CANDIDATES 5
+int hy_pm_autotune_gather(hy_pm_ctx* ctx, uint64_t n, const double* x, const double* y,
+                          const double* z, double uniform_mass, uint32_t reps,
+                          uint32_t* rows_out, float* ms_out);
+int hy_pm_set_gather_rows(hy_pm_ctx* ctx, uint32_t rows);
+
 /* Adjoint (vector-Jacobian products) for differentiable runs.  No reference
  * counterpart: the reference is not differentiable.  Device pointers, single-box
  * contexts, uniform mass; adjoint arrays are binary64.

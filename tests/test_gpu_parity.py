@@ -359,9 +359,11 @@ def test_fft_tuning_rejects_uncompiled_variant(gpu):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
-def test_lattice_block_schedule_bit_identical(gpu, prec, monkeypatch):
+def test_lattice_block_schedule_bit_identical(gpu, prec):
     """256^3 particles in lattice order take the Lagrangian-tiled gather
-    schedule; results equal the identity schedule bit for bit."""
+    schedule; results equal, bit for bit, under every tile size the gather
+    autotuner can choose (and the particle-order schedule), and the
+    autotuner's choice is one of them."""
     import hyowon_amd as H
     S, N, L = 256, 64, 64.0
     n = S ** 3
@@ -372,9 +374,14 @@ def test_lattice_block_schedule_bit_identical(gpu, prec, monkeypatch):
     pos = np.mod(pos, L)
     mom = rng.normal(0, 5.0, (3, n))
 
-    def run():
+    def run(rows, tune=False):
         s = _solver(L, N, prec, n)
         a = H.ParticleState.from_numpy(pos, mom, precision=prec)
+        if tune:
+            t = s.autotune_gather(a.x, a.y, a.z, 1.0, reps=1)
+            assert t["rows"] in t["ms"] and t["ms"][t["rows"]] > 0.0
+        else:
+            s.set_gather_rows(rows)
         b = a.empty_like()
         for i in range(2):
             s.kdk(a, b, 1.0, 0.01 if i else None, 0.01, 0.02, reuse_keys=True)
@@ -383,11 +390,10 @@ def test_lattice_block_schedule_bit_identical(gpu, prec, monkeypatch):
         s.close()
         return out
 
-    tiled = run()
-    monkeypatch.setenv("HY_NO_LATTICE_SCHEDULE", "1")
-    plain = run()
-    for u, v in zip(tiled, plain):
-        assert np.array_equal(u, v)
+    tiled = run(None)
+    for other in [run(0), run(16), run(1024), run(None, tune=True)]:
+        for u, v in zip(tiled, other):
+            assert np.array_equal(u, v)
 
 
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])

@@ -142,9 +142,8 @@ struct hy_pm_ctx {
     uint32_t* grand_total = nullptr;
     uint32_t* xcounts = nullptr;   // x-plane bins (N)
     uint32_t* xoffsets = nullptr;  // N+1
-    double* tile_mass = nullptr;
+    unsigned long long* mass_digits = nullptr;  // per box: exact deposited-mass digit counters
     double* mass_total = nullptr;
-    double* mass_partials = nullptr;  // REDUCE_BLOCKS partial tile-mass sums
     uint64_t* scratch = nullptr;
     uint64_t scratch_cap = 0;
     unsigned long long* scratch_cursor = nullptr;
@@ -426,7 +425,7 @@ DepositPlan plan_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double
         P.norm_exp = e - 64;
     }
     P.direct = (exact_mean && std::isfinite(inv_volume) && inv_volume > 0.0) ? 1 : 0;
-    P.tile_mass = c->tile_mass;
+    P.mass_digits = c->mass_digits;
     P.scratch = c->scratch;
     P.scratch_cap = c->scratch_cap;
     P.scratch_cursor = c->scratch_cursor;
@@ -454,6 +453,7 @@ long double mass_tol_factor(const hy_pm_ctx* c, uint64_t n) {
 // carried in a capacity layout, whose overflow prefix is ovf).
 void launch_deposit(hy_pm_ctx* c, const hy::DepParams& P, bool status_check = true,
                     const uint32_t* start = nullptr, const uint32_t* ovf = nullptr) {
+    fill_u32(c, c->mass_digits, (uint64_t)hy::SA_WORDS * c->nbox * 2);
     if (c->prec == HY_PREC_FP64)
         hipLaunchKernelGGL((c->geom.T == 16 ? hy::k_deposit<double, 16> : hy::k_deposit<double, 0>),
                            dim3(c->ntiles * c->nbox), dim3(hy::DEP_THREADS), 0, c->stream, P,
@@ -467,11 +467,10 @@ void launch_deposit(hy_pm_ctx* c, const hy::DepParams& P, bool status_check = tr
                            static_cast<float*>(c->R));
     }
     launch_check("k_deposit");
-    hipLaunchKernelGGL(hy::k_reduce_partials, dim3(hy::REDUCE_BLOCKS, c->nbox), dim3(256), 0, c->stream,
-                       c->tile_mass, c->ntiles, c->mass_partials);
-    hipLaunchKernelGGL(hy::k_reduce_sum, dim3(c->nbox), dim3(256), 0, c->stream, c->mass_partials,
-                       (uint64_t)hy::REDUCE_BLOCKS, c->mass_total);
-    launch_check("k_reduce_sum");
+    // the exact sum of every cell's deposited mass (ExactPositiveDoubleSum), per box
+    hipLaunchKernelGGL(hy::k_mass_exact, dim3((uint32_t)blocks_for(c->nbox, 64)), dim3(64), 0, c->stream,
+                       c->mass_digits, c->nbox, c->mass_total);
+    launch_check("k_mass_exact");
     if (status_check) {
         const double tol = static_cast<double>(mass_tol_factor(c, P.n / c->nbox));
         hipLaunchKernelGGL(hy::k_deposit_status, dim3((uint32_t)blocks_for(c->nbox, 256)), dim3(256), 0, c->stream,
@@ -1012,9 +1011,8 @@ int hy_pm_create(const hy_pm_desc* desc, hy_pm_ctx** out) {
         c->grand_total = (uint32_t*)c->alloc(16);
         c->xcounts = (uint32_t*)c->alloc((size_t)N * 4);
         c->xoffsets = (uint32_t*)c->alloc(((size_t)N + 1) * 4);
-        c->tile_mass = (double*)c->alloc(c->ntiles * c->nbox * 8);
+        c->mass_digits = (unsigned long long*)c->alloc((uint64_t)hy::SA_WORDS * c->nbox * 8);
         c->mass_total = (double*)c->alloc(std::max<uint64_t>(2, c->nbox) * 8);
-        c->mass_partials = (double*)c->alloc((uint64_t)hy::REDUCE_BLOCKS * c->nbox * sizeof(double));
         c->scratch_cap = 2 * cap + (1u << 16);
         c->scratch = (uint64_t*)c->alloc(c->scratch_cap * 8);
         c->scratch_cursor = (unsigned long long*)c->alloc(16);

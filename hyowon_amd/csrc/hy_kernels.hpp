@@ -503,6 +503,68 @@ struct DepCfg<float> { static constexpr int CAP = 2048; };
 template <>
 struct DepCfg<double> { static constexpr int CAP = 1024; };
 
+// ---------------------------------------------------------------------------
+// Exact deposited mass: the reference sums every cell's mass with
+// ExactPositiveDoubleSum (src/gravity/pm_solver.cpp:132-180).  Here each box
+// has SA_DIGITS counters, one per 32-bit digit of the binary positions
+// 2^-1074 .. 2^1230 (every binary64 bit), each holding its digit's sum with
+// the carries in its upper bits, plus a word flagging an input that is not a
+// finite non-negative number.  The counters are integers, so the total is
+// exact whatever the order of the additions; k_mass_exact rounds it once to
+// nearest (ties to even), as the host's exact_sum does.
+// ---------------------------------------------------------------------------
+constexpr int SA_DIGITS = 72;
+constexpr int SA_WORDS = SA_DIGITS + 1;
+
+// d += (v[0] + v[1] 2^32 + ... ) * 2^shift * 2^-1074, v: ND 32-bit digits.
+template <int ND>
+__device__ __forceinline__ void sa_add_digits(unsigned long long* d, const uint32_t* v, int shift) {
+    const int limb = shift >> 5, off = shift & 31;
+#pragma unroll
+    for (int k = 0; k <= ND; ++k) {
+        const uint64_t lo = k < ND ? (uint64_t)v[k] << off : 0ull;
+        const uint64_t hi = (k > 0 && off) ? (uint64_t)v[k - 1] >> (32 - off) : 0ull;
+        const uint32_t digit = (uint32_t)((lo | hi) & 0xffffffffull);
+        if (digit && limb + k < SA_DIGITS) atomicAdd(&d[limb + k], (unsigned long long)digit);
+    }
+}
+
+// d += x (a cell mass); anything but a finite non-negative x marks the sum invalid.
+__device__ __forceinline__ void sa_add_double(unsigned long long* d, double x) {
+    if (x == 0.0) return;
+    if (!(x > 0.0) || !isfinite(x)) {
+        atomicOr(&d[SA_DIGITS], 1ull);
+        return;
+    }
+    const uint64_t bits = (uint64_t)__double_as_longlong(x);
+    const int e = (int)((bits >> 52) & 0x7ff);
+    const uint64_t frac = bits & ((1ull << 52) - 1ull);
+    const uint64_t mant = e ? (frac | (1ull << 52)) : frac;
+    const uint32_t v[2] = {(uint32_t)mant, (uint32_t)(mant >> 32)};
+    sa_add_digits<2>(d, v, e ? e - 1 : 0);
+}
+
+// d += Q * unit * 2^-32 exactly, Q = (q1 2^64 + q0) the exact integer sum of a
+// tile's 32.32 fixed-point cells (q1 < 2^32), unit > 0 a normal binary64.
+__device__ __forceinline__ void sa_add_fixed(unsigned long long* d, uint64_t q0, uint64_t q1, double unit) {
+    const uint64_t bits = (uint64_t)__double_as_longlong(unit);
+    const int e = (int)((bits >> 52) & 0x7ff);
+    const int shift = e - 1 - 32;
+    if (e == 0 || e == 0x7ff || shift < 0) {
+        atomicOr(&d[SA_DIGITS], 1ull);
+        return;
+    }
+    const uint64_t m = (bits & ((1ull << 52) - 1ull)) | (1ull << 52);
+    // P = Q * m as three 64-bit words
+    const uint64_t a_lo = q0 * m, a_hi = __umul64hi(q0, m);
+    const uint64_t b_lo = q1 * m, b_hi = __umul64hi(q1, m);
+    const uint64_t p1 = a_hi + b_lo;
+    const uint64_t p2 = b_hi + (p1 < a_hi ? 1ull : 0ull);
+    const uint32_t v[6] = {(uint32_t)a_lo, (uint32_t)(a_lo >> 32), (uint32_t)p1, (uint32_t)(p1 >> 32),
+                           (uint32_t)p2, (uint32_t)(p2 >> 32)};
+    sa_add_digits<6>(d, v, shift);
+}
+
 struct DepParams {
     Geom g;
     const double* x;
@@ -524,7 +586,7 @@ struct DepParams {
     int norm_exp;             // ... times 2^norm_exp (the reference holds it in long double)
     int direct;               // 1: (m - mean) * inv_volume ; 0: fma-centred path
     int raw;                  // 1: write the deposited sum itself (adjoint deposits of signed values)
-    double* tile_mass;        // per-tile deposited mass (binary64)
+    unsigned long long* mass_digits;  // per box SA_WORDS counters of the exact deposited mass
     uint64_t* scratch;        // global fallback keys
     uint64_t scratch_cap;
     unsigned long long* scratch_cursor;
@@ -590,7 +652,7 @@ template <typename T, bool LDS_PAYLOAD, int TE, typename KP, typename FP>
 __device__ __forceinline__ void deposit_cells(const DepParams& P, T* __restrict__ mesh, int tx,
                                               int ty, int tz, KP keys, uint32_t M,
                                               const uint32_t* start, FP fx, FP fy, FP fz, FP fm,
-                                              double& mass_acc) {
+                                              unsigned long long* sa) {
     const Geom& g = P.g;
     const int TT = TE > 0 ? TE : g.T;
     const int Te = TT + 1;
@@ -642,7 +704,7 @@ __device__ __forceinline__ void deposit_cells(const DepParams& P, T* __restrict_
                 sum += mm * xw * wy * wz;
             }
         }
-        mass_acc += (double)sum;
+        if (!P.raw) sa_add_double(sa, (double)sum);
         const uint64_t gx = (uint64_t)(tx * g.T + lx), gy = (uint64_t)(ty * g.T + ly),
                        gz = (uint64_t)(tz * g.T + lz);
         mesh[(gx * g.N + gy) * g.N + gz] = contrast_of(sum, P);
@@ -749,7 +811,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restr
     __shared__ uint32_t sh_rs[DEP_NRANGES], sh_rc[DEP_NRANGES + 1];
     __shared__ uint8_t sh_rd[DEP_NRANGES];
     __shared__ uint64_t sh_gbase;
-    __shared__ double sh_red[DEP_THREADS / 64];
+    __shared__ unsigned long long sh_sa[SA_WORDS];  // the tile's exact cell-mass digits
     __shared__ uint32_t sh_part[DEP_THREADS / 64];
 
     const Geom& g = P.g;
@@ -763,6 +825,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restr
     const int Te = TT + 1;
     const int ncell = Te * Te * Te;
     mesh += (uint64_t)box * g.N * g.N * g.N;
+    for (int w = threadIdx.x; w < SA_WORDS; w += blockDim.x) sh_sa[w] = 0ull;  // ordered by the range barrier
 
     __shared__ uint8_t sh_rcls[DEP_NRANGES];
     load_source_ranges(P.offsets + (uint64_t)box * buckets_per_box(g), tx, ty, tz, nT, sh_rs, sh_rc, sh_rd,
@@ -834,7 +897,6 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restr
     __threadfence_block();
     __syncthreads();
 
-    double mass_acc = 0.0;
     if (use_lds) {
         // counting sort by cell, then each cell's few records by particle index
         wg_exclusive_scan_lds(sh_start, ncell + 1, sh_part);
@@ -858,24 +920,18 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restr
         }
         __syncthreads();
         deposit_cells<T, true, TE>(P, mesh, tx, ty, tz, sh_keys, M, sh_start, sh_fx, sh_fy, sh_fz,
-                               sh_m, mass_acc);
+                               sh_m, sh_sa);
     } else {
         wg_sort_u64(gkeys, M);
         build_cell_starts(gkeys, M, sh_start, ncell);
         deposit_cells<T, false, TE>(P, mesh, tx, ty, tz, gkeys, M, sh_start, sh_fx, sh_fy, sh_fz,
-                                sh_m, mass_acc);
+                                sh_m, sh_sa);
     }
 
-    // Deterministic per-tile mass: a fixed shuffle tree per wavefront, then
-    // the wavefront sums in order.
-    for (int off = 32; off > 0; off >>= 1) mass_acc += __shfl_down(mass_acc, off);
-    if (lane_id() == 0) sh_red[threadIdx.x / 64] = mass_acc;
+    // the tile's exact mass digits into its box's counters
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = 0.0;
-        for (int w = 0; w < DEP_THREADS / 64; ++w) m += sh_red[w];
-        P.tile_mass[gtile] = m;
-    }
+    for (int w = threadIdx.x; w < SA_WORDS; w += blockDim.x)
+        if (sh_sa[w]) atomicAdd(&P.mass_digits[(uint64_t)box * SA_WORDS + w], sh_sa[w]);
 }
 
 template __global__ void k_deposit<double, 0>(DepParams, double*);
@@ -993,7 +1049,7 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
     __shared__ uint32_t sh_rs[DEPFX_NRANGES], sh_rc[DEPFX_NRANGES + 1];
     __shared__ uint8_t sh_rd[DEPFX_NRANGES];
     __shared__ uint32_t sh_nr;
-    __shared__ double sh_red[DEPFX_THREADS / 64];
+    __shared__ uint64_t sh_red[2 * (DEPFX_THREADS / 64)];
 
     const Geom& g = P.g;
     const int nT = g.nT;
@@ -1075,7 +1131,8 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
     if (bad) atomicOr(P.err, ERR_COORDINATE);
     __syncthreads();
 
-    double mass_acc = 0.0;
+    // the tile's exact mass: its cells' 32.32 integers summed as two 32-bit digit sums
+    uint64_t dig0 = 0, dig1 = 0;
     if constexpr (TE > 0 && TE % 4 == 0) {
         // four z-adjacent cells per lane, one 16-byte store (a tile's z-runs
         // start at multiples of TE cells)
@@ -1087,7 +1144,8 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const long long q = (long long)acc[c + j];
-                mass_acc += (double)q;
+                dig0 += (uint64_t)q & 0xffffffffull;
+                dig1 += (uint64_t)q >> 32;
                 o[j] = contrast_fp32((double)q * Q.fixed_to_mass, P);
             }
             const uint64_t gx = (uint64_t)(tx * T + lx), gy = (uint64_t)(ty * T + ly), gz = (uint64_t)(tz * T + lz);
@@ -1096,21 +1154,32 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
     } else {
         for (int c = threadIdx.x; c < cells; c += blockDim.x) {
             const long long q = (long long)acc[c];
-            mass_acc += (double)q;
+            dig0 += (uint64_t)q & 0xffffffffull;
+            dig1 += (uint64_t)q >> 32;
             const double mcell = (double)q * Q.fixed_to_mass;
             const int lz = c % T, ly = (c / T) % T, lx = c / (T * T);
             const uint64_t gx = (uint64_t)(tx * T + lx), gy = (uint64_t)(ty * T + ly), gz = (uint64_t)(tz * T + lz);
             mesh[(gx * g.N + gy) * g.N + gz] = contrast_fp32(mcell, P);
         }
     }
-    // deterministic per-tile mass in units of the fixed-point quantum
-    for (int off = 32; off > 0; off >>= 1) mass_acc += __shfl_down(mass_acc, off);
-    if (lane_id() == 0) sh_red[threadIdx.x / 64] = mass_acc;
+    for (int off = 32; off > 0; off >>= 1) {
+        dig0 += __shfl_down(dig0, off);
+        dig1 += __shfl_down(dig1, off);
+    }
+    if (lane_id() == 0) {
+        sh_red[2 * (threadIdx.x / 64)] = dig0;
+        sh_red[2 * (threadIdx.x / 64) + 1] = dig1;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int w = 0; w < (int)(blockDim.x / 64); ++w) s += sh_red[w];
-        P.tile_mass[gtile] = s * Q.fixed_to_mass;
+    if (threadIdx.x == 0 && !P.raw) {
+        uint64_t t0 = 0, t1 = 0;  // each below 2^44 (4096 cells of 32-bit digits)
+        for (int w = 0; w < (int)(blockDim.x / 64); ++w) {
+            t0 += sh_red[2 * w];
+            t1 += sh_red[2 * w + 1];
+        }
+        const uint64_t q0 = t0 + (t1 << 32);
+        const uint64_t q1 = (t1 >> 32) + (q0 < t0 ? 1ull : 0ull);
+        sa_add_fixed(P.mass_digits + (uint64_t)box * SA_WORDS, q0, q1, Q.mass_unit);
     }
 }
 
@@ -1130,42 +1199,49 @@ __global__ void k_set_factors(double* __restrict__ f, double k_prev, double k_ne
     }
 }
 
-// Fixed-order two-level reduction of per-tile masses: REDUCE_BLOCKS
-// workgroups sum fixed contiguous chunks into partials, then k_reduce_sum
-// adds the partials (the same order on every run).
-constexpr int REDUCE_BLOCKS = 256;
-__global__ void k_reduce_partials(const double* __restrict__ v, uint64_t n, double* __restrict__ partials) {
-    __shared__ double sh[256];
-    v += (uint64_t)blockIdx.y * n;  // one box per grid row
-    partials += (uint64_t)blockIdx.y * REDUCE_BLOCKS;
-    const uint64_t chunk = (n + REDUCE_BLOCKS - 1) / REDUCE_BLOCKS;
-    const uint64_t lo = (uint64_t)blockIdx.x * chunk;
-    const uint64_t hi = lo + chunk < n ? lo + chunk : n;
-    double s = 0.0;
-    for (uint64_t i = lo + threadIdx.x; i < hi; i += 256) s += v[i];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
-        __syncthreads();
+// Per box: the exact deposited mass from its digit counters, rounded once to
+// nearest with ties to even (NaN when a cell was not a finite non-negative
+// number).  One thread per box.
+__global__ void k_mass_exact(const unsigned long long* __restrict__ digits, uint32_t nbox,
+                             double* __restrict__ out) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    const unsigned long long* d = digits + (uint64_t)b * SA_WORDS;
+    if (d[SA_DIGITS]) {
+        out[b] = __builtin_nan("");
+        return;
     }
-    if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
-}
-
-// Fixed-order reduction (one workgroup per box).
-__global__ void k_reduce_sum(const double* __restrict__ v, uint64_t n, double* __restrict__ out) {
-    __shared__ double sh[256];
-    v += (uint64_t)blockIdx.x * n;
-    out += blockIdx.x;
-    double s = 0.0;
-    for (uint64_t i = threadIdx.x; i < n; i += 256) s += v[i];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
-        __syncthreads();
+    uint32_t v[SA_DIGITS + 2];
+    uint64_t carry = 0;
+    for (int i = 0; i < SA_DIGITS; ++i) {
+        const uint64_t s = d[i] + carry;  // d[i] < 2^63, carry < 2^32
+        v[i] = (uint32_t)s;
+        carry = s >> 32;
     }
-    if (threadIdx.x == 0) *out = sh[0];
+    v[SA_DIGITS] = (uint32_t)carry;
+    v[SA_DIGITS + 1] = (uint32_t)(carry >> 32);
+    int top = -1;
+    for (int i = SA_DIGITS + 1; i >= 0 && top < 0; --i)
+        if (v[i]) top = i * 32 + (31 - __clz(v[i]));
+    if (top < 0) {
+        out[b] = 0.0;
+        return;
+    }
+    auto bit = [&](int k) -> uint64_t { return k < 0 ? 0ull : (uint64_t)((v[k >> 5] >> (k & 31)) & 1u); };
+    if (top < 53) {
+        uint64_t m = 0;
+        for (int k = 0; k <= top; ++k) m |= bit(k) << k;
+        out[b] = ldexp((double)m, -1074);
+        return;
+    }
+    const int lo = top - 52;
+    uint64_t m = 0;
+    for (int k = 0; k < 53; ++k) m |= bit(lo + k) << k;
+    const bool round = bit(lo - 1) != 0;
+    bool sticky = false;
+    for (int k = 0; k < lo - 1 && !sticky; ++k) sticky = bit(k) != 0;
+    if (round && (sticky || (m & 1ull))) ++m;
+    out[b] = ldexp((double)m, lo - 1074);
 }
 
 // Deposit status on the device, so that calls which do not synchronise stay

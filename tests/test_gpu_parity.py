@@ -446,3 +446,55 @@ def test_potential_energy_bit_exact_on_a_given_potential(gpu, oracle_mod):
     e_ref = oracle_mod.potential_energy(phi, pos[0], pos[1], pos[2], L, uniform_mass=m)
     assert d.potential_energy_comoving == e_ref
     s.close()
+
+
+@pytest.mark.parametrize("explicit", [False, True])
+def test_deposited_mass_exact_fp64(gpu, oracle_mod, explicit):
+    """The deposited mass the mass-conservation check uses is the exact sum of
+    every cell's mass, rounded once (ExactPositiveDoubleSum,
+    src/gravity/pm_solver.cpp:132-180): equal to the oracle's exact sum of its
+    own (bit-identical) cell masses."""
+    import ctypes as C
+    N, L, n = 48, 48.0, 30000
+    pos = clumpy_particles(n, L, 41, n_clumps=20, sigma=0.4)
+    masses = np.random.default_rng(5).uniform(0.01, 3.0, n) if explicit else None
+    m = 0.37
+    s = _solver(L, N, "fp64", n)
+    s.density(*(_dev(p, gpu) for p in pos), _dev(masses, gpu) if explicit else None, m)
+    got = s.info().last_deposited_mass
+    mesh = oracle_mod.deposit(pos[0], pos[1], pos[2], L, N, masses=masses, uniform_mass=m)
+    flat = np.ascontiguousarray(mesh.reshape(-1))
+    ref = oracle_mod.lib().or_exact_sum(flat.ctypes.data_as(C.POINTER(C.c_double)), flat.size)
+    assert got == ref, (got, ref, got - ref)
+
+
+def test_deposited_mass_exact_fp32(gpu, oracle_mod):
+    """Binary32 contexts deposit 32.32 fixed-point cell masses; the deposited
+    mass is their exact sum times the mass unit, rounded once.  Restated here
+    from the particles: per corner q = round(((m wx) wy) wz 2^32) in binary32
+    with the reference's CIC fractions (oracle cic_coordinate), summed exactly
+    with Python integers, scaled and rounded with fractions.Fraction."""
+    from fractions import Fraction
+    N, L, n = 32, 32.0, 6000
+    pos = clumpy_particles(n, L, 42, n_clumps=10, sigma=0.5)
+    m = 0.7
+    s = _solver(L, N, "fp32", n)
+    s.density(*(_dev(p, gpu) for p in pos), None, m)
+    got = s.info().last_deposited_mass
+    fr = np.array([[oracle_mod.cic_coordinate(float(v), L, N)[1] for v in pos[a]] for a in range(3)])
+    f = fr.astype(np.float32)
+    one = np.float32(1.0)
+    Q = 0
+    for ox in range(2):
+        wx = (one - f[0]) if ox == 0 else f[0]
+        for oy in range(2):
+            wy = (one - f[1]) if oy == 0 else f[1]
+            mxy = (one * wx).astype(np.float32) * wy  # uniform mass: the records carry m = 1 in mass units
+            mxy = mxy.astype(np.float32)
+            for oz in range(2):
+                wz = (one - f[2]) if oz == 0 else f[2]
+                w = (mxy * wz).astype(np.float32)
+                scaled = (w * np.float32(4294967296.0)).astype(np.float32)
+                Q += int(np.rint(scaled.astype(np.float64)).astype(np.int64).sum())
+    exact = Fraction(Q) * Fraction(m) / (1 << 32)
+    assert got == float(exact), (got, float(exact))

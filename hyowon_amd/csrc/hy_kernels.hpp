@@ -515,22 +515,26 @@ struct DepCfg<double> { static constexpr int CAP = 1024; };
 // ---------------------------------------------------------------------------
 constexpr int SA_DIGITS = 72;
 constexpr int SA_WORDS = SA_DIGITS + 1;
+// Each wave of a deposit workgroup stores its sum as a window of SA_WAVE_WIN
+// counters from a base digit (SA_WAVE_WORDS words: base, window; base -1:
+// empty); digits outside the window go to the box's counters directly.
+// k_mass_reduce adds the windows up per box.
+constexpr int SA_WAVE_WIN = 8;
+constexpr int SA_WAVE_WORDS = SA_WAVE_WIN + 1;
+constexpr int SA_WAVES_PER_TILE = 8;  // deposit workgroups of 512 threads
 
-// d += (v[0] + v[1] 2^32 + ... ) * 2^shift * 2^-1074, v: ND 32-bit digits.
-template <int ND>
-__device__ __forceinline__ void sa_add_digits(unsigned long long* d, const uint32_t* v, int shift) {
-    const int limb = shift >> 5, off = shift & 31;
-#pragma unroll
-    for (int k = 0; k <= ND; ++k) {
-        const uint64_t lo = k < ND ? (uint64_t)v[k] << off : 0ull;
-        const uint64_t hi = (k > 0 && off) ? (uint64_t)v[k - 1] >> (32 - off) : 0ull;
-        const uint32_t digit = (uint32_t)((lo | hi) & 0xffffffffull);
-        if (digit && limb + k < SA_DIGITS) atomicAdd(&d[limb + k], (unsigned long long)digit);
-    }
-}
+// A lane's running sum of cell masses: a 192-bit integer (a2:a1:a0) in
+// units of the binary position 32*base (base < 0: empty), base one digit
+// below the first mass it took.  A mass m 2^s (s its bit position, 53-bit m)
+// is added exactly when 0 <= s - 32 base <= 127; others go to the counters d
+// directly.  sa_acc_flush reduces the wave's sums with shuffles and adds them
+// to d from one lane.
+struct SaAcc {
+    int base = -1;
+    uint64_t a0 = 0, a1 = 0, a2 = 0;
+};
 
-// d += x (a cell mass); anything but a finite non-negative x marks the sum invalid.
-__device__ __forceinline__ void sa_add_double(unsigned long long* d, double x) {
+__device__ __forceinline__ void sa_acc_add(SaAcc& a, unsigned long long* d, double x) {
     if (x == 0.0) return;
     if (!(x > 0.0) || !isfinite(x)) {
         atomicOr(&d[SA_DIGITS], 1ull);
@@ -540,18 +544,83 @@ __device__ __forceinline__ void sa_add_double(unsigned long long* d, double x) {
     const int e = (int)((bits >> 52) & 0x7ff);
     const uint64_t frac = bits & ((1ull << 52) - 1ull);
     const uint64_t mant = e ? (frac | (1ull << 52)) : frac;
-    const uint32_t v[2] = {(uint32_t)mant, (uint32_t)(mant >> 32)};
-    sa_add_digits<2>(d, v, e ? e - 1 : 0);
+    const int shift = e ? e - 1 : 0;
+    if (a.base < 0) a.base = (shift >> 5) > 0 ? (shift >> 5) - 1 : 0;
+    const int s = shift - 32 * a.base;
+    if (s >= 0 && s <= 127) {
+        const int off = s & 63;
+        const uint64_t lo = mant << off;
+        const uint64_t hi = off ? mant >> (64 - off) : 0ull;
+        if (s < 64) {
+            a.a0 += lo;
+            const uint64_t c0 = a.a0 < lo ? 1ull : 0ull;
+            const uint64_t t = a.a1 + hi;
+            const uint64_t c1 = t < hi ? 1ull : 0ull;
+            a.a1 = t + c0;
+            a.a2 += c1 + (a.a1 < c0 ? 1ull : 0ull);
+        } else {
+            a.a1 += lo;
+            a.a2 += hi + (a.a1 < lo ? 1ull : 0ull);
+        }
+    } else {
+        const int limb = shift >> 5, o = shift & 31;
+        const uint64_t l = mant << o, h = o ? mant >> (64 - o) : 0ull;
+        if ((uint32_t)l) atomicAdd(&d[limb], (unsigned long long)(uint32_t)l);
+        if ((uint32_t)(l >> 32) && limb + 1 < SA_DIGITS) atomicAdd(&d[limb + 1], (unsigned long long)(uint32_t)(l >> 32));
+        if (h && limb + 2 < SA_DIGITS) atomicAdd(&d[limb + 2], (unsigned long long)h);
+    }
 }
 
-// d += Q * unit * 2^-32 exactly, Q = (q1 2^64 + q0) the exact integer sum of a
-// tile's 32.32 fixed-point cells (q1 < 2^32), unit > 0 a normal binary64.
-__device__ __forceinline__ void sa_add_fixed(unsigned long long* d, uint64_t q0, uint64_t q1, double unit) {
+// Every lane of the wave calls this (the shuffles need all lanes): the wave's
+// window (base, SA_WAVE_WIN digits) goes to out from lane 0; lanes whose sums
+// start more than 2 digits above the wave's lowest add theirs to d directly.
+__device__ __forceinline__ void sa_acc_flush(const SaAcc& a, unsigned long long* d, unsigned long long* out) {
+    int b0 = a.base < 0 ? 0x7fffffff : a.base;
+    for (int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_xor(b0, off);
+        b0 = o < b0 ? o : b0;
+    }
+    if (b0 == 0x7fffffff) {  // no lane holds a mass
+        if (lane_id() == 0) out[0] = ~0ull;
+        return;
+    }
+    const uint32_t w[6] = {(uint32_t)a.a0, (uint32_t)(a.a0 >> 32), (uint32_t)a.a1, (uint32_t)(a.a1 >> 32),
+                           (uint32_t)a.a2, (uint32_t)(a.a2 >> 32)};
+    // lanes whose base is within 2 digits of the wave's lowest share an
+    // 8-digit window; the others add their own digits directly
+    const int k = a.base - b0;
+    const bool shared = a.base >= 0 && k <= 2;
+    if (a.base >= 0 && !shared) {
+        for (int j = 0; j < 6; ++j)
+            if (w[j] && a.base + j < SA_DIGITS) atomicAdd(&d[a.base + j], (unsigned long long)w[j]);
+    }
+    static_assert(SA_WAVE_WIN == 8, "window of the 6-digit lane sums at offsets 0..2");
+    if (lane_id() == 0) out[0] = (unsigned long long)b0;
+#pragma unroll
+    for (int j = 0; j < SA_WAVE_WIN; ++j) {
+        uint64_t v = 0;
+        if (shared) {
+#pragma unroll
+            for (int t = 0; t < 6; ++t)
+                if (t + k == j) v = w[t];
+        }
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane_id() == 0) out[1 + j] = b0 + j < SA_DIGITS ? v : 0ull;
+    }
+}
+
+// The window (out: base, SA_WAVE_WIN digits) of Q * unit * 2^-32 exactly,
+// Q = (q1 2^64 + q0) the exact integer sum of a wave's 32.32 fixed-point
+// cells (q1 < 2^32), unit > 0 a normal binary64 (else the sum is flagged
+// invalid in d).
+__device__ __forceinline__ void sa_fixed_window(unsigned long long* d, unsigned long long* out, uint64_t q0,
+                                                uint64_t q1, double unit) {
     const uint64_t bits = (uint64_t)__double_as_longlong(unit);
     const int e = (int)((bits >> 52) & 0x7ff);
     const int shift = e - 1 - 32;
     if (e == 0 || e == 0x7ff || shift < 0) {
         atomicOr(&d[SA_DIGITS], 1ull);
+        out[0] = ~0ull;
         return;
     }
     const uint64_t m = (bits & ((1ull << 52) - 1ull)) | (1ull << 52);
@@ -562,7 +631,15 @@ __device__ __forceinline__ void sa_add_fixed(unsigned long long* d, uint64_t q0,
     const uint64_t p2 = b_hi + (p1 < a_hi ? 1ull : 0ull);
     const uint32_t v[6] = {(uint32_t)a_lo, (uint32_t)(a_lo >> 32), (uint32_t)p1, (uint32_t)(p1 >> 32),
                            (uint32_t)p2, (uint32_t)(p2 >> 32)};
-    sa_add_digits<6>(d, v, shift);
+    // the product shifted by `shift` bits: seven 32-bit digits from shift / 32
+    const int limb = shift >> 5, off = shift & 31;
+    out[0] = (unsigned long long)limb;
+#pragma unroll
+    for (int k = 0; k < SA_WAVE_WIN; ++k) {
+        const uint64_t lo = k < 6 ? (uint64_t)v[k] << off : 0ull;
+        const uint64_t hi = (k > 0 && k <= 6 && off) ? (uint64_t)v[k - 1] >> (32 - off) : 0ull;
+        out[1 + k] = limb + k < SA_DIGITS ? ((lo | hi) & 0xffffffffull) : 0ull;
+    }
 }
 
 struct DepParams {
@@ -587,6 +664,7 @@ struct DepParams {
     int direct;               // 1: (m - mean) * inv_volume ; 0: fma-centred path
     int raw;                  // 1: write the deposited sum itself (adjoint deposits of signed values)
     unsigned long long* mass_digits;  // per box SA_WORDS counters of the exact deposited mass
+    unsigned long long* mass_waves;   // per tile and wave SA_WAVE_WORDS: its sum's digit window
     uint64_t* scratch;        // global fallback keys
     uint64_t scratch_cap;
     unsigned long long* scratch_cursor;
@@ -652,11 +730,12 @@ template <typename T, bool LDS_PAYLOAD, int TE, typename KP, typename FP>
 __device__ __forceinline__ void deposit_cells(const DepParams& P, T* __restrict__ mesh, int tx,
                                               int ty, int tz, KP keys, uint32_t M,
                                               const uint32_t* start, FP fx, FP fy, FP fz, FP fm,
-                                              unsigned long long* sa) {
+                                              unsigned long long* sa, unsigned long long* wave_out) {
     const Geom& g = P.g;
     const int TT = TE > 0 ? TE : g.T;
     const int Te = TT + 1;
     const int cells = TT * TT * TT;
+    SaAcc win;
     for (int c = threadIdx.x; c < cells; c += blockDim.x) {
         const int lz = c % TT, ly = (c / TT) % TT, lx = c / (TT * TT);
         T sum = (T)0;
@@ -704,11 +783,12 @@ __device__ __forceinline__ void deposit_cells(const DepParams& P, T* __restrict_
                 sum += mm * xw * wy * wz;
             }
         }
-        if (!P.raw) sa_add_double(sa, (double)sum);
+        if (!P.raw) sa_acc_add(win, sa, (double)sum);
         const uint64_t gx = (uint64_t)(tx * g.T + lx), gy = (uint64_t)(ty * g.T + ly),
                        gz = (uint64_t)(tz * g.T + lz);
         mesh[(gx * g.N + gy) * g.N + gz] = contrast_of(sum, P);
     }
+    sa_acc_flush(win, sa, wave_out);
 }
 
 // Build start[0..Te^3] (prefix positions per ext cell) from sorted keys.
@@ -811,7 +891,6 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restr
     __shared__ uint32_t sh_rs[DEP_NRANGES], sh_rc[DEP_NRANGES + 1];
     __shared__ uint8_t sh_rd[DEP_NRANGES];
     __shared__ uint64_t sh_gbase;
-    __shared__ unsigned long long sh_sa[SA_WORDS];  // the tile's exact cell-mass digits
     __shared__ uint32_t sh_part[DEP_THREADS / 64];
 
     const Geom& g = P.g;
@@ -825,7 +904,11 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restr
     const int Te = TT + 1;
     const int ncell = Te * Te * Te;
     mesh += (uint64_t)box * g.N * g.N * g.N;
-    for (int w = threadIdx.x; w < SA_WORDS; w += blockDim.x) sh_sa[w] = 0ull;  // ordered by the range barrier
+    static_assert(DEP_THREADS / 64 == SA_WAVES_PER_TILE, "one mass window per wave");
+    // exact cell-mass sum: each wave's window, plus the box's counters for the rest
+    unsigned long long* box_digits = P.mass_digits + (uint64_t)box * SA_WORDS;
+    unsigned long long* wave_out =
+        P.mass_waves + ((uint64_t)gtile * SA_WAVES_PER_TILE + threadIdx.x / 64) * SA_WAVE_WORDS;
 
     __shared__ uint8_t sh_rcls[DEP_NRANGES];
     load_source_ranges(P.offsets + (uint64_t)box * buckets_per_box(g), tx, ty, tz, nT, sh_rs, sh_rc, sh_rd,
@@ -920,18 +1003,14 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restr
         }
         __syncthreads();
         deposit_cells<T, true, TE>(P, mesh, tx, ty, tz, sh_keys, M, sh_start, sh_fx, sh_fy, sh_fz,
-                               sh_m, sh_sa);
+                               sh_m, box_digits, wave_out);
     } else {
         wg_sort_u64(gkeys, M);
         build_cell_starts(gkeys, M, sh_start, ncell);
         deposit_cells<T, false, TE>(P, mesh, tx, ty, tz, gkeys, M, sh_start, sh_fx, sh_fy, sh_fz,
-                                sh_m, sh_sa);
+                                sh_m, box_digits, wave_out);
     }
 
-    // the tile's exact mass digits into its box's counters
-    __syncthreads();
-    for (int w = threadIdx.x; w < SA_WORDS; w += blockDim.x)
-        if (sh_sa[w]) atomicAdd(&P.mass_digits[(uint64_t)box * SA_WORDS + w], sh_sa[w]);
 }
 
 template __global__ void k_deposit<double, 0>(DepParams, double*);
@@ -1049,7 +1128,6 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
     __shared__ uint32_t sh_rs[DEPFX_NRANGES], sh_rc[DEPFX_NRANGES + 1];
     __shared__ uint8_t sh_rd[DEPFX_NRANGES];
     __shared__ uint32_t sh_nr;
-    __shared__ uint64_t sh_red[2 * (DEPFX_THREADS / 64)];
 
     const Geom& g = P.g;
     const int nT = g.nT;
@@ -1162,24 +1240,22 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
             mesh[(gx * g.N + gy) * g.N + gz] = contrast_fp32(mcell, P);
         }
     }
+    // each wave's exact sum (digit sums below 2^41) as its window
+    static_assert(DEPFX_THREADS / 64 == SA_WAVES_PER_TILE, "one mass window per wave");
     for (int off = 32; off > 0; off >>= 1) {
         dig0 += __shfl_down(dig0, off);
         dig1 += __shfl_down(dig1, off);
     }
     if (lane_id() == 0) {
-        sh_red[2 * (threadIdx.x / 64)] = dig0;
-        sh_red[2 * (threadIdx.x / 64) + 1] = dig1;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0 && !P.raw) {
-        uint64_t t0 = 0, t1 = 0;  // each below 2^44 (4096 cells of 32-bit digits)
-        for (int w = 0; w < (int)(blockDim.x / 64); ++w) {
-            t0 += sh_red[2 * w];
-            t1 += sh_red[2 * w + 1];
+        unsigned long long* out =
+            P.mass_waves + ((uint64_t)gtile * SA_WAVES_PER_TILE + threadIdx.x / 64) * SA_WAVE_WORDS;
+        if (P.raw) {
+            out[0] = ~0ull;
+        } else {
+            const uint64_t q0 = dig0 + (dig1 << 32);
+            const uint64_t q1 = (dig1 >> 32) + (q0 < dig0 ? 1ull : 0ull);
+            sa_fixed_window(P.mass_digits + (uint64_t)box * SA_WORDS, out, q0, q1, Q.mass_unit);
         }
-        const uint64_t q0 = t0 + (t1 << 32);
-        const uint64_t q1 = (t1 >> 32) + (q0 < t0 ? 1ull : 0ull);
-        sa_add_fixed(P.mass_digits + (uint64_t)box * SA_WORDS, q0, q1, Q.mass_unit);
     }
 }
 
@@ -1199,9 +1275,57 @@ __global__ void k_set_factors(double* __restrict__ f, double k_prev, double k_ne
     }
 }
 
-// Per box: the exact deposited mass from its digit counters, rounded once to
-// nearest with ties to even (NaN when a cell was not a finite non-negative
-// number).  One thread per box.
+// Per box (grid row): the waves' digit windows added up, windows of nearby
+// bases in a thread's registers first, then LDS counters, then one atomic per
+// non-zero counter and workgroup into the box's counters.  Integer sums: the
+// total is exact whatever the order.
+__global__ void __launch_bounds__(256) k_mass_reduce(const unsigned long long* __restrict__ waves,
+                                                     uint64_t waves_per_box,
+                                                     unsigned long long* __restrict__ digits) {
+    __shared__ unsigned long long cnt[SA_DIGITS];
+    const uint64_t box = blockIdx.y;
+    for (int w = threadIdx.x; w < SA_DIGITS; w += blockDim.x) cnt[w] = 0ull;
+    __syncthreads();
+    constexpr int AW = SA_WAVE_WIN + 2;
+    long long tb = -1;
+    uint64_t acc[AW];
+#pragma unroll
+    for (int j = 0; j < AW; ++j) acc[j] = 0ull;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < waves_per_box;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const unsigned long long* ww = waves + (box * waves_per_box + t) * SA_WAVE_WORDS;
+        const long long b = (long long)ww[0];
+        if (b < 0) continue;
+        if (tb < 0) tb = b;
+        const long long k = b - tb;
+        if (k >= 0 && k <= 2) {
+#pragma unroll
+            for (int j = 0; j < SA_WAVE_WIN; ++j) {
+                const uint64_t v = ww[1 + j];
+#pragma unroll
+                for (int o = 0; o <= 2; ++o)
+                    if (k == o) acc[j + o] += v;
+            }
+        } else {
+            for (int j = 0; j < SA_WAVE_WIN; ++j) {
+                const unsigned long long v = ww[1 + j];
+                if (v && b + j < SA_DIGITS) atomicAdd(&cnt[b + j], v);
+            }
+        }
+    }
+    if (tb >= 0) {
+#pragma unroll
+        for (int j = 0; j < AW; ++j)
+            if (acc[j] && tb + j < SA_DIGITS) atomicAdd(&cnt[tb + j], (unsigned long long)acc[j]);
+    }
+    __syncthreads();
+    for (int w = threadIdx.x; w < SA_DIGITS; w += blockDim.x)
+        if (cnt[w]) atomicAdd(&digits[box * SA_WORDS + w], cnt[w]);
+}
+
+// Per box: the exact deposited mass from its counters, rounded once to nearest
+// with ties to even (NaN when a cell was not a finite non-negative number).
+// One thread per box.
 __global__ void k_mass_exact(const unsigned long long* __restrict__ digits, uint32_t nbox,
                              double* __restrict__ out) {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1211,35 +1335,45 @@ __global__ void k_mass_exact(const unsigned long long* __restrict__ digits, uint
         out[b] = __builtin_nan("");
         return;
     }
-    uint32_t v[SA_DIGITS + 2];
+    constexpr int ND = SA_DIGITS + 2;
+    uint32_t v[ND];
     uint64_t carry = 0;
     for (int i = 0; i < SA_DIGITS; ++i) {
-        const uint64_t s = d[i] + carry;  // d[i] < 2^63, carry < 2^32
-        v[i] = (uint32_t)s;
-        carry = s >> 32;
+        const uint64_t sum = d[i] + carry;  // d[i] < 2^62, carry < 2^32
+        v[i] = (uint32_t)sum;
+        carry = sum >> 32;
     }
     v[SA_DIGITS] = (uint32_t)carry;
     v[SA_DIGITS + 1] = (uint32_t)(carry >> 32);
     int top = -1;
-    for (int i = SA_DIGITS + 1; i >= 0 && top < 0; --i)
+    for (int i = ND - 1; i >= 0 && top < 0; --i)
         if (v[i]) top = i * 32 + (31 - __clz(v[i]));
     if (top < 0) {
         out[b] = 0.0;
         return;
     }
-    auto bit = [&](int k) -> uint64_t { return k < 0 ? 0ull : (uint64_t)((v[k >> 5] >> (k & 31)) & 1u); };
+    // bits [lo, lo + 64) of the sum
+    auto bits64 = [&](int lo) -> uint64_t {
+        uint64_t r = 0;
+        for (int k = 0; k < 3; ++k) {
+            const int i = (lo >> 5) + k;
+            if (i < 0 || i >= ND) continue;
+            const int sh = k * 32 - (lo & 31);
+            const uint64_t w = v[i];
+            r |= sh >= 0 ? (sh < 64 ? w << sh : 0ull) : w >> (-sh);
+        }
+        return r;
+    };
     if (top < 53) {
-        uint64_t m = 0;
-        for (int k = 0; k <= top; ++k) m |= bit(k) << k;
-        out[b] = ldexp((double)m, -1074);
+        out[b] = ldexp((double)(bits64(0) & ((1ull << 53) - 1ull)), -1074);
         return;
     }
     const int lo = top - 52;
-    uint64_t m = 0;
-    for (int k = 0; k < 53; ++k) m |= bit(lo + k) << k;
-    const bool round = bit(lo - 1) != 0;
-    bool sticky = false;
-    for (int k = 0; k < lo - 1 && !sticky; ++k) sticky = bit(k) != 0;
+    uint64_t m = bits64(lo) & ((1ull << 53) - 1ull);
+    const int r = lo - 1;  // the rounding bit
+    const bool round = (v[r >> 5] >> (r & 31)) & 1u;
+    bool sticky = (v[r >> 5] & ((1u << (r & 31)) - 1u)) != 0u;
+    for (int i = 0; i < (r >> 5) && !sticky; ++i) sticky = v[i] != 0u;
     if (round && (sticky || (m & 1ull))) ++m;
     out[b] = ldexp((double)m, lo - 1074);
 }

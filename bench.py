@@ -31,16 +31,21 @@ STAGES = ["bucket", "deposit", "fft_z_r2c", "fft_y_fwd", "fft_x_green", "fft_y_b
 
 
 def stage_bytes(stage: str, n_p: int, N: int, real: int) -> float:
-    """Algorithmic HBM bytes per launch of each stage (real = bytes per real)."""
+    """Algorithmic HBM bytes per launch of each stage of a timed step (real =
+    bytes per real).  Timed steps are asynchronous with reused keys: binary32
+    carries 16-B deposit records from the gather to the next deposit, binary64
+    carries bucket keys and ranks."""
     M = N ** 3
     NH = N // 2 + 1
     half = N * N * NH * 2 * real  # one half-spectrum array
     pos = 24.0 * n_p
     mom = 3.0 * real * n_p
     fp32 = real == 4
+    nb = (N // 16) ** 3 * 8  # deposit buckets: 16^3 tiles x 8 boundary classes
     return {
-        # keys + ranks read; FP32: positions read, 16-B deposit records written; FP64: 4-B order written
-        "bucket": (8.0 * n_p + pos + 16.0 * n_p) if fp32 else 12.0 * n_p,
+        # FP32: capacity scan (counts, overflow counts read; offsets, overflow
+        # prefix, next capacities written); FP64: keys + ranks read, 4-B order written
+        "bucket": 20.0 * nb if fp32 else 12.0 * n_p,
         # FP32: records read; FP64: order + positions read; mesh written
         "deposit": (16.0 * n_p if fp32 else 4.0 * n_p + pos) + real * M,
         "fft_z_r2c": real * M + half,               # real mesh in, half spectrum out
@@ -48,18 +53,23 @@ def stage_bytes(stage: str, n_p: int, N: int, real: int) -> float:
         "fft_x_green": 2.0 * half,
         "fft_y_bwd": 2.0 * half,
         "fft_z_c2r": half + real * M,
-        "gather_kdk": 2.0 * (pos + mom) + real * M,  # state in/out + one potential read
+        # state in/out + one potential read; FP32: deposit records written,
+        # FP64: keys + ranks written
+        "gather_kdk": 2.0 * (pos + mom) + real * M + (16.0 if fp32 else 8.0) * n_p,
     }[stage]
 
 
 def step_alg_bytes(n_p: int, N: int, real: int) -> float:
-    """Compulsory bytes of one KDK step: FP32-mixed 96*N_p + 24*M, FP64 120*N_p + 48*M."""
+    """Compulsory bytes of one KDK step: the particle state read and written
+    once (FP32-mixed 72 B, FP64 96 B per particle), the positions read once
+    more by the deposit (24 B), and the mesh passes (FP32-mixed 24 B, FP64
+    48 B per cell): FP32-mixed 96*N_p + 24*M, FP64 120*N_p + 48*M."""
     M = N ** 3
     return (96.0 * n_p + 24.0 * M) if real == 4 else (120.0 * n_p + 48.0 * M)
 
 
-STAGE_KERNEL = {"bucket": "hy::k_scatter_records", "deposit": "void hy::k_deposit_fixed<16>",
-                "gather_kdk": "void hy::k_gather_kdk<float, float, true>"}
+STAGE_KERNEL = {"bucket": "hy::k_cap_apply", "deposit": "void hy::k_deposit_fixed<16>",
+                "gather_kdk": "void hy::k_gather_kdk<float, float, true, true>"}
 FFT_KERNEL = {"fft_z_r2c": "k_fft_z_r2c", "fft_y_fwd": "k_fft_y_fwd", "fft_x_green": "k_fft_x_green",
               "fft_y_bwd": "k_fft_y_bwd", "fft_z_c2r": "k_fft_z_c2r"}
 

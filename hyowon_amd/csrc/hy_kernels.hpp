@@ -1580,10 +1580,19 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
             const bool exact = g.inv_exact != 0;
             auto cdiff = [&](T d) -> T { return exact ? (-d) * inv_two_dx : -d / two_dx; };
             T gx = 0, gy = 0, gz = 0;
-            // lost non-zero products (src/mesh/mass_assignment_gradient.cpp:157-196):
+            // Lost non-zero products (src/mesh/mass_assignment_gradient.cpp:157-196):
             // a weight whose three factors are non-zero, a gradient of a non-zero
-            // difference, or a contribution of a non-zero weight and gradient
-            // that rounds to zero
+            // difference, or a contribution of a non-zero weight and gradient,
+            // that rounds to zero.  The reference's corner condition
+            //   ((w != 0 || w_lost) && g_lost) || (w_lost && g_nonzero) || (w != 0 && c_lost)
+            // equals, for the finite weights here, "all three factors non-zero and
+            // some axis with d != 0 and (c == 0 or w == 0)": with w == 0 it asks
+            // for a non-zero difference (g_lost || g_nonzero), with w != 0 for a
+            // non-zero difference whose contribution w * g is zero (g lost, or
+            // g != 0 and the product lost).
+            const bool nzx[2] = {wx[0] != (T)0, wx[1] != (T)0};
+            const bool nzy[2] = {wy[0] != (T)0, wy[1] != (T)0};
+            const bool nzz[2] = {wz[0] != (T)0, wz[1] != (T)0};
             bool lost = false;
 #pragma unroll
             for (int ox = 0; ox < 2; ++ox) {
@@ -1599,16 +1608,11 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
                         const T dxv = X[ox + 2][oy][oz] - X[ox][oy][oz];
                         const T dyv = ypl - ymi;
                         const T dzv = zpl - zmi;
-                        const T gxv = cdiff(dxv), gyv = cdiff(dyv), gzv = cdiff(dzv);
-                        const T cx = w * gxv, cy = w * gyv, cz = w * gzv;
-                        const bool w_lost = w == (T)0 && wx[ox] != (T)0 && wy[oy] != (T)0 && wz[oz] != (T)0;
-                        const bool g_lost = (dxv != (T)0 && gxv == (T)0) || (dyv != (T)0 && gyv == (T)0) ||
-                                            (dzv != (T)0 && gzv == (T)0);
-                        const bool g_nonzero = gxv != (T)0 || gyv != (T)0 || gzv != (T)0;
-                        const bool c_lost = (gxv != (T)0 && cx == (T)0) || (gyv != (T)0 && cy == (T)0) ||
-                                            (gzv != (T)0 && cz == (T)0);
-                        lost |= ((w != (T)0 || w_lost) && g_lost) || (w_lost && g_nonzero) ||
-                                (w != (T)0 && c_lost);
+                        const T cx = w * cdiff(dxv), cy = w * cdiff(dyv), cz = w * cdiff(dzv);
+                        const bool wz0 = w == (T)0;
+                        const bool any = (dxv != (T)0 && (cx == (T)0 || wz0)) || (dyv != (T)0 && (cy == (T)0 || wz0)) ||
+                                         (dzv != (T)0 && (cz == (T)0 || wz0));
+                        lost |= nzx[ox] && nzy[oy] && nzz[oz] && any;
                         gx += cx;
                         gy += cy;
                         gz += cz;

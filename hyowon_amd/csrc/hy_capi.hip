@@ -143,7 +143,8 @@ struct hy_pm_ctx {
     uint32_t* xcounts = nullptr;   // x-plane bins (N)
     uint32_t* xoffsets = nullptr;  // N+1
     unsigned long long* mass_digits = nullptr;  // per box: exact deposited-mass digit counters
-    unsigned long long* mass_waves = nullptr;   // per deposit wave: its exact mass as a digit window
+    unsigned long long* mass_waves = nullptr;   // per deposit wave: its exact mass (binary64: digit window,
+                                                // binary32: two digit sums of its 32.32 cells)
     double* mass_total = nullptr;
     uint64_t* scratch = nullptr;
     uint64_t scratch_cap = 0;
@@ -471,8 +472,12 @@ void launch_deposit(hy_pm_ctx* c, const hy::DepParams& P, bool status_check = tr
     launch_check("k_deposit");
     // the exact sum of every cell's deposited mass (ExactPositiveDoubleSum), per box
     const uint64_t waves = c->ntiles * hy::SA_WAVES_PER_TILE;
-    hipLaunchKernelGGL(hy::k_mass_reduce, dim3((uint32_t)std::min<uint64_t>(256, blocks_for(waves, 256)), c->nbox),
-                       dim3(256), 0, c->stream, c->mass_waves, waves, c->mass_digits);
+    const dim3 rgrid((uint32_t)std::min<uint64_t>(256, blocks_for(waves, 256)), c->nbox);
+    if (c->prec == HY_PREC_FP64)
+        hipLaunchKernelGGL(hy::k_mass_reduce, rgrid, dim3(256), 0, c->stream, c->mass_waves, waves, c->mass_digits);
+    else
+        hipLaunchKernelGGL(hy::k_mass_reduce_fixed, rgrid, dim3(256), 0, c->stream, c->mass_waves, waves,
+                           c->mass_unit, c->mass_digits);
     hipLaunchKernelGGL(hy::k_mass_exact, dim3((uint32_t)blocks_for(c->nbox, 64)), dim3(64), 0, c->stream,
                        c->mass_digits, c->nbox, c->mass_total);
     launch_check("k_mass_exact");
@@ -1017,8 +1022,8 @@ int hy_pm_create(const hy_pm_desc* desc, hy_pm_ctx** out) {
         c->xcounts = (uint32_t*)c->alloc((size_t)N * 4);
         c->xoffsets = (uint32_t*)c->alloc(((size_t)N + 1) * 4);
         c->mass_digits = (unsigned long long*)c->alloc((uint64_t)hy::SA_WORDS * c->nbox * 8);
-        c->mass_waves = (unsigned long long*)c->alloc((uint64_t)hy::SA_WAVE_WORDS * hy::SA_WAVES_PER_TILE *
-                                                      c->ntiles * c->nbox * 8);
+        const uint64_t wave_words = c->prec == HY_PREC_FP64 ? hy::SA_WAVE_WORDS : 2;  // window, or digit-sum pair
+        c->mass_waves = (unsigned long long*)c->alloc(wave_words * hy::SA_WAVES_PER_TILE * c->ntiles * c->nbox * 8);
         c->mass_total = (double*)c->alloc(std::max<uint64_t>(2, c->nbox) * 8);
         c->scratch_cap = 2 * cap + (1u << 16);
         c->scratch = (uint64_t*)c->alloc(c->scratch_cap * 8);

@@ -523,6 +523,46 @@ constexpr int SA_WAVE_WIN = 8;
 constexpr int SA_WAVE_WORDS = SA_WAVE_WIN + 1;
 constexpr int SA_WAVES_PER_TILE = 8;  // deposit workgroups of 512 threads
 
+// Wave-wide integer sum and minimum in the VALU pipe (no LDS permutes): DPP
+// row shifts by 1, 2, 4 and 8 lanes leave each 16-lane row's total (minimum)
+// in its last lane, and the four rows' last lanes are read as scalars.  Every
+// lane of the wave must be active; the result is the same in every lane.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v, uint32_t fill) {
+    // lanes whose source lies outside their row keep fill
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, CTRL, 0xf, 0xf, false);
+}
+
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+    return ((uint64_t)dpp_u32<CTRL>((uint32_t)(v >> 32), 0u) << 32) | dpp_u32<CTRL>((uint32_t)v, 0u);
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+    v += dpp_u64<0x111>(v);  // row_shr:1
+    v += dpp_u64<0x112>(v);  // row_shr:2
+    v += dpp_u64<0x114>(v);  // row_shr:4
+    v += dpp_u64<0x118>(v);  // row_shr:8
+    uint64_t s = 0;
+#pragma unroll
+    for (int r = 15; r < 64; r += 16)
+        s += ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), r) << 32) |
+             (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, r);
+    return s;
+}
+
+__device__ __forceinline__ int wave_min_i32(int v) {
+    constexpr uint32_t big = 0x7fffffffu;
+    v = min(v, (int)dpp_u32<0x111>((uint32_t)v, big));
+    v = min(v, (int)dpp_u32<0x112>((uint32_t)v, big));
+    v = min(v, (int)dpp_u32<0x114>((uint32_t)v, big));
+    v = min(v, (int)dpp_u32<0x118>((uint32_t)v, big));
+    int m = __builtin_amdgcn_readlane(v, 15);
+#pragma unroll
+    for (int r = 31; r < 64; r += 16) m = min(m, __builtin_amdgcn_readlane(v, r));
+    return m;
+}
+
 // A lane's running sum of cell masses: a 192-bit integer (a2:a1:a0) in
 // units of the binary position 32*base (base < 0: empty), base one digit
 // below the first mass it took.  A mass m 2^s (s its bit position, 53-bit m)
@@ -571,17 +611,15 @@ __device__ __forceinline__ void sa_acc_add(SaAcc& a, unsigned long long* d, doub
     }
 }
 
-// Every lane of the wave calls this (the shuffles need all lanes): the wave's
-// window (base, SA_WAVE_WIN digits) goes to out from lane 0; lanes whose sums
-// start more than 2 digits above the wave's lowest add theirs to d directly.
+// Every lane of the wave calls this (the wave reductions need all lanes): the
+// wave's window (base, SA_WAVE_WIN digits) goes to out from lanes
+// 0..SA_WAVE_WIN; lanes whose sums start more than 2 digits above the wave's
+// lowest add theirs to d directly.
 __device__ __forceinline__ void sa_acc_flush(const SaAcc& a, unsigned long long* d, unsigned long long* out) {
-    int b0 = a.base < 0 ? 0x7fffffff : a.base;
-    for (int off = 32; off > 0; off >>= 1) {
-        const int o = __shfl_xor(b0, off);
-        b0 = o < b0 ? o : b0;
-    }
+    const int b0 = wave_min_i32(a.base < 0 ? 0x7fffffff : a.base);
+    const uint32_t lane = lane_id();
     if (b0 == 0x7fffffff) {  // no lane holds a mass
-        if (lane_id() == 0) out[0] = ~0ull;
+        if (lane == 0) out[0] = ~0ull;
         return;
     }
     const uint32_t w[6] = {(uint32_t)a.a0, (uint32_t)(a.a0 >> 32), (uint32_t)a.a1, (uint32_t)(a.a1 >> 32),
@@ -595,7 +633,7 @@ __device__ __forceinline__ void sa_acc_flush(const SaAcc& a, unsigned long long*
             if (w[j] && a.base + j < SA_DIGITS) atomicAdd(&d[a.base + j], (unsigned long long)w[j]);
     }
     static_assert(SA_WAVE_WIN == 8, "window of the 6-digit lane sums at offsets 0..2");
-    if (lane_id() == 0) out[0] = (unsigned long long)b0;
+    uint64_t word = (uint64_t)b0;
 #pragma unroll
     for (int j = 0; j < SA_WAVE_WIN; ++j) {
         uint64_t v = 0;
@@ -604,25 +642,22 @@ __device__ __forceinline__ void sa_acc_flush(const SaAcc& a, unsigned long long*
             for (int t = 0; t < 6; ++t)
                 if (t + k == j) v = w[t];
         }
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (lane_id() == 0) out[1 + j] = b0 + j < SA_DIGITS ? v : 0ull;
+        v = wave_sum_u64(v);
+        if (lane == (uint32_t)j + 1) word = b0 + j < SA_DIGITS ? v : 0ull;
     }
+    if (lane <= (uint32_t)SA_WAVE_WIN) out[lane] = word;
 }
 
-// The window (out: base, SA_WAVE_WIN digits) of Q * unit * 2^-32 exactly,
-// Q = (q1 2^64 + q0) the exact integer sum of a wave's 32.32 fixed-point
-// cells (q1 < 2^32), unit > 0 a normal binary64 (else the sum is flagged
-// invalid in d).
-__device__ __forceinline__ void sa_fixed_window(unsigned long long* d, unsigned long long* out, uint64_t q0,
-                                                uint64_t q1, double unit) {
+// Q * unit * 2^-32 exactly as SA_WAVE_WIN 32-bit digits from digit limb, Q =
+// (q1 2^64 + q0) an exact integer sum of 32.32 fixed-point cells (q1 < 2^32).
+// False when unit is not a normal binary64 of at least 2^-1042 (the sum is
+// then flagged invalid).
+__device__ __forceinline__ bool sa_fixed_digits(uint64_t q0, uint64_t q1, double unit, int& limb,
+                                                uint32_t (&dig)[SA_WAVE_WIN]) {
     const uint64_t bits = (uint64_t)__double_as_longlong(unit);
     const int e = (int)((bits >> 52) & 0x7ff);
     const int shift = e - 1 - 32;
-    if (e == 0 || e == 0x7ff || shift < 0) {
-        atomicOr(&d[SA_DIGITS], 1ull);
-        out[0] = ~0ull;
-        return;
-    }
+    if (e == 0 || e == 0x7ff || shift < 0) return false;
     const uint64_t m = (bits & ((1ull << 52) - 1ull)) | (1ull << 52);
     // P = Q * m as three 64-bit words
     const uint64_t a_lo = q0 * m, a_hi = __umul64hi(q0, m);
@@ -632,14 +667,15 @@ __device__ __forceinline__ void sa_fixed_window(unsigned long long* d, unsigned 
     const uint32_t v[6] = {(uint32_t)a_lo, (uint32_t)(a_lo >> 32), (uint32_t)p1, (uint32_t)(p1 >> 32),
                            (uint32_t)p2, (uint32_t)(p2 >> 32)};
     // the product shifted by `shift` bits: seven 32-bit digits from shift / 32
-    const int limb = shift >> 5, off = shift & 31;
-    out[0] = (unsigned long long)limb;
+    limb = shift >> 5;
+    const int off = shift & 31;
 #pragma unroll
     for (int k = 0; k < SA_WAVE_WIN; ++k) {
         const uint64_t lo = k < 6 ? (uint64_t)v[k] << off : 0ull;
         const uint64_t hi = (k > 0 && k <= 6 && off) ? (uint64_t)v[k - 1] >> (32 - off) : 0ull;
-        out[1 + k] = limb + k < SA_DIGITS ? ((lo | hi) & 0xffffffffull) : 0ull;
+        dig[k] = (uint32_t)((lo | hi) & 0xffffffffull);
     }
+    return true;
 }
 
 struct DepParams {
@@ -922,6 +958,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(DepParams P, T* __restr
         __syncthreads();
         if (sh_gbase + M > P.scratch_cap) {
             if (threadIdx.x == 0) atomicOr(P.err, ERR_SCRATCH);
+            if (lane_id() == 0) wave_out[0] = ~0ull;  // this tile's waves hold no mass
             return;
         }
         gkeys = P.scratch + sh_gbase;
@@ -1240,22 +1277,14 @@ __global__ void __launch_bounds__(DEPFX_THREADS) k_deposit_fixed(DepFxParams Q, 
             mesh[(gx * g.N + gy) * g.N + gz] = contrast_fp32(mcell, P);
         }
     }
-    // each wave's exact sum (digit sums below 2^41) as its window
-    static_assert(DEPFX_THREADS / 64 == SA_WAVES_PER_TILE, "one mass window per wave");
-    for (int off = 32; off > 0; off >>= 1) {
-        dig0 += __shfl_down(dig0, off);
-        dig1 += __shfl_down(dig1, off);
-    }
+    // each wave's two digit sums (below 2^41), for k_mass_reduce_fixed
+    static_assert(DEPFX_THREADS / 64 == SA_WAVES_PER_TILE, "one mass pair per wave");
+    if (P.raw) dig0 = dig1 = 0;  // signed cells: no mass identity
+    const uint64_t s0 = wave_sum_u64(dig0), s1 = wave_sum_u64(dig1);
     if (lane_id() == 0) {
-        unsigned long long* out =
-            P.mass_waves + ((uint64_t)gtile * SA_WAVES_PER_TILE + threadIdx.x / 64) * SA_WAVE_WORDS;
-        if (P.raw) {
-            out[0] = ~0ull;
-        } else {
-            const uint64_t q0 = dig0 + (dig1 << 32);
-            const uint64_t q1 = (dig1 >> 32) + (q0 < dig0 ? 1ull : 0ull);
-            sa_fixed_window(P.mass_digits + (uint64_t)box * SA_WORDS, out, q0, q1, Q.mass_unit);
-        }
+        typedef unsigned long long U2 __attribute__((ext_vector_type(2)));
+        *reinterpret_cast<U2*>(P.mass_waves + ((uint64_t)gtile * SA_WAVES_PER_TILE + threadIdx.x / 64) * 2) =
+            U2{s0, s1};
     }
 }
 
@@ -1321,6 +1350,50 @@ __global__ void __launch_bounds__(256) k_mass_reduce(const unsigned long long* _
     __syncthreads();
     for (int w = threadIdx.x; w < SA_DIGITS; w += blockDim.x)
         if (cnt[w]) atomicAdd(&digits[box * SA_WORDS + w], cnt[w]);
+}
+
+// Binary32 deposit: per box (grid row), the waves' digit-sum pairs (s0, s1)
+// added up per workgroup (below 2^54 each: at most 2^13 waves of under 2^41
+// per thread at the grid's 2^16 threads per box and a mesh of 4096^3), and
+// the workgroup's exact total (s0 + s1 2^32) * unit * 2^-32 added to the box's
+// counters as digits.
+__global__ void __launch_bounds__(256) k_mass_reduce_fixed(const unsigned long long* __restrict__ waves,
+                                                           uint64_t waves_per_box, double unit,
+                                                           unsigned long long* __restrict__ digits) {
+    __shared__ uint64_t part[2][4];
+    const uint64_t box = blockIdx.y;
+    typedef unsigned long long U2 __attribute__((ext_vector_type(2)));
+    uint64_t s0 = 0, s1 = 0;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < waves_per_box;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const U2 v = *reinterpret_cast<const U2*>(waves + (box * waves_per_box + t) * 2);
+        s0 += v.x;
+        s1 += v.y;
+    }
+    s0 = wave_sum_u64(s0);
+    s1 = wave_sum_u64(s1);
+    if (lane_id() == 0) {
+        part[0][threadIdx.x / 64] = s0;
+        part[1][threadIdx.x / 64] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    uint64_t a0 = 0, a1 = 0;
+    for (int w = 0; w < (int)blockDim.x / 64; ++w) {
+        a0 += part[0][w];
+        a1 += part[1][w];
+    }
+    const uint64_t q0 = a0 + (a1 << 32);
+    const uint64_t q1 = (a1 >> 32) + (q0 < a0 ? 1ull : 0ull);
+    unsigned long long* d = digits + box * SA_WORDS;
+    int limb = 0;
+    uint32_t dig[SA_WAVE_WIN];
+    if (!sa_fixed_digits(q0, q1, unit, limb, dig)) {
+        atomicOr(&d[SA_DIGITS], 1ull);
+        return;
+    }
+    for (int k = 0; k < SA_WAVE_WIN; ++k)
+        if (dig[k] && limb + k < SA_DIGITS) atomicAdd(&d[limb + k], (unsigned long long)dig[k]);
 }
 
 // Per box: the exact deposited mass from its counters, rounded once to nearest

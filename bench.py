@@ -44,10 +44,11 @@ def stage_bytes(stage: str, n_p: int, N: int, real: int) -> float:
     nb = (N // 16) ** 3 * 8  # deposit buckets: 16^3 tiles x 8 boundary classes
     return {
         # FP32: capacity scan (counts, overflow counts read; offsets, overflow
-        # prefix, next capacities written); FP64: keys + ranks read, 4-B order written
-        "bucket": 20.0 * nb if fp32 else 12.0 * n_p,
-        # FP32: records read; FP64: order + positions read; mesh written
-        "deposit": (16.0 * n_p if fp32 else 4.0 * n_p + pos) + real * M,
+        # prefix, next capacities written); FP64: keys + ranks and positions
+        # read, 32-B deposit records (cell, index, three fractions) written
+        "bucket": 20.0 * nb if fp32 else 8.0 * n_p + pos + 32.0 * n_p,
+        # records read (FP32 16 B, FP64 32 B); mesh written
+        "deposit": (16.0 if fp32 else 32.0) * n_p + real * M,
         "fft_z_r2c": real * M + half,               # real mesh in, half spectrum out
         "fft_y_fwd": 2.0 * half,
         "fft_x_green": 2.0 * half,

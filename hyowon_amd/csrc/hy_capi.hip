@@ -657,9 +657,15 @@ void launch_gather(hy_pm_ctx* c, hy::GatherArgs<T, PT> a) {
     a.dfac = c->dfac;
     const bool o32 = c->nbox == 1 && (uint64_t)c->N * c->N * c->N * sizeof(T) <= (1ull << 32);
     const dim3 grid((uint32_t)blocks_for(a.n, 256)), block(256);
-    void (*kern)(hy::GatherArgs<T, PT>) = o32 ? hy::k_gather_kdk<T, PT, true> : hy::k_gather_kdk<T, PT, false>;
+    const bool out = a.acc_x || a.g_x;
+    void (*kern)(hy::GatherArgs<T, PT>) =
+        out ? (o32 ? hy::k_gather_kdk<T, PT, true, false, true> : hy::k_gather_kdk<T, PT, false, false, true>)
+            : (o32 ? hy::k_gather_kdk<T, PT, true, false, false> : hy::k_gather_kdk<T, PT, false, false, false>);
     if constexpr (std::is_same_v<T, float>) {
-        if (a.rec_out) kern = o32 ? hy::k_gather_kdk<T, PT, true, true> : hy::k_gather_kdk<T, PT, false, true>;
+        if (a.rec_out)
+            kern = out ? (o32 ? hy::k_gather_kdk<T, PT, true, true, true> : hy::k_gather_kdk<T, PT, false, true, true>)
+                       : (o32 ? hy::k_gather_kdk<T, PT, true, true, false>
+                              : hy::k_gather_kdk<T, PT, false, true, false>);
     }
     hipLaunchKernelGGL(kern, grid, block, 0, c->stream, a);
     launch_check("k_gather_kdk");

@@ -1552,7 +1552,9 @@ struct GatherArgs {
 // 32-bit byte offsets from the mesh base (one scalar base, no per-lane 64-bit
 // address arithmetic).  REC: the drifted positions' deposit records are
 // written into the capacity layout (binary32); otherwise keys and ranks.
-template <typename T, typename PT, bool O32 = false, bool REC = false>
+// OUT: the force outputs acc and g may be set; without them the kernel keeps
+// twelve fewer scalar registers live (8 waves per SIMD instead of 7).
+template <typename T, typename PT, bool O32 = false, bool REC = false, bool OUT = true>
 __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
     // each XCD walks a contiguous particle range (spatially compact for
     // lattice-ordered particles), so potential lines are reused in its L2
@@ -1697,15 +1699,17 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
             } else if (!isfinite(gx) || !isfinite(gy) || !isfinite(gz)) {
                 fail = ERR_FORCE_NONFINITE;
             } else {
-                if (a.acc_x) {
-                    a.acc_x[p] += (double)gx;
-                    a.acc_y[p] += (double)gy;
-                    a.acc_z[p] += (double)gz;
-                }
-                if (a.g_x) {
-                    a.g_x[p] = gx;
-                    a.g_y[p] = gy;
-                    a.g_z[p] = gz;
+                if constexpr (OUT) {
+                    if (a.acc_x) {
+                        a.acc_x[p] += (double)gx;
+                        a.acc_y[p] += (double)gy;
+                        a.acc_z[p] += (double)gz;
+                    }
+                    if (a.g_x) {
+                        a.g_x[p] = gx;
+                        a.g_y[p] = gy;
+                        a.g_z[p] = gz;
+                    }
                 }
                 if (kick) {
                     PT qx = qx0, qy = qy0, qz = qz0;

@@ -655,7 +655,9 @@ template <typename T, typename PT>
 void launch_gather(hy_pm_ctx* c, hy::GatherArgs<T, PT> a) {
     a.lat = c->nbox == 1 ? lattice_tiling(a.n, 256, c->gather_rows) : hy::LatticeTiling{0, 0, 0, 0, -1, -1, -1, -1};
     a.dfac = c->dfac;
-    const bool o32 = c->nbox == 1 && (uint64_t)c->N * c->N * c->N * sizeof(T) <= (1ull << 32);
+    // 32-bit stencil offsets from a per-wave box base: no wave may span two boxes
+    const bool wave_in_box = c->nbox == 1 || (a.n % c->nbox == 0 && (a.n / c->nbox) % 64 == 0);
+    const bool o32 = wave_in_box && (uint64_t)c->N * c->N * c->N * sizeof(T) <= (1ull << 32);
     const dim3 grid((uint32_t)blocks_for(a.n, 256)), block(256);
     const bool out = a.acc_x || a.g_x;
     void (*kern)(hy::GatherArgs<T, PT>) =

@@ -1548,9 +1548,10 @@ struct GatherArgs {
     uint32_t* err;
 };
 
-// O32: one box whose mesh spans at most 2^32 bytes; the stencil then uses
-// 32-bit byte offsets from the mesh base (one scalar base, no per-lane 64-bit
-// address arithmetic).  REC: the drifted positions' deposit records are
+// O32: every wave's particles lie in one box (one box, or a particle count
+// per box that is a multiple of 64), whose mesh spans at most 2^32 bytes; the
+// stencil then uses 32-bit byte offsets from the box's mesh base (one scalar
+// base, no per-lane 64-bit address arithmetic).  REC: the drifted positions' deposit records are
 // written into the capacity layout (binary32); otherwise keys and ranks.
 // OUT: the force outputs acc and g may be set; without them the kernel keeps
 // twelve fewer scalar registers live (8 waves per SIMD instead of 7).
@@ -1594,7 +1595,9 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
             using Off = typename std::conditional<O32, uint32_t, uint64_t>::type;
             const Off NN = (Off)N * (Off)N;
             const Off es = O32 ? (Off)sizeof(T) : (Off)1;  // offsets in bytes (O32) or elements
-            const T* phi = O32 ? a.phi : a.phi + (uint64_t)box_of(p, g) * (uint64_t)NN * (uint64_t)N;
+            // O32: the wave's box, read from its first active lane, gives a scalar base
+            const uint64_t box = O32 ? (uint64_t)__builtin_amdgcn_readfirstlane(box_of(p, g)) : (uint64_t)box_of(p, g);
+            const T* phi = a.phi + box * (uint64_t)N * (uint64_t)N * (uint64_t)N;
             auto ld = [&](Off o) -> T {
                 if constexpr (O32) return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(phi) + o);
                 else return phi[o];

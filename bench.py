@@ -680,13 +680,23 @@ def cpu_baseline(args):
     mom = [p.cpu().numpy() for p in mom]
     m = ic.particle_mass(ns, Ls)
     b = np.array([a0, a0 * math.exp(math.log(1.0 / a0) / args.schedule_steps)])
+    oracle.stage_reset()
     t0 = time.perf_counter()
     oracle.run(pos[0], pos[1], pos[2], mom[0], mom[1], mom[2], Ls, Ns, b, m)
     t1 = time.perf_counter()
     # the run does 2 force refreshes for 1 step; a steady step has 1 refresh
     step_s = (t1 - t0) / 2.0
+    stages = oracle.stage_times()
+    stage_ms = {k: 1e3 * (v if k == "kdk" else v / 2.0) for k, v in stages.items()}
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
+    except OSError:
+        pass
     return {"value": ns ** 3 / step_s, "unit": "particle-updates/s", "cores": cores,
-            "kind": "port",
+            "kind": "port", "cpu_model": model, "fft": "the oracle's own binary64 FFT",
+            "stage_ms_per_step": stage_ms,
             "sample": f"oracle (binary64 C/OpenMP) one KDK step of {ns}^3 particles on a "
                       f"{Ns}^3 mesh, box {Ls:g} Mpc/h (same spacing as the GPU workload); "
                       f"time of 2 force refreshes + kicks + drift halved; {t1 - t0:.1f} s total"}

@@ -124,6 +124,8 @@ def lib():
         L.fn("or_lpt_first_order", C.c_int, _u64, _pd, _u64, _d, _d, _pd, _pd, _pd, _pd, _pd, _pd)
         L.fn("or_set_threads", None, C.c_int)
         L.fn("or_get_threads", C.c_int)
+        L.fn("or_stage_reset", None)
+        L.fn("or_stage_times", None, _pd)
         L.fn("or_gravitational_constant", _d)
         L.fn("or_rho_crit0", _d)
         _oracle = L.lib
@@ -336,6 +338,21 @@ def run(x, y, z, px, py, pz, L, N, boundaries, uniform_mass, om=0.3, ol=0.7,
     _check(lib().or_run(len(st[0]), *[_ptr(a) for a in st], *[_ptr(a) for a in acc], None,
                         uniform_mass, L, N, int(deconvolve), om, ol, _ptr(b), len(b)), "or_run")
     return st, acc
+
+
+STAGES = ("sort", "deposit", "mass_contrast", "fft_forward", "green", "fft_inverse", "gather", "kdk")
+
+
+def stage_reset() -> None:
+    """Zero the per-stage wall-time accumulators of the force refresh and step loop."""
+    lib().or_stage_reset()
+
+
+def stage_times() -> dict:
+    """Seconds per stage accumulated since stage_reset()."""
+    out = np.zeros(len(STAGES))
+    lib().or_stage_times(_ptr(out))
+    return dict(zip(STAGES, out.tolist()))
 
 
 def set_threads(n: int) -> None:

@@ -12,6 +12,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -30,6 +31,18 @@ int or_get_threads(void) {
 }
 
 #define OR_NT or_get_threads()
+
+static double g_stage_s[OR_STAGES];
+
+static double wall_s(void) {
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+
+void or_stage_reset(void) { memset(g_stage_s, 0, sizeof g_stage_s); }
+
+void or_stage_times(double* out) { memcpy(out, g_stage_s, sizeof g_stage_s); }
 
 /* ------------------------------------------------------------------------- */
 /* Constants: include/cosmo_nbody/cosmology/units.hpp:14-30                   */
@@ -199,8 +212,11 @@ int or_deposit(uint64_t n, const double* x, const double* y, const double* z,
     uint64_t* offsets = (uint64_t*)malloc((N + 1) * sizeof(uint64_t));
     uint32_t* order = (uint32_t*)malloc(n * sizeof(uint32_t));
     if (!offsets || !order) { free(offsets); free(order); return OR_ERUNTIME; }
+    double t0 = wall_s();
     int err = or_xplane_order(n, x, L, N, offsets, order);
     if (err != OR_OK) { free(offsets); free(order); return err; }
+    g_stage_s[OR_STAGE_SORT] += wall_s() - t0;
+    t0 = wall_s();
 
     int failed = 0;
 #pragma omp parallel for schedule(dynamic, 1) num_threads(OR_NT) reduction(|: failed)
@@ -234,6 +250,7 @@ int or_deposit(uint64_t n, const double* x, const double* y, const double* z,
     }
     free(offsets);
     free(order);
+    g_stage_s[OR_STAGE_DEPOSIT] += wall_s() - t0;
     return failed ? OR_ELOGIC : OR_OK;
 }
 
@@ -810,6 +827,7 @@ int or_pm_force(uint64_t n, const double* x, const double* y, const double* z,
     }
     int err = or_deposit(n, x, y, z, masses, uniform_mass, L, N, mesh);
     double input_mass = 0.0;
+    double t0 = wall_s();
     if (err == OR_OK) {
         input_mass = masses ? or_exact_sum(masses, n) : or_exact_repeated(uniform_mass, n);
         if (!isfinite(input_mass) || input_mass <= 0.0 || n == 0) err = OR_ERUNTIME;
@@ -820,10 +838,19 @@ int or_pm_force(uint64_t n, const double* x, const double* y, const double* z,
         else err = or_require_cic_mass_conservation(input_mass, deposited, n);
     }
     if (err == OR_OK) err = or_density_contrast(mesh, N, L / (double)N, input_mass);
+    double t1 = wall_s();
+    g_stage_s[OR_STAGE_MASS_CONTRAST] += t1 - t0;
     if (err == OR_OK) err = or_fft_r2c_3d(N, mesh, spec);
+    t0 = wall_s();
+    g_stage_s[OR_STAGE_FFT_FORWARD] += t0 - t1;
     if (err == OR_OK) err = or_green_apply(N, L, deconvolve_cic, spec);
+    t1 = wall_s();
+    g_stage_s[OR_STAGE_GREEN] += t1 - t0;
     if (err == OR_OK) err = or_fft_c2r_3d(N, spec, mesh);
+    t0 = wall_s();
+    g_stage_s[OR_STAGE_FFT_INVERSE] += t0 - t1;
     if (err == OR_OK) err = or_gather_centered_gradient_add(n, mesh, x, y, z, L, N, ax, ay, az);
+    g_stage_s[OR_STAGE_GATHER] += wall_s() - t0;
     free(spec);
     if (!phi_out) free(mesh);
     return err;
@@ -1414,12 +1441,14 @@ int or_run(uint64_t n, double* x, double* y, double* z,
         err = or_drift_factor(a0, a1, om, ol, &D);
         if (!err) err = or_kick_factor(a0, ah, om, ol, &K1);
         if (!err) err = or_kick_factor(ah, a1, om, ol, &K2);
+        double t0 = wall_s();
         if (!err) err = or_kick_drift(n, x, y, z, px, py, pz, ax, ay, az, K1, D, L);
-        if (!err) {
-            zero3(n, ax, ay, az);
-            err = or_pm_force(n, x, y, z, masses, uniform_mass, L, N, deconvolve_cic, ax, ay, az, NULL);
-        }
+        if (!err) zero3(n, ax, ay, az);
+        g_stage_s[OR_STAGE_KDK] += wall_s() - t0;
+        if (!err) err = or_pm_force(n, x, y, z, masses, uniform_mass, L, N, deconvolve_cic, ax, ay, az, NULL);
+        t0 = wall_s();
         if (!err) err = or_kick(n, px, py, pz, ax, ay, az, K2);
+        g_stage_s[OR_STAGE_KDK] += wall_s() - t0;
     }
     return err;
 }

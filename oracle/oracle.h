@@ -255,6 +255,24 @@ double or_pk_raw_power_minus(double L, double re, double im, double s);
 void or_set_threads(int threads);
 int or_get_threads(void);
 
+/* Wall time per stage of or_deposit, or_pm_force and or_run, accumulated in
+ * seconds since the last or_stage_reset: x-plane sort, deposit, exact mass
+ * check and density contrast, forward FFT, Green multiply, inverse FFT,
+ * gradient gather, kick/drift passes. */
+#define OR_STAGES 8
+enum {
+    OR_STAGE_SORT = 0,
+    OR_STAGE_DEPOSIT,
+    OR_STAGE_MASS_CONTRAST,
+    OR_STAGE_FFT_FORWARD,
+    OR_STAGE_GREEN,
+    OR_STAGE_FFT_INVERSE,
+    OR_STAGE_GATHER,
+    OR_STAGE_KDK
+};
+void or_stage_reset(void);
+void or_stage_times(double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -515,10 +515,11 @@ struct DepCfg<double> { static constexpr int CAP = 1024; };
 // ---------------------------------------------------------------------------
 constexpr int SA_DIGITS = 72;
 constexpr int SA_WORDS = SA_DIGITS + 1;
-// Each wave of a deposit workgroup stores its sum as a window of SA_WAVE_WIN
-// counters from a base digit (SA_WAVE_WORDS words: base, window; base -1:
-// empty); digits outside the window go to the box's counters directly.
-// k_mass_reduce adds the windows up per box.
+// Each wave of a binary64 deposit workgroup stores its sum as a window of
+// SA_WAVE_WIN counters from a base digit (SA_WAVE_WORDS words: base, window;
+// base -1: empty); digits outside the window go to the box's counters
+// directly.  k_mass_reduce adds the windows up per box.  (Binary32 waves store
+// the two digit sums of their fixed-point cells; k_mass_reduce_fixed.)
 constexpr int SA_WAVE_WIN = 8;
 constexpr int SA_WAVE_WORDS = SA_WAVE_WIN + 1;
 constexpr int SA_WAVES_PER_TILE = 8;  // deposit workgroups of 512 threads
@@ -700,7 +701,8 @@ struct DepParams {
     int direct;               // 1: (m - mean) * inv_volume ; 0: fma-centred path
     int raw;                  // 1: write the deposited sum itself (adjoint deposits of signed values)
     unsigned long long* mass_digits;  // per box SA_WORDS counters of the exact deposited mass
-    unsigned long long* mass_waves;   // per tile and wave SA_WAVE_WORDS: its sum's digit window
+    unsigned long long* mass_waves;   // per tile and wave: binary64 its digit window (SA_WAVE_WORDS),
+                                      // binary32 the two digit sums of its cells
     uint64_t* scratch;        // global fallback keys
     uint64_t scratch_cap;
     unsigned long long* scratch_cursor;

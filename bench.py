@@ -70,7 +70,7 @@ def step_alg_bytes(n_p: int, N: int, real: int) -> float:
 
 
 STAGE_KERNEL = {"bucket": "hy::k_cap_apply", "deposit": "void hy::k_deposit_fixed<16>",
-                "gather_kdk": "void hy::k_gather_kdk<float, float, true, true>"}
+                "gather_kdk": "void hy::k_gather_kdk<float, float, true, true, false>"}
 FFT_KERNEL = {"fft_z_r2c": "k_fft_z_r2c", "fft_y_fwd": "k_fft_y_fwd", "fft_x_green": "k_fft_x_green",
               "fft_y_bwd": "k_fft_y_bwd", "fft_z_c2r": "k_fft_z_c2r"}
 

@@ -2114,10 +2114,19 @@ struct hy_ensemble {
     double* dfac = nullptr;  // k_prev, k_next, d_next for the next replay
     hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
     bool fresh = false;  // next step re-keys the members eagerly (positions were replaced)
+    // capture only: each member's step is recorded on its own stream, forked
+    // from and joined back to `stream`, so the graph runs the members'
+    // independent kernel chains side by side
+    std::vector<hipStream_t> branch;
+    std::vector<hipEvent_t> joined;
+    hipEvent_t forked = nullptr;
     ~hy_ensemble() {
         for (hipGraphExec_t g : exec)
             if (g) (void)hipGraphExecDestroy(g);
         if (dfac) (void)hipFree(dfac);
+        for (hipEvent_t ev : joined) (void)hipEventDestroy(ev);
+        if (forked) (void)hipEventDestroy(forked);
+        for (hipStream_t s : branch) (void)hipStreamDestroy(s);
     }
 };
 
@@ -2144,6 +2153,15 @@ int hy_ensemble_create(const hy_ensemble_member* members, uint32_t count, hy_ens
         hip_check(hipSetDevice(e->device), "hipSetDevice");
         hip_check(hipMalloc(&e->dfac, 4 * sizeof(double)), "hipMalloc");
         hip_check(hipMemset(e->dfac, 0, 4 * sizeof(double)), "hipMemset");
+        hip_check(hipEventCreateWithFlags(&e->forked, hipEventDisableTiming), "hipEventCreate");
+        for (uint32_t i = 0; i < count; ++i) {
+            hipStream_t s = nullptr;
+            hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");  // default priority
+            e->branch.push_back(s);
+            hipEvent_t ev = nullptr;
+            hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+            e->joined.push_back(ev);
+        }
         *out = e.release();
     });
 }
@@ -2198,16 +2216,28 @@ int hy_ensemble_step(hy_ensemble* e, double k_prev, double k_next, double d_next
             hip_check(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
             int rc = HY_OK;
             std::string msg;
-            for (const hy_ensemble_member& m : e->members) {
+            hipError_t eb = hipEventRecord(e->forked, e->stream);
+            for (size_t i = 0; i < e->members.size() && eb == hipSuccess; ++i) {
+                const hy_ensemble_member& m = e->members[i];
+                eb = hipStreamWaitEvent(e->branch[i], e->forked, 0);
+                if (eb != hipSuccess) break;
+                m.ctx->stream = e->branch[i];
                 rc = hy_pm_kdk(m.ctx, m.n, m.x, m.y, m.z, m.px, m.py, m.pz, m.x, m.y, m.z, m.px, m.py, m.pz,
                                nullptr, m.uniform_mass, 0.0, 0.0, 0.0, nullptr, nullptr, nullptr, kflags);
+                m.ctx->stream = e->stream;
                 if (rc != HY_OK) {
                     msg = g_last_error;
                     break;
                 }
+                eb = hipEventRecord(e->joined[i], e->branch[i]);
+                if (eb == hipSuccess) eb = hipStreamWaitEvent(e->stream, e->joined[i], 0);
             }
             hipGraph_t graph = nullptr;
             const hipError_t ec = hipStreamEndCapture(e->stream, &graph);
+            if (rc == HY_OK && eb != hipSuccess) {
+                rc = HY_ERUNTIME;
+                msg = std::string("ensemble capture fork/join: ") + hipGetErrorString(eb);
+            }
             for (size_t i = 0; i < e->members.size(); ++i) {
                 e->members[i].ctx->prof = prof[i];
                 e->members[i].ctx->dfac = nullptr;

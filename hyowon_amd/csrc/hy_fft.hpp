@@ -258,6 +258,38 @@ HY_HD void tl_twiddle_apply(C* v, const C* w) {
     }
 }
 
+// Level-1 twiddles W^(col k1), k1 = 1 .. A-1, as the products
+// W^(col j) W^(col L h) (k1 = j + L h) of L - 1 + H - 1 table entries instead
+// of A - 1: for A = 32 a row holds 10 complex factors in registers rather
+// than 31 (the registers that bound the passes' occupancy), at one rounding
+// more per twiddle.
+template <int A>
+struct TwSplit {
+    static constexpr int L = A >= 16 ? 8 : A;
+    static constexpr int H = (A + L - 1) / L;
+};
+
+template <int A, int LEN, typename C>
+HY_HD void tl_twiddle_load_split(C* lo, C* hi, int col, const C* tw, int stride) {
+    constexpr int L = TwSplit<A>::L, H = TwSplit<A>::H;
+#pragma unroll
+    for (int j = 1; j < L; ++j) lo[j] = tw[((col * j) % LEN) * stride];
+#pragma unroll
+    for (int h = 1; h < H; ++h) hi[h] = tw[((col * L * h) % LEN) * stride];
+}
+
+template <int A, int S, typename C>
+HY_HD void tl_twiddle_apply_split(C* v, const C* lo, const C* hi) {
+    constexpr int L = TwSplit<A>::L;
+#pragma unroll
+    for (int k1 = 1; k1 < A; ++k1) {
+        const int j = k1 % L, h = k1 / L;
+        C x = h == 0 ? lo[j] : (j == 0 ? hi[h] : cmul(lo[j], hi[h]));
+        if constexpr (S > 0) x.y = -x.y;
+        v[k1] = cmul(v[k1], x);
+    }
+}
+
 // The table loads go out in batches of TB ahead of their multiplies (a
 // scheduling barrier keeps the compiler from interleaving them with the
 // uses), so a row pays (A - 1) / TB memory round trips for its twiddles rather

@@ -647,11 +647,11 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
         }
         // backward transform of the held stride-A column (A and B exchanged);
         // its level-1 twiddles load while the length-B DFT runs
-        C w2[B];
-        tl_twiddle_load<B, LEN>(w2, t, a.tw, 1);
+        C w2l[TwSplit<B>::L], w2h[TwSplit<B>::H];
+        tl_twiddle_load_split<B, LEN>(w2l, w2h, t, a.tw, 1);
         __builtin_amdgcn_sched_barrier(0);
         dft_reg<B, +1>(v);
-        tl_twiddle_apply<B, +1>(v, w2);
+        tl_twiddle_apply_split<B, +1>(v, w2l, w2h);
 #pragma unroll
         for (int k1 = 0; k1 < B; ++k1) xr[k1 * (A + 1) + t] = v[k1];
     }
@@ -725,8 +725,8 @@ __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_bwd(FftAr
     const int xb = b % (N / ROWS), kz = a.kz0 + (int)(b / (N / ROWS));
     const int x0 = xb * ROWS;
     const C* src = a.B + (uint64_t)kz * N * N + x0;
-    C w[A];
-    if (t < B) tl_twiddle_load<A, LEN>(w, t, a.tw, 1);
+    C wl[TwSplit<A>::L], wh[TwSplit<A>::H];
+    if (t < B) tl_twiddle_load_split<A, LEN>(wl, wh, t, a.tw, 1);
     if constexpr (sizeof(T) == 4 && !SG && ROWS % 2 == 0 && (ROWS * N / 2) % NT == 0) {
         constexpr int PER = ROWS * N / 2 / NT;
         F4 q[PER];
@@ -760,7 +760,7 @@ __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_bwd(FftAr
     __syncthreads();
     if (t < B) {
         dft_reg<A, +1>(v);
-        tl_twiddle_apply<A, +1>(v, w);
+        tl_twiddle_apply_split<A, +1>(v, wl, wh);
 #pragma unroll
         for (int k1 = 0; k1 < A; ++k1) xr[k1 * (B + 1) + t] = v[k1];
     }
@@ -796,11 +796,11 @@ __global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_c2r(F
     const C* src = a.A + (uint64_t)x * NH * N + y0;
     // the untangle twiddles W^k of this thread's column (k = t + B n1) and its
     // level-1 twiddles, in flight with the row loads
-    C wu[A], w[A];
+    C wu[A], wl[TwSplit<A>::L], wh[TwSplit<A>::H];
     if (t < B) {
 #pragma unroll
         for (int n1 = 0; n1 < A; ++n1) wu[n1] = a.tw[t + B * n1];
-        tl_twiddle_load<A, LEN>(w, t, a.tw, 2);
+        tl_twiddle_load_split<A, LEN>(wl, wh, t, a.tw, 2);
     }
     const auto put = [&](int l, C val) {
         const int rr = l % ROWS, k = l / ROWS;
@@ -852,7 +852,7 @@ __global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_c2r(F
     __syncthreads();
     if (t < B) {
         dft_reg<A, +1>(v);
-        tl_twiddle_apply<A, +1>(v, w);
+        tl_twiddle_apply_split<A, +1>(v, wl, wh);
 #pragma unroll
         for (int k1 = 0; k1 < A; ++k1) xr[k1 * (B + 1) + t] = v[k1];
     }

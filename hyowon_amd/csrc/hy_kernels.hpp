@@ -787,6 +787,52 @@ __device__ __forceinline__ void deposit_cells(const DepParams& P, T* __restrict_
                 cur[q] = start[cell];
                 end[q] = start[cell + 1];
             }
+            if constexpr (LDS_PAYLOAD) {
+                // at most one record per source cell (the common case): order the
+                // up to four contributions by particle index with a sorting network
+                // and add them in that order, with every load issued up front
+                bool simple = true;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) simple = simple && end[q] - cur[q] <= 1u;
+                if (simple) {
+                    uint32_t idx[4], slot[4];
+                    int src[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const bool have = cur[q] < end[q];
+                        const uint64_t k = have ? keys[cur[q]] : 0ull;
+                        idx[q] = have ? key_index(k) : 0xffffffffu;  // empty lists sort last
+                        slot[q] = key_slot(k);
+                        src[q] = q;
+                    }
+                    auto cswap = [&](int i, int j) {
+                        if (idx[j] < idx[i]) {
+                            const uint32_t ti = idx[i], ts = slot[i];
+                            const int tq = src[i];
+                            idx[i] = idx[j]; slot[i] = slot[j]; src[i] = src[j];
+                            idx[j] = ti; slot[j] = ts; src[j] = tq;
+                        }
+                    };
+                    cswap(0, 1); cswap(2, 3); cswap(0, 2); cswap(1, 3); cswap(1, 2);
+                    T pfx[4], pfy[4], pfz[4], pm[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const uint32_t sl = idx[i] != 0xffffffffu ? slot[i] : 0u;
+                        pfx[i] = fx[sl]; pfy[i] = fy[sl]; pfz[i] = fz[sl];
+                        pm[i] = P.masses ? fm[sl] : (T)P.uniform_mass;
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        if (idx[i] == 0xffffffffu) break;
+                        const int oy = src[i] >> 1, oz = src[i] & 1;
+                        const T xw = phase == 0 ? (T)1 - pfx[i] : pfx[i];
+                        const T wy = oy ? pfy[i] : (T)1 - pfy[i];
+                        const T wz = oz ? pfz[i] : (T)1 - pfz[i];
+                        sum += pm[i] * xw * wy * wz;
+                    }
+                    continue;
+                }
+            }
             // merge the four source cells by particle index
             while (true) {
                 int best = -1;

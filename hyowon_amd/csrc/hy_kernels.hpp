@@ -790,7 +790,7 @@ __device__ __forceinline__ void deposit_cells(const DepParams& P, T* __restrict_
             if constexpr (LDS_PAYLOAD) {
                 // at most one record per source cell (the common case): order the
                 // up to four contributions by particle index with a sorting network
-                // and add them in that order, with every load issued up front
+                // and add them in that order
                 bool simple = true;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) simple = simple && end[q] - cur[q] <= 1u;
@@ -814,21 +814,17 @@ __device__ __forceinline__ void deposit_cells(const DepParams& P, T* __restrict_
                         }
                     };
                     cswap(0, 1); cswap(2, 3); cswap(0, 2); cswap(1, 3); cswap(1, 2);
-                    T pfx[4], pfy[4], pfz[4], pm[4];
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        const uint32_t sl = idx[i] != 0xffffffffu ? slot[i] : 0u;
-                        pfx[i] = fx[sl]; pfy[i] = fy[sl]; pfz[i] = fz[sl];
-                        pm[i] = P.masses ? fm[sl] : (T)P.uniform_mass;
-                    }
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        if (idx[i] == 0xffffffffu) break;
+                        if (idx[i] == 0xffffffffu) break;  // the rest are empty too
+                        const uint32_t sl = slot[i];
+                        const T ffx = fx[sl], ffy = fy[sl], ffz = fz[sl];
+                        const T mm = P.masses ? fm[sl] : (T)P.uniform_mass;
                         const int oy = src[i] >> 1, oz = src[i] & 1;
-                        const T xw = phase == 0 ? (T)1 - pfx[i] : pfx[i];
-                        const T wy = oy ? pfy[i] : (T)1 - pfy[i];
-                        const T wz = oz ? pfz[i] : (T)1 - pfz[i];
-                        sum += pm[i] * xw * wy * wz;
+                        const T xw = phase == 0 ? (T)1 - ffx : ffx;
+                        const T wy = oy ? ffy : (T)1 - ffy;
+                        const T wz = oz ? ffz : (T)1 - ffz;
+                        sum += mm * xw * wy * wz;
                     }
                     continue;
                 }

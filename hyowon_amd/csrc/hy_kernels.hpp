@@ -552,6 +552,21 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     return s;
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave (every lane active): the
+// DPP row shifts give each lane its 16-lane row's prefix, then the totals of
+// the rows below its own (read as scalars) are added.
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
+    v += dpp_u32<0x111>(v, 0u);  // row_shr:1
+    v += dpp_u32<0x112>(v, 0u);  // row_shr:2
+    v += dpp_u32<0x114>(v, 0u);  // row_shr:4
+    v += dpp_u32<0x118>(v, 0u);  // row_shr:8
+    const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 15);
+    const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 31);
+    const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 47);
+    const uint32_t row = lane_id() >> 4;
+    return v + (row > 0 ? r0 : 0u) + (row > 1 ? r1 : 0u) + (row > 2 ? r2 : 0u);
+}
+
 __device__ __forceinline__ int wave_min_i32(int v) {
     constexpr uint32_t big = 0x7fffffffu;
     v = min(v, (int)dpp_u32<0x111>((uint32_t)v, big));
@@ -902,7 +917,7 @@ __device__ __forceinline__ void load_source_ranges(const uint32_t* __restrict__ 
                                                    int tz, int nT, uint32_t* rs, uint32_t* rc,
                                                    uint8_t* rd, uint8_t* rcls) {
     const int t = threadIdx.x;
-    if (t < 64) {  // the first wave: one lane per range, then a shuffle scan of the counts
+    if (t < 64) {  // the first wave: one lane per range, then a wave scan of the counts
         uint32_t cnt = 0;
         if (t < 20) {
             const RangeSlot sl = range_slot(t);
@@ -918,12 +933,7 @@ __device__ __forceinline__ void load_source_ranges(const uint32_t* __restrict__ 
             rd[t] = (uint8_t)d;
             rcls[t] = sl.c;
         }
-        uint32_t incl = cnt;
-#pragma unroll
-        for (int o = 1; o < 32; o <<= 1) {
-            const uint32_t y = __shfl_up(incl, o);
-            if (t >= o) incl += y;
-        }
+        const uint32_t incl = wave_incl_scan_u32(cnt);
         if (t < 20) rc[t + 1] = incl;
         if (t == 0) rc[0] = 0;
     }
@@ -931,7 +941,7 @@ __device__ __forceinline__ void load_source_ranges(const uint32_t* __restrict__ 
 }
 
 // Exclusive scan of a[0..count) in LDS, in place, by all threads of the
-// workgroup (each thread owns a contiguous chunk): a shuffle scan inside each
+// workgroup (each thread owns a contiguous chunk): a scan inside each
 // wavefront, then the wavefront totals (part: one word per wavefront).
 __device__ __forceinline__ void wg_exclusive_scan_lds(uint32_t* a, int count, uint32_t* part) {
     const int nt = (int)blockDim.x;
@@ -941,12 +951,7 @@ __device__ __forceinline__ void wg_exclusive_scan_lds(uint32_t* a, int count, ui
     uint32_t s = 0;
     for (int i = b; i < e; ++i) s += a[i];
     const int lane = (int)lane_id(), wave = (int)threadIdx.x / 64;
-    uint32_t incl = s;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(incl, o);
-        if (lane >= o) incl += y;
-    }
+    const uint32_t incl = wave_incl_scan_u32(s);
     if (lane == 63) part[wave] = incl;
     __syncthreads();
     uint32_t run = incl - s;
@@ -1170,12 +1175,7 @@ __device__ __forceinline__ void load_fx_ranges(const DepFxParams& Q, uint32_t bo
         const uint64_t live = __ballot(cnt != 0u);
         const uint32_t at = (uint32_t)__popcll(live & ((1ull << t) - 1ull));
         const uint32_t R = (uint32_t)__popcll(live);
-        uint32_t incl = cnt;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(incl, o);
-            if (t >= o) incl += y;
-        }
+        const uint32_t incl = wave_incl_scan_u32(cnt);
         if (cnt != 0u) {
             rs[at] = rs_lane;
             rd[at] = rd_lane;

@@ -58,7 +58,7 @@ __device__ __forceinline__ uint32_t aggregated_rank(uint32_t* counts, uint32_t k
     uint64_t my_mask = 0;
     while (remaining) {
         const int leader = __ffsll((unsigned long long)remaining) - 1;
-        const uint32_t lead_key = __shfl(key, leader);
+        const uint32_t lead_key = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);  // uniform lane: a scalar read
         const uint64_t mask = __ballot(valid && key == lead_key) & remaining;
         if ((mask >> lane) & 1ull) {
             my_leader = leader;
@@ -1181,7 +1181,7 @@ __device__ __forceinline__ void load_fx_ranges(const DepFxParams& Q, uint32_t bo
             rd[at] = rd_lane;
             rc[at + 1] = incl;
         }
-        const uint32_t total = __shfl(incl, 63);
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
         if (t == 0) rc[0] = 0;
         if ((uint32_t)t > R && t <= DEPFX_NRANGES) rc[t] = total;
         if (t == 0) *nranges = R;
@@ -1839,7 +1839,7 @@ __global__ void __launch_bounds__(256) k_gather_kdk(GatherArgs<T, PT> a) {
             if (om) {  // one atomic per wave for its overflow records
                 const int leader = __ffsll((unsigned long long)om) - 1;
                 if ((int)lane_id() == leader) obase = atomicAdd(a.ovf_count, (uint32_t)__popcll(om));
-                obase = __shfl(obase, leader);
+                obase = (uint32_t)__builtin_amdgcn_readlane((int)obase, leader);
             }
             if (have) {
                 if (!over) {

@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(64) k_xp_scatter(uint64_t n, const double* __r
         uint32_t slot = 0;
         while (remaining) {
             const int leader = __ffsll((unsigned long long)remaining) - 1;
-            const int key = __shfl(bx, leader);
+            const int key = __builtin_amdgcn_readlane(bx, leader);
             const uint64_t mask = __ballot(ok && bx == key) & remaining;
             if (ok && bx == key) slot = next[key] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
             __syncthreads();

@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(PK_BLOCK) k_pk_bin(PkArgs a) {
                 unsigned long long pending = __ballot(shell >= 0);
                 while (pending) {
                     const int leader = __ffsll((long long)pending) - 1;
-                    const int s = __shfl(shell, leader);
+                    const int s = __builtin_amdgcn_readlane(shell, leader);
                     const bool mine = shell == s;
                     double t[PK_Q];
 #pragma unroll

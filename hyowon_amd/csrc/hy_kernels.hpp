@@ -583,8 +583,8 @@ __device__ __forceinline__ int wave_min_i32(int v) {
 // units of the binary position 32*base (base < 0: empty), base one digit
 // below the first mass it took.  A mass m 2^s (s its bit position, 53-bit m)
 // is added exactly when 0 <= s - 32 base <= 127; others go to the counters d
-// directly.  sa_acc_flush reduces the wave's sums with shuffles and adds them
-// to d from one lane.
+// directly.  sa_acc_flush reduces the wave's sums (DPP wave sums) into the
+// wave's digit window.
 struct SaAcc {
     int base = -1;
     uint64_t a0 = 0, a1 = 0, a2 = 0;

@@ -177,7 +177,6 @@ struct hy_pm_ctx {
     void* ic_kc = nullptr;       // binary64 k tables of the 1LPT displacements (allocated on first use)
     void* ic_kd = nullptr;
     void* pk_aux = nullptr;      // interlaced partner spectrum (allocated on first use)
-    double* pk_pos = nullptr;    // shifted positions, 3 * cap (allocated on first use)
     double* pk_tab = nullptr;    // k components, windows, alias factors (3N), shell edges
     double* pk_part = nullptr;   // workgroup shell partials
     uint64_t pk_part_cap = 0;    // doubles held by pk_part
@@ -333,11 +332,11 @@ void cap_scan(hy_pm_ctx* c, const uint32_t* cur, uint32_t* dense, uint32_t* capn
 // step's carried records from these counts.
 void bucketize(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z, int mode,
                uint32_t* counts, uint32_t* offsets, uint64_t nb, uint32_t* order, bool carried = false,
-               const double* masses = nullptr, uint32_t* capn = nullptr) {
+               const double* masses = nullptr, uint32_t* capn = nullptr, double shift = 0.0) {
     if (!carried) {
         fill_u32(c, counts, nb);
         hipLaunchKernelGGL(hy::k_bucket_key, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, x, y, z,
-                           c->geom, mode, c->keys, c->ranks, counts, c->err);
+                           c->geom, mode, c->keys, c->ranks, counts, c->err, shift);
         launch_check("k_bucket_key");
     }
     if (mode == 0 && c->prec == HY_PREC_FP32_MIXED && c->rec_layout) {
@@ -355,12 +354,12 @@ void bucketize(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const
     if (mode == 0 && c->prec == HY_PREC_FP32_MIXED) {
         hipLaunchKernelGGL((hy::k_scatter_records<hy::DepRecord, float>), dim3(blocks_for(n, 256)), dim3(256), 0,
                            c->stream, n, x, y, z, masses, 1.0 / c->mass_unit, c->geom, c->keys, c->ranks, offsets,
-                           order, c->rec, c->rec_mass, c->err);
+                           order, c->rec, c->rec_mass, c->err, shift);
         launch_check("k_scatter_records");
     } else if (mode == 0) {
         hipLaunchKernelGGL((hy::k_scatter_records<hy::DepRecord64, double>), dim3(blocks_for(n, 256)), dim3(256), 0,
                            c->stream, n, x, y, z, masses, 1.0, c->geom, c->keys, c->ranks, offsets, order,
-                           c->rec64, c->rec_mass64, c->err);
+                           c->rec64, c->rec_mass64, c->err, shift);
         launch_check("k_scatter_records");
     } else if (carried) {
         hipLaunchKernelGGL(hy::k_scatter_checked, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, x, y,
@@ -393,7 +392,7 @@ struct DepositPlan {
 };
 
 DepositPlan plan_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z,
-                         const double* masses, double uniform, double Min) {
+                         const double* masses, double uniform, double Min, double shift = 0.0) {
     const double dx = c->geom.dx;
     const double cells = static_cast<double>(static_cast<uint64_t>(c->N) * c->N * c->N);
     const double mean = Min / cells;
@@ -404,6 +403,7 @@ DepositPlan plan_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double
     P.x = x; P.y = y; P.z = z;
     P.masses = masses;
     P.uniform_mass = uniform;
+    P.pos_shift = shift;
     P.offsets = c->offsets;
     P.order = c->order;
     P.rec64 = c->rec64;
@@ -713,9 +713,11 @@ void check_mass(hy_pm_ctx* c, double input, double deposited, uint64_t n) {
 // carried_rec (binary32, asynchronous): deposit the records the previous
 // fused step wrote into its capacity layout for these positions.  rec_out:
 // also lay out the capacities of the records this step's gather writes.
+// shift: deposit the positions x - shift (the power spectrum's interlacing
+// partner), routed and wrapped as the reference's shifted copy would be.
 void run_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, const double* z,
                  const double* masses, double uniform, double Min, bool sync, bool carried = false,
-                 bool carried_rec = false, bool rec_out = false) {
+                 bool carried_rec = false, bool rec_out = false, double shift = 0.0) {
     mark(c, 0);
     // the binary32 deposit reads records only: no particle order is written
     uint32_t* order = nullptr;
@@ -736,10 +738,10 @@ void run_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, con
         start = c->cap_cur;
         ovf = c->ovf_start;
     } else {
-        bucketize(c, n, x, y, z, 0, c->counts, c->offsets, c->nball, order, carried, masses, capn);
+        bucketize(c, n, x, y, z, 0, c->counts, c->offsets, c->nball, order, carried, masses, capn, shift);
     }
     mark(c, HY_STAGE_BUCKET + 1);
-    auto plan = plan_deposit(c, n, x, y, z, masses, uniform, Min);
+    auto plan = plan_deposit(c, n, x, y, z, masses, uniform, Min, shift);
     launch_deposit(c, plan.P, true, start, ovf);
     mark(c, HY_STAGE_DEPOSIT + 1);
     c->last_input_mass = Min;
@@ -752,8 +754,8 @@ void run_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, con
             raise_errors(st.err & (hy::ERR_POSITION_NONFINITE | hy::ERR_MASS_INVALID));
             carried = false;
             reset_status(c);
-            bucketize(c, n, x, y, z, 0, c->counts, c->offsets, c->nball, order, false, masses);
-            plan = plan_deposit(c, n, x, y, z, masses, uniform, Min);
+            bucketize(c, n, x, y, z, 0, c->counts, c->offsets, c->nball, order, false, masses, nullptr, shift);
+            plan = plan_deposit(c, n, x, y, z, masses, uniform, Min, shift);
             launch_deposit(c, plan.P);
             continue;
         }
@@ -769,7 +771,7 @@ void run_deposit(hy_pm_ctx* c, uint64_t n, const double* x, const double* y, con
             c->scratch = static_cast<uint64_t*>(p);
             c->scratch_cap = grow;
             reset_status(c);
-            plan = plan_deposit(c, n, x, y, z, masses, uniform, Min);
+            plan = plan_deposit(c, n, x, y, z, masses, uniform, Min, shift);
             launch_deposit(c, plan.P);
             continue;
         }
@@ -1981,14 +1983,11 @@ int hy_pm_power_spectrum(hy_pm_ctx* c, uint64_t n, const double* x, const double
             }
         }
         const uint64_t ccount = (uint64_t)N * (uint64_t)(N / 2 + 1) * (uint64_t)N;
-        if (interlaced && !c->pk_aux) {
-            c->pk_aux = c->alloc(ccount * 2 * c->real_bytes);
-            c->pk_pos = static_cast<double*>(c->alloc(c->cap * 3 * sizeof(double)));
-        }
-        auto deposit = [&](const double* dx_, const double* dy_, const double* dz_) {
+        if (interlaced && !c->pk_aux) c->pk_aux = c->alloc(ccount * 2 * c->real_bytes);
+        auto deposit = [&](double shift) {
             reset_status(c);
             try {
-                run_deposit(c, n, dx_, dy_, dz_, pm, uniform, M, true);
+                run_deposit(c, n, px, py, pz, pm, uniform, M, true, false, false, false, shift);
             } catch (const HyError& e) {
                 if (e.code == HY_EINVAL && e.what == kMsgPositions)
                     fail(HY_EINVAL, "FourierDensityBuilder particle positions must be finite");
@@ -1996,15 +1995,13 @@ int hy_pm_power_spectrum(hy_pm_ctx* c, uint64_t n, const double* x, const double
             }
         };
         // contrast transforms (src/analysis/fourier_density_field.cpp:230-336)
-        deposit(px, py, pz);
+        deposit(0.0);
         run_forward(c, c->B);
         const double shift = 0.5 * dx;
         if (interlaced) {
-            double* s = c->pk_pos;
-            hipLaunchKernelGGL(hy::k_pk_shift, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, px, py, pz,
-                               shift, L, s, s + c->cap, s + 2 * c->cap);
-            launch_check("k_pk_shift");
-            deposit(s, s + c->cap, s + 2 * c->cap);
+            // the partner positions wrap(x - dx/2) are formed inside the routing
+            // and deposit kernels (src/analysis/fourier_density_field.cpp:300-311)
+            deposit(shift);
             run_forward(c, c->pk_aux);
         }
 

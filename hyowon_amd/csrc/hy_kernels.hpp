@@ -73,10 +73,12 @@ __device__ __forceinline__ uint32_t aggregated_rank(uint32_t* counts, uint32_t k
 }
 
 // mode 0: deposit buckets; mode 1: reference x-plane bins (key = base x).
+// shift: positions are routed as x - shift (the interlacing partner of the
+// power-spectrum estimator; 0 otherwise, where x - 0 is x exactly).
 __global__ void k_bucket_key(uint64_t n, const double* __restrict__ x, const double* __restrict__ y,
                              const double* __restrict__ z, Geom g, int mode,
                              uint32_t* __restrict__ keys, uint32_t* __restrict__ ranks,
-                             uint32_t* __restrict__ counts, uint32_t* __restrict__ err) {
+                             uint32_t* __restrict__ counts, uint32_t* __restrict__ err, double shift) {
     const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = p < n;
     uint32_t key = 0;
@@ -84,8 +86,8 @@ __global__ void k_bucket_key(uint64_t n, const double* __restrict__ x, const dou
     if (in) {
         int bx, by = 0, bz = 0;
         double f;
-        ok = cic_axis(x[p], g, bx, f);
-        if (mode == 0) ok = ok && cic_axis(y[p], g, by, f) && cic_axis(z[p], g, bz, f);
+        ok = cic_axis(x[p] - shift, g, bx, f);
+        if (mode == 0) ok = ok && cic_axis(y[p] - shift, g, by, f) && cic_axis(z[p] - shift, g, bz, f);
         if (ok) key = mode == 0 ? box_of(p, g) * buckets_per_box(g) + bucket_of(bx, by, bz, g) : (uint32_t)bx;
         else atomicOr(err, ERR_POSITION_NONFINITE);
     }
@@ -370,7 +372,7 @@ __global__ void k_scatter_records(uint64_t n, const double* __restrict__ x, cons
                                   double inv_mass_unit, Geom g, const uint32_t* __restrict__ keys,
                                   const uint32_t* __restrict__ ranks, const uint32_t* __restrict__ offsets,
                                   uint32_t* __restrict__ order, R* __restrict__ rec,
-                                  MT* __restrict__ rec_mass, uint32_t* __restrict__ err) {
+                                  MT* __restrict__ rec_mass, uint32_t* __restrict__ err, double shift) {
     const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const uint32_t k = keys[p];
@@ -378,7 +380,8 @@ __global__ void k_scatter_records(uint64_t n, const double* __restrict__ x, cons
     double fx, fy, fz;
     // The coordinate loads stay behind the short-circuit checks: issued
     // together (cic_axes), the three streams measured 8% slower here.
-    const bool ok = cic_axis(x[p], g, bx, fx) && cic_axis(y[p], g, by, fy) && cic_axis(z[p], g, bz, fz);
+    const bool ok = cic_axis(x[p] - shift, g, bx, fx) && cic_axis(y[p] - shift, g, by, fy) &&
+                    cic_axis(z[p] - shift, g, bz, fz);
     if (!ok) {
         atomicOr(err, ERR_POSITION_NONFINITE);
         return;
@@ -701,6 +704,7 @@ struct DepParams {
     const double* z;
     const double* masses;   // nullptr -> uniform
     double uniform_mass;
+    double pos_shift;         // positions are deposited as x - pos_shift (0: as given)
     const uint32_t* offsets;  // bucket offsets (nb+1)
     const uint32_t* order;    // bucketed particle indices
     const DepRecord64* rec64; // binary64 deposit records in bucket order
@@ -866,9 +870,9 @@ __device__ __forceinline__ void deposit_cells(const DepParams& P, T* __restrict_
                 } else {
                     const uint32_t pidx = key_index(k);
                     int b; double f;
-                    cic_axis(P.x[pidx], g, b, f); ffx = (T)f;
-                    cic_axis(P.y[pidx], g, b, f); ffy = (T)f;
-                    cic_axis(P.z[pidx], g, b, f); ffz = (T)f;
+                    cic_axis(P.x[pidx] - P.pos_shift, g, b, f); ffx = (T)f;
+                    cic_axis(P.y[pidx] - P.pos_shift, g, b, f); ffy = (T)f;
+                    cic_axis(P.z[pidx] - P.pos_shift, g, b, f); ffz = (T)f;
                     mm = P.masses ? (T)P.masses[pidx] : (T)P.uniform_mass;
                 }
                 const int oy = best >> 1, oz = best & 1;

@@ -200,17 +200,4 @@ __global__ void __launch_bounds__(256) k_pk_reduce(const double* __restrict__ pa
     if (threadIdx.x == 0) out[i] = red[0];
 }
 
-// Interlacing partner positions: wrap(x - shift, L)
-// (src/analysis/fourier_density_field.cpp:300-311).
-__global__ void __launch_bounds__(256) k_pk_shift(uint64_t n, const double* __restrict__ x,
-                                                  const double* __restrict__ y, const double* __restrict__ z,
-                                                  double shift, double L, double* __restrict__ sx,
-                                                  double* __restrict__ sy, double* __restrict__ sz) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    sx[i] = wrap_pos(x[i] - shift, L);
-    sy[i] = wrap_pos(y[i] - shift, L);
-    sz[i] = wrap_pos(z[i] - shift, L);
-}
-
 }  // namespace hy

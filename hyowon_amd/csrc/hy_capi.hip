@@ -557,6 +557,9 @@ void run_fft(hy_pm_ctx* c) {
 // the spectral-gradient passes (src/gravity/pm_solver.cpp:768-810):
 //   x: B2 = X^-1(-i k_x G phi), B = X^-1(G phi)
 //   F_x = Z^-1 Y^-1 B2 -> R,  F_y = Z^-1 Y^-1(-i k_y B) -> R2,  F_z = Z^-1(-i k_z Y^-1 B) -> R3
+// The dual passes (binary32 two-level meshes) form both x outputs from one
+// read of B, and Y^-1(-i k_y B) -> A with Y^-1 B -> B2 (free once F_x is
+// done) from one read of B.
 template <typename T>
 void launch_fft_sg(hy_pm_ctx* c, const hy::FftKernels<T>& K) {
     using C = hy::cplx<T>;
@@ -567,24 +570,41 @@ void launch_fft_sg(hy_pm_ctx* c, const hy::FftKernels<T>& K) {
     const uint64_t N = (uint64_t)c->N, NH = N / 2 + 1;
     launch_pass(c, K, a, hy::FFT_Z_R2C, c->fft_var[hy::FFT_Z_R2C], 0, (int)NH);
     launch_pass(c, K, a, hy::FFT_Y_FWD, c->fft_var[hy::FFT_Y_FWD], 0, (int)NH);
-    auto sg = [&](int which, const hy::FftArgs<T>& aa) {
-        const hy::FftPass<T>& P = K.sg[which];
+    auto launch = [&](const hy::FftPass<T>& P, int which, const hy::FftArgs<T>& aa) {
         const uint64_t blocks = which == 0 ? NH * N / (uint64_t)P.rows
                                            : (which == 1 ? NH * (N / (uint64_t)P.rows) : N * (N / (uint64_t)P.rows));
         hipLaunchKernelGGL(P.fn, dim3((uint32_t)blocks, 1), dim3(P.threads), P.lds, c->stream, aa);
         launch_check(which == 0 ? "k_fft_x_green (spectral gradient)"
                                 : (which == 1 ? "k_fft_y_bwd (spectral gradient)" : "k_fft_z_c2r (spectral gradient)"));
     };
+    auto sg = [&](int which, const hy::FftArgs<T>& aa) { launch(K.sg[which], which, aa); };
     a.kz0 = 0;
     hy::FftArgs<T> b = a;
-    b.deriv = 1;
     b.xout = static_cast<C*>(c->B2);
-    sg(0, b);
-    sg(0, a);
+    if (K.sg_x_dual) {
+        sg(0, b);  // -i k_x spectrum -> B2, plain -> B, one read of B
+    } else {
+        b.deriv = 1;
+        sg(0, b);
+        sg(0, a);
+    }
     b = a;
     b.B = static_cast<C*>(c->B2);
     sg(1, b);
     sg(2, b);  // F_x -> R
+    if (K.sg_y_dual.fn) {
+        b = a;
+        b.xout = static_cast<C*>(c->B2);
+        launch(K.sg_y_dual, 1, b);  // Y^-1(-i k_y B) -> A, Y^-1 B -> B2
+        b.R = static_cast<T*>(c->R2);
+        sg(2, b);  // F_y -> R2
+        b = a;
+        b.A = static_cast<C*>(c->B2);
+        b.deriv = 3;
+        b.R = static_cast<T*>(c->R3);
+        sg(2, b);  // F_z -> R3
+        return;
+    }
     b = a;
     b.deriv = 2;
     sg(1, b);

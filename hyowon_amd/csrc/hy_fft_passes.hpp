@@ -576,8 +576,10 @@ __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_fwd(FftAr
 
 // x pass: forward x FFT, Green multiply (with 1/N^3), backward x FFT, in place on B.
 // SG = 1: TreePM Green, optional -i k_x, output to xout when given.
-template <typename T, int N, int V, int SG = 0>
-__global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_tl_x_green(FftArgs<T> a) {
+// SG = 2: TreePM Green, two outputs from one read of the row: the inverse of
+// the -i k_x spectrum to xout, the inverse of the plain spectrum in place.
+template <typename T, int N, int V, int SG>
+__device__ __forceinline__ void tl_x_green_body(FftArgs<T> a) {
     fft_box(a);
     using C = cplx<T>;
     constexpr int LEN = N, NH = N / 2 + 1;
@@ -636,6 +638,9 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
         for (int n2 = 0; n2 < B; ++n2) v[n2] = xr[t * (B + 1) + n2];
     }
     __syncthreads();
+    // level-1 twiddles of the backward transform of the held stride-A column
+    // (A and B exchanged)
+    C w2l[TwSplit<B>::L], w2h[TwSplit<B>::H];
     if (t < A) {
         dft_reg<B, -1>(v);  // v[k2] = X[t + A k2]
 #pragma unroll
@@ -643,12 +648,35 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
             if constexpr (SG) f[k2] = green_factor_treepm(a, t + A * k2, ky, kz);
             v[k2].x *= f[k2];
             v[k2].y *= f[k2];
-            if (SG && a.deriv == 1) v[k2] = mul_mi(v[k2], a.kd[t + A * k2]);
+            if (SG == 1 && a.deriv == 1) v[k2] = mul_mi(v[k2], a.kd[t + A * k2]);
         }
-        // backward transform of the held stride-A column (A and B exchanged);
-        // its level-1 twiddles load while the length-B DFT runs
-        C w2l[TwSplit<B>::L], w2h[TwSplit<B>::H];
         tl_twiddle_load_split<B, LEN>(w2l, w2h, t, a.tw, 1);
+    }
+    if constexpr (SG == 2) {
+        // the -i k_x spectrum's inverse to xout; v keeps the plain spectrum
+        // for the in-place inverse below
+        C u[TL::EMAX];
+        if (t < A) {
+#pragma unroll
+            for (int k2 = 0; k2 < B; ++k2) u[k2] = mul_mi(v[k2], a.kd[t + A * k2]);
+            dft_reg<B, +1>(u);
+            tl_twiddle_apply_split<B, +1>(u, w2l, w2h);
+#pragma unroll
+            for (int k1 = 0; k1 < B; ++k1) xr[k1 * (A + 1) + t] = u[k1];
+        }
+        __syncthreads();
+        if (t < B) {
+#pragma unroll
+            for (int n = 0; n < A; ++n) u[n] = xr[t * (A + 1) + n];
+            dft_reg<A, +1>(u);
+            C* drow = a.xout + q * N;
+#pragma unroll
+            for (int k2 = 0; k2 < A; ++k2) st_stream(&drow[t + B * k2], u[k2]);
+        }
+        __syncthreads();
+    }
+    if (t < A) {
+        // the twiddle loads above complete while the length-B DFT runs
         __builtin_amdgcn_sched_barrier(0);
         dft_reg<B, +1>(v);
         tl_twiddle_apply_split<B, +1>(v, w2l, w2h);
@@ -660,10 +688,25 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
 #pragma unroll
         for (int n = 0; n < A; ++n) v[n] = xr[t * (A + 1) + n];
         dft_reg<A, +1>(v);
-        C* orow = (SG && a.xout) ? a.xout + q * N : row;
+        C* orow = (SG == 1 && a.xout) ? a.xout + q * N : row;
 #pragma unroll
         for (int k2 = 0; k2 < A; ++k2) st_stream(&orow[t + B * k2], v[k2]);
     }
+}
+
+template <typename T, int N, int V, int SG = 0>
+__global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_tl_x_green(FftArgs<T> a) {
+    tl_x_green_body<T, N, V, SG>(a);
+}
+
+// The SG = 2 pass holds two spectral columns per thread.  The LDS allows two
+// waves per SIMD; asked for them, the compiler fits the kernel in 256
+// registers with a few spilled (left alone it takes 272 and runs one wave per
+// SIMD, 1.3x slower at N = 1024).
+template <typename T, int N, int V>
+__global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) __attribute__((amdgpu_waves_per_eu(2)))
+k_tl_x_green_dual(FftArgs<T> a) {
+    tl_x_green_body<T, N, V, 2>(a);
 }
 
 // x pass without a Green multiply: forward (S = -1, spectral analysis) or
@@ -707,7 +750,9 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
 }
 
 // y pass backward: B[kz][ky][x] -> A rows over y through a transposing tile
-// (SG = 1: optional -i k_y on load).
+// (SG = 1: optional -i k_y on load; SG = 2: two outputs from one read of B,
+// the inverse of the -i k_y column to A and of the plain column to xout, in
+// the layout of A).
 template <typename T, int N, int V, int SG = 0>
 __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_bwd(FftArgs<T> a) {
     fft_box(a);
@@ -727,7 +772,9 @@ __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_bwd(FftAr
     const C* src = a.B + (uint64_t)kz * N * N + x0;
     C wl[TwSplit<A>::L], wh[TwSplit<A>::H];
     if (t < B) tl_twiddle_load_split<A, LEN>(wl, wh, t, a.tw, 1);
-    if constexpr (sizeof(T) == 4 && !SG && ROWS % 2 == 0 && (ROWS * N / 2) % NT == 0) {
+    if constexpr (sizeof(T) == 4 && ROWS % 2 == 0 && (ROWS * N / 2) % NT == 0) {
+        // 16-byte loads of two adjacent x of one ky row (ROWS even: a pair
+        // never straddles rows, so one -i k_y factor serves both)
         constexpr int PER = ROWS * N / 2 / NT;
         F4 q[PER];
 #pragma unroll
@@ -738,15 +785,21 @@ __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_bwd(FftAr
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int l = 2 * (tid + i * NT);
-            lds[(l / ROWS) * TS + (l % ROWS)] = C{q[i].x, q[i].y};
-            lds[(l / ROWS) * TS + (l % ROWS) + 1] = C{q[i].z, q[i].w};
+            C c0{q[i].x, q[i].y}, c1{q[i].z, q[i].w};
+            if (SG == 1 && a.deriv == 2) {
+                const T kk = a.kd[l / ROWS];
+                c0 = mul_mi(c0, kk);
+                c1 = mul_mi(c1, kk);
+            }
+            lds[(l / ROWS) * TS + (l % ROWS)] = c0;
+            lds[(l / ROWS) * TS + (l % ROWS) + 1] = c1;
         }
     } else {
         staged_copy<ROWS * N, NT, C>(
             tid,
             [&](int l) {
                 C val = src[(uint64_t)(l / ROWS) * N + (l % ROWS)];
-                if (SG && a.deriv == 2) val = mul_mi(val, a.kd[l / ROWS]);
+                if (SG == 1 && a.deriv == 2) val = mul_mi(val, a.kd[l / ROWS]);
                 return val;
             },
             [&](int l, C val) { lds[(l / ROWS) * TS + (l % ROWS)] = val; });
@@ -758,6 +811,30 @@ __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_bwd(FftAr
         for (int n1 = 0; n1 < A; ++n1) v[n1] = lds[(t + B * n1) * TS + r];
     }
     __syncthreads();
+    const uint64_t orow = ((uint64_t)(x0 + r) * NH + kz) * N;
+    if constexpr (SG == 2) {
+        // the -i k_y column's inverse to A; v keeps the plain column for the
+        // inverse to xout below (LDS holds the pass to two waves per SIMD, so
+        // the second held column costs no occupancy)
+        C u[TL::EMAX];
+        if (t < B) {
+#pragma unroll
+            for (int n1 = 0; n1 < A; ++n1) u[n1] = mul_mi(v[n1], a.kd[t + B * n1]);
+            dft_reg<A, +1>(u);
+            tl_twiddle_apply_split<A, +1>(u, wl, wh);
+#pragma unroll
+            for (int k1 = 0; k1 < A; ++k1) xr[k1 * (B + 1) + t] = u[k1];
+        }
+        __syncthreads();
+        if (t < A) {
+#pragma unroll
+            for (int n2 = 0; n2 < B; ++n2) u[n2] = xr[t * (B + 1) + n2];
+            dft_reg<B, +1>(u);
+#pragma unroll
+            for (int k2 = 0; k2 < B; ++k2) st_stream(&a.A[orow + t + A * k2], u[k2]);
+        }
+        __syncthreads();
+    }
     if (t < B) {
         dft_reg<A, +1>(v);
         tl_twiddle_apply_split<A, +1>(v, wl, wh);
@@ -769,7 +846,7 @@ __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_bwd(FftAr
 #pragma unroll
         for (int n2 = 0; n2 < B; ++n2) v[n2] = xr[t * (B + 1) + n2];
         dft_reg<B, +1>(v);
-        C* dst = a.A + ((uint64_t)(x0 + r) * NH + kz) * N;
+        C* dst = (SG == 2 ? a.xout : a.A) + orow;
 #pragma unroll
         for (int k2 = 0; k2 < B; ++k2) st_stream(&dst[t + A * k2], v[k2]);
     }
@@ -807,7 +884,9 @@ __global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_c2r(F
         if (k == 0 || k == LEN) val.y = (T)0;  // c2r reads only the independent half
         lds[rr * NAT + k] = val;
     };
-    if constexpr (sizeof(T) == 4 && !SG && ROWS % 2 == 0) {
+    if constexpr (sizeof(T) == 4 && ROWS % 2 == 0) {
+        // 16-byte loads of two adjacent y of one kz row (ROWS even: one -i k_z
+        // factor serves the pair)
         constexpr int PAIRS = ROWS * NH / 2, PER = (PAIRS + NT - 1) / NT;
         F4 q[PER];
 #pragma unroll
@@ -821,8 +900,14 @@ __global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_c2r(F
             const int m = tid + i * NT;
             const int l = 2 * m;
             if (PAIRS % NT == 0 || m < PAIRS) {
-                put(l, C{q[i].x, q[i].y});
-                put(l + 1, C{q[i].z, q[i].w});
+                C c0{q[i].x, q[i].y}, c1{q[i].z, q[i].w};
+                if (SG && a.deriv == 3) {
+                    const T kk = a.kd[l / ROWS];
+                    c0 = mul_mi(c0, kk);
+                    c1 = mul_mi(c1, kk);
+                }
+                put(l, c0);
+                put(l + 1, c1);
             }
         }
     } else {

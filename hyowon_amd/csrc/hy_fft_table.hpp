@@ -24,6 +24,8 @@ struct FftKernels {
     FftPass<T> x_fwd;  // forward-only x pass of the spectral estimators (in place on B)
     FftPass<T> x_inv;  // inverse-only x pass of the IC displacements (in place on B)
     FftPass<T> sg[3];  // spectral-gradient x, y^-1 and z^-1 passes (TreePM long range; variant 0)
+    bool sg_x_dual = false;  // sg[0] writes the -i k_x and the plain inverse from one read of B
+    FftPass<T> sg_y_dual;    // y^-1 pass writing the -i k_y and the plain inverse from one read of B (or none)
     int variants = 0;  // launch variants compiled for this size (variant 0 is the default)
 };
 
@@ -54,7 +56,14 @@ void fill_variant(FftKernels<T>& k) {
         if constexpr (V == 0) {
             k.x_fwd = {k_tl_x_dft<T, N, V, -1>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
             k.x_inv = {k_tl_x_dft<T, N, V, +1>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
-            k.sg[0] = {k_tl_x_green<T, N, V, 1>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
+            if constexpr (sizeof(T) == 4) {
+                // binary32: the second held column fits the register file
+                k.sg[0] = {k_tl_x_green_dual<T, N, V>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
+                k.sg_x_dual = true;
+                k.sg_y_dual = {k_tl_y_bwd<T, N, V, 2>, RY::ROWS, RY::THREADS, RY::LDS_BYTES};
+            } else {
+                k.sg[0] = {k_tl_x_green<T, N, V, 1>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
+            }
             k.sg[1] = {k_tl_y_bwd<T, N, V, 1>, RY::ROWS, RY::THREADS, RY::LDS_BYTES};
             k.sg[2] = {k_tl_z_c2r<T, N, V, 1>, RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES};
         }

@@ -71,19 +71,25 @@ def test_treepm_force_matches_oracle(gpu, oracle_mod, prec, tol, N, L, rs_cells,
             assert abs(a.sum()) < 1e-9 * np.abs(a).sum()
 
 
-def test_force_meshes_match_oracle(gpu, oracle_mod):
+# fp64 at N = 32 runs the LDS Stockham passes; fp32 at N = 128 runs the
+# two-level dual passes (both x outputs, and the -i k_y and plain y^-1 outputs,
+# from one read each), so each of F_x, F_y, F_z is checked on its own.
+@pytest.mark.parametrize("prec,N,L,tol", [("fp64", 32, 40.0, 1e-10), ("fp32", 128, 160.0, 1e-4)])
+def test_force_meshes_match_oracle(gpu, oracle_mod, prec, N, L, tol):
     import torch
-    N, L, n = 32, 40.0, 6000
-    pos = clumpy_particles(n, L, 3, sigma=1.0)
-    rs = 1.7
-    s = _solver(L, N, "fp64", n, rs)
+    n = 6000 if N == 32 else 40000
+    pos = clumpy_particles(n, L, 3, sigma=L / 40)
+    rs = 1.7 * L / 40
+    print(f"force meshes prec={prec} N={N} n={n}", flush=True)
+    s = _solver(L, N, prec, n, rs)
     x, y, z = (_dev(p) for p in pos)
     src = s.density(x, y, z, None, 1.3)
     fx, fy, fz = s.force_meshes(src)
     *_, meshes = oracle_mod.pm_force_treepm(*pos, L, N, rs, uniform_mass=1.3, meshes=True)
     for got, ref in zip((fx, fy, fz), meshes):
-        g = got.cpu().numpy()
-        assert np.abs(g - ref).max() < 1e-10 * np.abs(ref).max()
+        g = got.double().cpu().numpy()
+        err = np.abs(g - ref).max() / np.abs(ref).max()
+        assert err < tol, err
 
 
 def test_treepm_context_rules(gpu):

@@ -558,8 +558,9 @@ void run_fft(hy_pm_ctx* c) {
 //   x: B2 = X^-1(-i k_x G phi), B = X^-1(G phi)
 //   F_x = Z^-1 Y^-1 B2 -> R,  F_y = Z^-1 Y^-1(-i k_y B) -> R2,  F_z = Z^-1(-i k_z Y^-1 B) -> R3
 // The dual passes (binary32 two-level meshes) form both x outputs from one
-// read of B, and Y^-1(-i k_y B) -> A with Y^-1 B -> B2 (free once F_x is
-// done) from one read of B.
+// read of B, and Y^-1(-i k_y B) -> A with -i k_z Y^-1 B -> B2 (free once F_x
+// is done) from one read of B; all three z c2r passes then run without a
+// derivative.
 template <typename T>
 void launch_fft_sg(hy_pm_ctx* c, const hy::FftKernels<T>& K) {
     using C = hy::cplx<T>;
@@ -595,12 +596,11 @@ void launch_fft_sg(hy_pm_ctx* c, const hy::FftKernels<T>& K) {
     if (K.sg_y_dual.fn) {
         b = a;
         b.xout = static_cast<C*>(c->B2);
-        launch(K.sg_y_dual, 1, b);  // Y^-1(-i k_y B) -> A, Y^-1 B -> B2
+        launch(K.sg_y_dual, 1, b);  // Y^-1(-i k_y B) -> A, -i k_z Y^-1 B -> B2
         b.R = static_cast<T*>(c->R2);
         sg(2, b);  // F_y -> R2
         b = a;
         b.A = static_cast<C*>(c->B2);
-        b.deriv = 3;
         b.R = static_cast<T*>(c->R3);
         sg(2, b);  // F_z -> R3
         return;

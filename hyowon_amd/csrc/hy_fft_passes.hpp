@@ -751,8 +751,8 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
 
 // y pass backward: B[kz][ky][x] -> A rows over y through a transposing tile
 // (SG = 1: optional -i k_y on load; SG = 2: two outputs from one read of B,
-// the inverse of the -i k_y column to A and of the plain column to xout, in
-// the layout of A).
+// the inverse of the -i k_y column to A and -i k_z times the inverse of the
+// plain column to xout, in the layout of A; k_z is uniform over a workgroup).
 template <typename T, int N, int V, int SG = 0>
 __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_bwd(FftArgs<T> a) {
     fft_box(a);
@@ -846,6 +846,11 @@ __global__ void __launch_bounds__((TLCfg<T, N, N, V>::THREADS)) k_tl_y_bwd(FftAr
 #pragma unroll
         for (int n2 = 0; n2 < B; ++n2) v[n2] = xr[t * (B + 1) + n2];
         dft_reg<B, +1>(v);
+        if constexpr (SG == 2) {
+            const T kzf = a.kd[kz];
+#pragma unroll
+            for (int k2 = 0; k2 < B; ++k2) v[k2] = mul_mi(v[k2], kzf);
+        }
         C* dst = (SG == 2 ? a.xout : a.A) + orow;
 #pragma unroll
         for (int k2 = 0; k2 < B; ++k2) st_stream(&dst[t + A * k2], v[k2]);

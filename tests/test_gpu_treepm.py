@@ -74,10 +74,11 @@ def test_treepm_force_matches_oracle(gpu, oracle_mod, prec, tol, N, L, rs_cells,
 # fp64 at N = 32 runs the LDS Stockham passes; fp32 at N = 128 runs the
 # two-level dual passes (both x outputs, and the -i k_y and plain y^-1 outputs,
 # from one read each), so each of F_x, F_y, F_z is checked on its own.
-@pytest.mark.parametrize("prec,N,L,tol", [("fp64", 32, 40.0, 1e-10), ("fp32", 128, 160.0, 1e-4)])
+@pytest.mark.parametrize("prec,N,L,tol", [("fp64", 32, 40.0, 1e-10), ("fp32", 128, 160.0, 1e-4),
+                                          ("fp32", 256, 256.0, 1e-4)])
 def test_force_meshes_match_oracle(gpu, oracle_mod, prec, N, L, tol):
     import torch
-    n = 6000 if N == 32 else 40000
+    n = {32: 6000, 128: 40000}.get(N, 200000)
     pos = clumpy_particles(n, L, 3, sigma=L / 40)
     rs = 1.7 * L / 40
     print(f"force meshes prec={prec} N={N} n={n}", flush=True)
@@ -90,6 +91,32 @@ def test_force_meshes_match_oracle(gpu, oracle_mod, prec, N, L, tol):
         g = got.double().cpu().numpy()
         err = np.abs(g - ref).max() / np.abs(ref).max()
         assert err < tol, err
+
+
+def test_treepm_fp32_matches_fp64_at_1024(gpu):
+    """The benchmarked size, 1024^3: the binary32 dual passes (one read, two
+    outputs in x and in y^-1) against the binary64 single-output passes on
+    the same particles, two independent kernel paths."""
+    import torch
+    N, L, n = 1024, 512.0, 2_000_000
+    pos = clumpy_particles(n, L, 11, n_clumps=64, sigma=4.0)
+    rs = 1.25 * L / N
+    x, y, z = (_dev(p) for p in pos)
+    got = {}
+    for prec in ("fp32", "fp64"):
+        print(f"treepm 1024 prec={prec} n={n}", flush=True)
+        s = _solver(L, N, prec, n, rs)
+        acc = [torch.zeros(n, dtype=torch.float64, device="cuda:0") for _ in range(3)]
+        s.compute_forces(x, y, z, None, 1.0, *acc)
+        got[prec] = torch.stack(acc).cpu().numpy()
+        s.close()
+        del acc
+        torch.cuda.empty_cache()
+    ref = got["fp64"]
+    rms = np.sqrt((ref ** 2).sum(0).mean())
+    for axis in range(3):
+        err = np.sqrt(((got["fp32"][axis] - ref[axis]) ** 2).mean()) / rms
+        assert err < 1e-4, (axis, err)
 
 
 def test_treepm_context_rules(gpu):

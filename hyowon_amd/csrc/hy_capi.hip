@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <numeric>
 #include <limits>
 #include <numbers>
 #include <string>
@@ -1924,6 +1925,7 @@ void run_forward(hy_pm_ctx* c, void* dst) {
 struct PkLaunch {
     int waves, groups;
     size_t lds;
+    uint64_t row_step;
 };
 
 PkLaunch pk_launch(int N, int nb) {
@@ -1937,6 +1939,12 @@ PkLaunch pk_launch(int N, int nb) {
     g = std::min<uint64_t>(g, std::max<uint64_t>(1, (1u << 20) / (uint64_t)nb));
     l.groups = (int)g;
     l.lds = (size_t)l.waves * per_wave + ((size_t)nb + 1) * sizeof(double);
+    // the first step coprime to rows from rows / golden ratio up: k -> k step
+    // mod rows permutes the rows, and each contiguous block of k is spread
+    // evenly over the half spectrum
+    uint64_t step = std::max<uint64_t>(1, (uint64_t)((double)rows * 0.6180339887498949));
+    while (std::gcd(step, rows) != 1) ++step;
+    l.row_step = step;
     return l;
 }
 
@@ -2086,6 +2094,7 @@ int hy_pm_power_spectrum(hy_pm_ctx* c, uint64_t n, const double* x, const double
         }
         a.shot = shot;
         a.shift = shift;
+        a.row_step = l.row_step;
         double* sums = c->pk_part + (uint64_t)l.groups * slab;
         if (c->prec == HY_PREC_FP64)
             hipLaunchKernelGGL(hy::k_pk_bin<double>, dim3(l.groups), dim3(hy::PK_BLOCK), l.lds, c->stream, a);

@@ -78,6 +78,7 @@ struct PkArgs {
     double volume;        // L^3
     double shot;          // shot-noise reference
     double shift;         // interlacing shift (dx / 2)
+    uint64_t row_step;    // visiting order q = (k row_step) mod rows, row_step coprime to rows
 };
 
 template <typename T>
@@ -95,11 +96,22 @@ __global__ void __launch_bounds__(PK_BLOCK) k_pk_bin(PkArgs a) {
     const int N = a.N, NH = N / 2 + 1;
     const bool even = (N & 1) == 0;
     const uint64_t rows = (uint64_t)NH * (uint64_t)N;
-    const uint64_t stride = (uint64_t)gridDim.x * (uint64_t)a.waves;
+    const uint64_t all_waves = (uint64_t)gridDim.x * (uint64_t)a.waves;
     bool bad = false;
     if (wave < a.waves) {
         double* acc = pk_lds + (size_t)wave * slab;
-        for (uint64_t q = (uint64_t)blockIdx.x * a.waves + wave; q < rows; q += stride) {
+        // The binned rows form a disk of small |k_perp|.  Strided over the
+        // rows in storage order (a stride that is a multiple of N), a wave
+        // would see one ky index in every row it owns: the waves on the disk's
+        // ky would bin every mode and the rest none.  Each wave takes a
+        // contiguous block of the permuted order q = (k row_step) mod rows
+        // instead, whose consecutive rows lie row_step (about rows / golden
+        // ratio) apart: every wave samples the whole half spectrum.
+        const uint64_t per = (rows + all_waves - 1) / all_waves;
+        const uint64_t k0 = ((uint64_t)blockIdx.x * a.waves + wave) * per;
+        const uint64_t k1 = k0 + per < rows ? k0 + per : rows;
+        uint64_t q = k0 < rows ? (k0 * a.row_step) % rows : 0;
+        for (uint64_t k = k0; k < k1; ++k, q = q + a.row_step >= rows ? q + a.row_step - rows : q + a.row_step) {
             const int iz = (int)(q / (uint64_t)N), iy = (int)(q % (uint64_t)N);
             const double ky = a.kc[iy], kz = a.kc[iz];
             if (hypot_corrected(ky, kz) > a.perp_skip) continue;  // uniform over the wave

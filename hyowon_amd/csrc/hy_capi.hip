@@ -1900,24 +1900,39 @@ int hy_pm_density_vjp(hy_pm_ctx* c, uint64_t n, const double* x, const double* y
 // ---------------------------------------------------------------------------
 namespace {
 
-// r2c transform of the mesh in R into dst ([kz][ky][kx] after the forward x pass).
+// r2c transform of the mesh in R into dst ([kz][ky][kx] after the forward x
+// pass), for modes with |m_y|, m_z <= kmode only (kmode = N/2: all of them).
+// The z pass stores the planes kz <= kmode (two-level passes), the y pass runs
+// over those planes, the x pass over their rows with |m_y| <= kmode (whole
+// workgroups of rows outside exit at once); every mode formed gets the same
+// bits as in the full transform.
 template <typename T>
-void launch_forward(hy_pm_ctx* c, const hy::FftKernels<T>& K, void* dst) {
+void launch_forward(hy_pm_ctx* c, const hy::FftKernels<T>& K, void* dst, int kmode) {
     hy::FftArgs<T> a = fft_args<T>(c);
     a.B = static_cast<hy::cplx<T>*>(dst);
     const int NH = c->N / 2 + 1;
-    launch_pass(c, K, a, hy::FFT_Z_R2C, c->fft_var[hy::FFT_Z_R2C], 0, NH);
-    launch_pass(c, K, a, hy::FFT_Y_FWD, c->fft_var[hy::FFT_Y_FWD], 0, NH);
+    const int planes = std::min(NH, kmode + 1);
+    a.mode_limit = kmode < c->N / 2 ? kmode : 0;
+    if (a.mode_limit) {
+        const hy::FftPass<T>& Z = K.z_fwd;
+        const uint64_t N = (uint64_t)c->N;
+        a.kz0 = 0;
+        hipLaunchKernelGGL(Z.fn, dim3((uint32_t)(N * (N / Z.rows)), 1), dim3(Z.threads), Z.lds, c->stream, a);
+        launch_check("k_fft_z_r2c (pruned)");
+    } else {
+        launch_pass(c, K, a, hy::FFT_Z_R2C, c->fft_var[hy::FFT_Z_R2C], 0, NH);
+    }
+    launch_pass(c, K, a, hy::FFT_Y_FWD, c->fft_var[hy::FFT_Y_FWD], 0, planes);
     const hy::FftPass<T>& P = K.x_fwd;
     a.kz0 = 0;
-    const uint64_t blocks = (uint64_t)NH * (uint64_t)c->N / (uint64_t)P.rows;
+    const uint64_t blocks = (uint64_t)planes * (uint64_t)c->N / (uint64_t)P.rows;
     hipLaunchKernelGGL(P.fn, dim3((uint32_t)blocks, 1), dim3(P.threads), P.lds, c->stream, a);
     launch_check("k_fft_x_fwd");
 }
 
-void run_forward(hy_pm_ctx* c, void* dst) {
-    if (c->prec == HY_PREC_FP64) launch_forward<double>(c, c->fft64, dst);
-    else launch_forward<float>(c, c->fft32, dst);
+void run_forward(hy_pm_ctx* c, void* dst, int kmode) {
+    if (c->prec == HY_PREC_FP64) launch_forward<double>(c, c->fft64, dst, kmode);
+    else launch_forward<float>(c, c->fft32, dst, kmode);
 }
 
 // Workgroups of the shell-binning pass: a function of the mesh and the shell
@@ -2022,15 +2037,19 @@ int hy_pm_power_spectrum(hy_pm_ctx* c, uint64_t n, const double* x, const double
                 throw;
             }
         };
-        // contrast transforms (src/analysis/fourier_density_field.cpp:230-336)
+        // contrast transforms (src/analysis/fourier_density_field.cpp:230-336).
+        // The shells read modes with |k| <= frac k_Nyq only, i.e. |m| <= frac N/2
+        // per axis; kmode bounds that with one mode to spare, so the rows the
+        // transform leaves out are never read.
+        const int kmode = std::min(N / 2, (int)(frac * (double)(N / 2) * (1.0 + 1e-9)) + 1);
         deposit(0.0);
-        run_forward(c, c->B);
+        run_forward(c, c->B, kmode);
         const double shift = 0.5 * dx;
         if (interlaced) {
             // the partner positions wrap(x - dx/2) are formed inside the routing
             // and deposit kernels (src/analysis/fourier_density_field.cpp:300-311)
             deposit(shift);
-            run_forward(c, c->pk_aux);
+            run_forward(c, c->pk_aux, kmode);
         }
 
         // shells (src/analysis/matter_power_spectrum.cpp:85-125)

@@ -38,7 +38,17 @@ struct FftArgs {
     cplx<T>* xout;       // x pass output (nullptr: in place on B)
     double rs;           // Gaussian split scale r_s
     int deriv;           // 0, or 1 / 2 / 3: multiply by -i k_x / k_y / k_z in the x / y^-1 / z^-1 pass
+    int mode_limit;      // spectral estimators: modes with |m_y|, m_z <= mode_limit only (0: all).
+                         // The pruned two-level z r2c pass stores the planes kz <= mode_limit;
+                         // the forward-only x pass skips workgroups whose rows all have
+                         // |m_y| > mode_limit.
 };
+
+// Forward-only x pass pruning: true when every row ky0 .. ky0 + rows - 1 of a
+// workgroup has |m_y| > limit (ky index in (limit, N - limit)).
+__device__ __forceinline__ bool x_rows_pruned(int limit, int N, int ky0, int rows) {
+    return limit > 0 && ky0 > limit && ky0 + rows - 1 < N - limit;
+}
 
 // Streaming store of a complex value (non-temporal).  Used for the passes'
 // contiguous row outputs, which nothing re-reads before the next pass; the
@@ -312,6 +322,7 @@ __global__ void __launch_bounds__((RowsCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) 
     C* lds = reinterpret_cast<C*>(smem_raw);
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const uint64_t q0 = (uint64_t)a.kz0 * N + (uint64_t)b * ROWS;
+    if (x_rows_pruned(a.mode_limit, N, (int)(q0 % N), ROWS)) return;
     const int tid = threadIdx.x;
     C* rows = a.B + q0 * N;
     staged_copy<ROWS * N, NT, C>(
@@ -423,7 +434,7 @@ __global__ void __launch_bounds__((RowsCfg<T, N / 2, N, V>::THREADS)) k_fft_z_c2
 // ===========================================================================
 
 // z pass: real rows (as LEN = N/2 complex) -> half spectrum, transposed into A.
-template <typename T, int N, int V>
+template <typename T, int N, int V, bool PRUNE = false>
 __global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_r2c(FftArgs<T> a) {
     fft_box(a);
     using C = cplx<T>;
@@ -477,21 +488,24 @@ __global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_r2c(F
         return cadd(e, cmul(w, o));
     };
     C* dst = a.A + (uint64_t)x * NH * N + y0;
+    // PRUNE (spectral estimators): only the planes kz <= mode_limit are stored
+    const int kz_cap = PRUNE && a.mode_limit > 0 && a.mode_limit < NH ? a.mode_limit + 1 : NH;
     if constexpr (sizeof(T) == 4 && ROWS % 2 == 0) {
         // pairs (k, rr) and (k, rr + 1), rr even: one twiddle, one 16-byte store
         constexpr int PAIRS = ROWS * NH / 2, PER = (PAIRS + NT - 1) / NT;
+        const int pair_cap = kz_cap * (ROWS / 2);
         C wk[PER];
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int m = tid + i * NT;
-            if (PAIRS % NT == 0 || m < PAIRS) wk[i] = a.tw[(2 * m) / ROWS];
+            if ((PRUNE || PAIRS % NT == 0 || m < PAIRS) && (!PRUNE || m < pair_cap)) wk[i] = a.tw[(2 * m) / ROWS];
         }
         __syncthreads();
         F4 q[PER];
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int m = tid + i * NT, l = 2 * m;
-            if (PAIRS % NT == 0 || m < PAIRS) {
+            if ((PRUNE || PAIRS % NT == 0 || m < PAIRS) && (!PRUNE || m < pair_cap)) {
                 const C c0 = untangle(l, wk[i]), c1 = untangle(l + 1, wk[i]);
                 q[i] = F4{c0.x, c0.y, c1.x, c1.y};
             }
@@ -499,28 +513,29 @@ __global__ void __launch_bounds__((TLCfg<T, N / 2, N, V>::THREADS)) k_tl_z_r2c(F
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int m = tid + i * NT, l = 2 * m;
-            if (PAIRS % NT == 0 || m < PAIRS)
+            if ((PRUNE || PAIRS % NT == 0 || m < PAIRS) && (!PRUNE || m < pair_cap))
                 *reinterpret_cast<F4*>(&dst[(uint64_t)(l / ROWS) * N + (l % ROWS)]) = q[i];
         }
     } else {
         constexpr int COUNT = ROWS * NH, PER = (COUNT + NT - 1) / NT;
+        const int cap = kz_cap * ROWS;
         C wk[PER];
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int l = tid + i * NT;
-            if (COUNT % NT == 0 || l < COUNT) wk[i] = a.tw[l / ROWS];
+            if ((PRUNE || COUNT % NT == 0 || l < COUNT) && (!PRUNE || l < cap)) wk[i] = a.tw[l / ROWS];
         }
         __syncthreads();
         C out[PER];
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int l = tid + i * NT;
-            if (COUNT % NT == 0 || l < COUNT) out[i] = untangle(l, wk[i]);
+            if ((PRUNE || COUNT % NT == 0 || l < COUNT) && (!PRUNE || l < cap)) out[i] = untangle(l, wk[i]);
         }
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int l = tid + i * NT;
-            if (COUNT % NT == 0 || l < COUNT) dst[(uint64_t)(l / ROWS) * N + (l % ROWS)] = out[i];
+            if ((PRUNE || COUNT % NT == 0 || l < COUNT) && (!PRUNE || l < cap)) dst[(uint64_t)(l / ROWS) * N + (l % ROWS)] = out[i];
         }
     }
 }
@@ -725,6 +740,7 @@ __global__ void __launch_bounds__((TLCfg<T, N, (N / 2 + 1) * N, V>::THREADS)) k_
     C* xr = lds + r * XS;
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const uint64_t q = (uint64_t)a.kz0 * N + (uint64_t)b * ROWS + r;
+    if (x_rows_pruned(a.mode_limit, N, (int)((q - r) % N), ROWS)) return;
     C* row = a.B + q * N;
     C v[TL::EMAX], w[A];
     if (t < B) {

@@ -22,6 +22,7 @@ template <typename T>
 struct FftKernels {
     FftPass<T> pass[FFT_PASSES][kFftVariants];
     FftPass<T> x_fwd;  // forward-only x pass of the spectral estimators (in place on B)
+    FftPass<T> z_fwd;  // z r2c pass of the spectral estimators (stores kz <= mode_limit only)
     FftPass<T> x_inv;  // inverse-only x pass of the IC displacements (in place on B)
     FftPass<T> sg[3];  // spectral-gradient x, y^-1 and z^-1 passes (TreePM long range; variant 0)
     bool sg_x_dual = false;  // sg[0] writes the -i k_x and the plain inverse from one read of B
@@ -55,6 +56,7 @@ void fill_variant(FftKernels<T>& k) {
         k.pass[FFT_Z_C2R][V] = {k_tl_z_c2r<T, N, V>, RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES};
         if constexpr (V == 0) {
             k.x_fwd = {k_tl_x_dft<T, N, V, -1>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
+            k.z_fwd = {k_tl_z_r2c<T, N, V, true>, RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES};
             k.x_inv = {k_tl_x_dft<T, N, V, +1>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
             if constexpr (sizeof(T) == 4) {
                 // binary32: the second held column fits the register file
@@ -78,6 +80,7 @@ void fill_variant(FftKernels<T>& k) {
         k.pass[FFT_Z_C2R][V] = {k_fft_z_c2r<T, N, V>, RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES};
         if constexpr (V == 0) {
             k.x_fwd = {k_fft_x_dft<T, N, V, -1>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
+            k.z_fwd = k.pass[FFT_Z_R2C][V];  // stores every plane
             k.x_inv = {k_fft_x_dft<T, N, V, +1>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
             k.sg[0] = {k_fft_x_green<T, N, V, 1>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
             k.sg[1] = {k_fft_y_bwd<T, N, V, 1>, RY::ROWS, RY::THREADS, RY::LDS_BYTES};

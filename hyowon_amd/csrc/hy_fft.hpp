@@ -446,6 +446,8 @@ struct TLCfg {
                                      ? (ROWS * TL::XS > ROWS * NAT ? ROWS * TL::XS : ROWS * NAT)
                                      : LEN * TS;
     static constexpr int LDS_BYTES = ELEMS * (int)sizeof(C);
+    // the x passes use only the per-row exchange regions (no transposed tile)
+    static constexpr int X_LDS_BYTES = ROWS * TL::XS * (int)sizeof(C);
     static_assert(LDS_BYTES <= 160 * 1024, "LDS per workgroup");
 };
 

@@ -51,20 +51,20 @@ void fill_variant(FftKernels<T>& k) {
         using RX = TLCfg<T, N, (N / 2 + 1) * N, V>;
         k.pass[FFT_Z_R2C][V] = {k_tl_z_r2c<T, N, V>, RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES};
         k.pass[FFT_Y_FWD][V] = {k_tl_y_fwd<T, N, V>, RY::ROWS, RY::THREADS, RY::LDS_BYTES};
-        k.pass[FFT_X_GREEN][V] = {k_tl_x_green<T, N, V>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
+        k.pass[FFT_X_GREEN][V] = {k_tl_x_green<T, N, V>, RX::ROWS, RX::THREADS, RX::X_LDS_BYTES};
         k.pass[FFT_Y_BWD][V] = {k_tl_y_bwd<T, N, V>, RY::ROWS, RY::THREADS, RY::LDS_BYTES};
         k.pass[FFT_Z_C2R][V] = {k_tl_z_c2r<T, N, V>, RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES};
         if constexpr (V == 0) {
-            k.x_fwd = {k_tl_x_dft<T, N, V, -1>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
+            k.x_fwd = {k_tl_x_dft<T, N, V, -1>, RX::ROWS, RX::THREADS, RX::X_LDS_BYTES};
             k.z_fwd = {k_tl_z_r2c<T, N, V, true>, RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES};
-            k.x_inv = {k_tl_x_dft<T, N, V, +1>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
+            k.x_inv = {k_tl_x_dft<T, N, V, +1>, RX::ROWS, RX::THREADS, RX::X_LDS_BYTES};
             if constexpr (sizeof(T) == 4) {
                 // binary32: the second held column fits the register file
-                k.sg[0] = {k_tl_x_green_dual<T, N, V>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
+                k.sg[0] = {k_tl_x_green_dual<T, N, V>, RX::ROWS, RX::THREADS, RX::X_LDS_BYTES};
                 k.sg_x_dual = true;
                 k.sg_y_dual = {k_tl_y_bwd<T, N, V, 2>, RY::ROWS, RY::THREADS, RY::LDS_BYTES};
             } else {
-                k.sg[0] = {k_tl_x_green<T, N, V, 1>, RX::ROWS, RX::THREADS, RX::LDS_BYTES};
+                k.sg[0] = {k_tl_x_green<T, N, V, 1>, RX::ROWS, RX::THREADS, RX::X_LDS_BYTES};
             }
             k.sg[1] = {k_tl_y_bwd<T, N, V, 1>, RY::ROWS, RY::THREADS, RY::LDS_BYTES};
             k.sg[2] = {k_tl_z_c2r<T, N, V, 1>, RZ::ROWS, RZ::THREADS, RZ::LDS_BYTES};

@@ -738,8 +738,8 @@ def main():
     args.precisions = [p for p in args.precisions.split(",") if p]
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the slab decomposition is described in DESIGN.md); run with --gpus 1",
+        print("bench.py measures one GPU: the library has no multi-GPU path "
+              "(a slab decomposition over GPUs is not built); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 
